@@ -1,0 +1,112 @@
+// Internal helpers shared by the HIP translation units of libarmsparse.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstring>
+
+#include "armsp.h"
+
+namespace armsp {
+
+constexpr int kWave = 64;
+
+// ------------------------------------------------------------------ runtime
+// Records the HIP error for armsp_last_hip_error() and maps it to a status.
+int hip_fail(hipError_t e);
+#define ARMSP_HIP(call)                                 \
+    do {                                                \
+        hipError_t e_ = (call);                         \
+        if (e_ != hipSuccess) return ::armsp::hip_fail(e_); \
+    } while (0)
+#define ARMSP_LAUNCHED()                                         \
+    do {                                                         \
+        hipError_t e_ = hipGetLastError();                       \
+        if (e_ != hipSuccess) return ::armsp::hip_fail(e_);      \
+    } while (0)
+
+inline hipStream_t as_stream(void* s) { return reinterpret_cast<hipStream_t>(s); }
+
+// Stream-ordered workspace from the device's default memory pool.  The pool's
+// release threshold is raised at first use so freed blocks stay cached.
+int ws_alloc(void** p, size_t bytes, hipStream_t s);
+void ws_free(void* p, hipStream_t s);
+
+// Optional per-launch timing (armsp_timing_enable).  `TimedLaunch t(family, s)`
+// records a start event; t.stop() records the stop event.
+bool timing_on();
+struct TimedLaunch {
+    void* rec = nullptr;
+    hipStream_t stream_ = nullptr;
+    TimedLaunch(const char* family, hipStream_t s);
+    void stop();
+};
+
+// Number of compute units on the current device (cached).
+int device_cus();
+
+// ------------------------------------------------------------ device utils
+__device__ __forceinline__ int wave_lane() { return __lane_id(); }
+
+__device__ __forceinline__ int uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
+
+__device__ __forceinline__ int read_lane(int v, int lane) { return __builtin_amdgcn_readlane(v, lane); }
+__device__ __forceinline__ float read_lane(float v, int lane) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
+}
+
+__device__ __forceinline__ float bf16_to_f32(uint16_t h) { return __uint_as_float(((uint32_t)h) << 16); }
+
+__device__ __forceinline__ uint16_t f32_to_bf16_rne(float f) {
+    uint32_t u = __float_as_uint(f);
+    if ((u & 0x7f800000u) == 0x7f800000u && (u & 0x007fffffu)) return (uint16_t)((u >> 16) | 0x40);  // quiet NaN
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return (uint16_t)(u >> 16);
+}
+
+// OCP fp8 e4m3 (e4m3fn: bias 7, no infinities, 0x7f/0xff = NaN) → fp32,
+// built from integer fields so it is exact for normals and subnormals.
+__device__ __forceinline__ float e4m3_to_f32(uint32_t b) {
+    uint32_t sign = (b & 0x80u) << 24;
+    uint32_t e = (b >> 3) & 0xfu, m = b & 7u;
+    float mag;
+    if (e == 0) {
+        mag = (float)m * 0x1p-9f;
+    } else if (e == 15 && m == 7) {
+        mag = __uint_as_float(0x7fc00000u);
+    } else {
+        mag = __uint_as_float(((e + 120u) << 23) | (m << 20));
+    }
+    return __uint_as_float(__float_as_uint(mag) | sign);
+}
+
+// Merge-path search: for diagonal d over (row ends rowptr[1..M], nnz 0..nnz-1)
+// returns the number of rows consumed; nnz consumed is d - result.
+__device__ __forceinline__ int merge_path_search(int d, const int* __restrict__ rowptr, int M, int nnz) {
+    int lo = d - nnz > 0 ? d - nnz : 0;
+    int hi = d < M ? d : M;
+    while (lo < hi) {
+        int mid = (lo + hi) >> 1;
+        if (rowptr[mid + 1] <= d - mid - 1)
+            lo = mid + 1;
+        else
+            hi = mid;
+    }
+    return lo;
+}
+
+// Number of rows whose end is ≤ j (rows fully before nonzero j).
+__device__ __forceinline__ int rows_ending_by(int j, const int* __restrict__ rowptr, int M) {
+    int lo = 0, hi = M;
+    while (lo < hi) {
+        int mid = (lo + hi) >> 1;
+        if (rowptr[mid + 1] <= j)
+            lo = mid + 1;
+        else
+            hi = mid;
+    }
+    return lo;
+}
+
+}  // namespace armsp

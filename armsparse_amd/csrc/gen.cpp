@@ -1,0 +1,315 @@
+// Seeded CSR-native generators and host format helpers (pure host C++).
+// The reference's own generators (prepareSparse + toCSR, SDMM.cpp:329-370;
+// SpMM.cpp:48-79; SpMV.cpp:39-50) are time-seeded and dense-first; these are
+// counter-based and CSR-native so graph-scale inputs are reproducible.
+#include "armsp_gen.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+namespace {
+
+constexpr uint64_t kGolden = 0x9E3779B97F4A7C15ULL;
+
+inline uint64_t mix64(uint64_t z) {
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+    return z ^ (z >> 31);
+}
+
+inline uint64_t stream_key(uint64_t seed, uint64_t stream) { return mix64(seed + kGolden * (stream + 1)); }
+inline uint64_t draw(uint64_t key, uint64_t i) { return mix64(key + kGolden * (i + 1)); }
+
+// (r>>40)·2^-24 in [0,1); lo + (hi-lo)·u is exact for (lo,hi) = (-1,1).
+inline float to_uniform(uint64_t r, float lo, float hi) {
+    float u = (float)(uint32_t)(r >> 40) * 0x1p-24f;
+    return lo + (hi - lo) * u;
+}
+
+inline float nonzero_value(uint64_t key, uint64_t i) {
+    float v = to_uniform(draw(key, i), -1.0f, 1.0f);
+    return v == 0.0f ? 0x1p-24f : v;
+}
+
+// Open-addressing set of 64-bit keys (UINT64_MAX reserved as "empty").
+struct KeySet {
+    std::vector<uint64_t> slots;
+    uint64_t mask;
+    explicit KeySet(int64_t expected) {
+        uint64_t cap = 1024;
+        while (cap < (uint64_t)expected * 2) cap <<= 1;
+        slots.assign(cap, ~0ULL);
+        mask = cap - 1;
+    }
+    // true if newly inserted
+    bool insert(uint64_t k) {
+        uint64_t h = mix64(k) & mask;
+        while (true) {
+            uint64_t s = slots[h];
+            if (s == ~0ULL) { slots[h] = k; return true; }
+            if (s == k) return false;
+            h = (h + 1) & mask;
+        }
+    }
+};
+
+// Sorted unique keys (row<<32 | col) → canonical CSR.
+void keys_to_csr(std::vector<uint64_t>& keys, int M, uint64_t vkey, int* rowptr, int* colind, float* val) {
+    std::sort(keys.begin(), keys.end());
+    std::fill(rowptr, rowptr + M + 1, 0);
+    for (uint64_t k : keys) rowptr[(k >> 32) + 1]++;
+    for (int r = 0; r < M; ++r) rowptr[r + 1] += rowptr[r];
+    for (size_t i = 0; i < keys.size(); ++i) {
+        colind[i] = (int)(keys[i] & 0xffffffffULL);
+        if (val) val[i] = nonzero_value(vkey, i);
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+uint64_t armsp_rng_u64(uint64_t seed, uint64_t stream, uint64_t i) { return draw(stream_key(seed, stream), i); }
+
+float armsp_rng_uniform(uint64_t seed, uint64_t stream, uint64_t i, float lo, float hi) {
+    return to_uniform(armsp_rng_u64(seed, stream, i), lo, hi);
+}
+
+int armsp_host_fill_uniform(int64_t n, uint64_t seed, uint64_t stream, float lo, float hi, float* out) {
+    if (n < 0 || (n > 0 && !out)) return 1;
+    const uint64_t key = stream_key(seed, stream);
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < n; ++i) out[i] = to_uniform(draw(key, (uint64_t)i), lo, hi);
+    return 0;
+}
+
+int armsp_host_fill_normal(int64_t n, uint64_t seed, uint64_t stream, float* out) {
+    if (n < 0 || (n > 0 && !out)) return 1;
+    const uint64_t key = stream_key(seed, stream);
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < n; ++i) {
+        double u1 = ((double)(draw(key, 2 * (uint64_t)i) >> 11) + 1.0) * 0x1p-53;  // (0,1]
+        double u2 = (double)(draw(key, 2 * (uint64_t)i + 1) >> 11) * 0x1p-53;      // [0,1)
+        out[i] = (float)(std::sqrt(-2.0 * std::log(u1)) * std::cos(2.0 * M_PI * u2));
+    }
+    return 0;
+}
+
+int armsp_gen_uniform_csr(int M, int K, int64_t nnz, uint64_t seed, int* rowptr, int* colind, float* val) {
+    if (M <= 0 || K <= 0 || nnz < 0 || !rowptr || (nnz > 0 && !colind)) return 1;
+    const uint64_t cells = (uint64_t)M * (uint64_t)K;
+    if ((uint64_t)nnz > cells) return 1;
+    const uint64_t pkey = stream_key(seed, ARMSP_STREAM_POS);
+    KeySet set(nnz);
+    std::vector<uint64_t> keys;
+    keys.reserve(nnz);
+    for (uint64_t i = 0; (int64_t)keys.size() < nnz; ++i) {
+        uint64_t p = draw(pkey, i) % cells;
+        uint64_t k = ((p / K) << 32) | (p % K);
+        if (set.insert(k)) keys.push_back(k);
+    }
+    keys_to_csr(keys, M, stream_key(seed, ARMSP_STREAM_VAL), rowptr, colind, val);
+    return 0;
+}
+
+int armsp_gen_rmat_csr(int scale, int64_t nnz, double a, double b, double c, uint64_t seed, int* rowptr,
+                       int* colind, float* val) {
+    if (scale < 1 || scale > 30 || nnz < 0 || !rowptr || (nnz > 0 && !colind)) return 1;
+    if (a < 0 || b < 0 || c < 0 || a + b + c > 1.0) return 1;
+    const int M = 1 << scale;
+    if ((uint64_t)nnz > (uint64_t)M * (uint64_t)M / 2) return 1;
+    // Quadrant thresholds on 16-bit draws.
+    const uint32_t ta = (uint32_t)std::lround(a * 65536.0);
+    const uint32_t tb = (uint32_t)std::lround((a + b) * 65536.0);
+    const uint32_t tc = (uint32_t)std::lround((a + b + c) * 65536.0);
+    const uint64_t rkey = stream_key(seed, ARMSP_STREAM_RMAT);
+    KeySet set(nnz);
+    std::vector<uint64_t> keys;
+    keys.reserve(nnz);
+    for (uint64_t e = 0; (int64_t)keys.size() < nnz; ++e) {
+        uint64_t row = 0, col = 0, q = 0;
+        for (int t = 0; t < scale; ++t) {
+            if ((t & 3) == 0) q = draw(rkey, e * 8 + (t >> 2));
+            uint32_t u = (uint32_t)((q >> (16 * (t & 3))) & 0xffff);
+            uint64_t rb = (u >= tb) ? 1 : 0;                       // quadrants c, d
+            uint64_t cb = (u >= ta && u < tb) || (u >= tc) ? 1 : 0;  // quadrants b, d
+            row = (row << 1) | rb;
+            col = (col << 1) | cb;
+        }
+        uint64_t k = (row << 32) | col;
+        if (set.insert(k)) keys.push_back(k);
+    }
+    keys_to_csr(keys, M, stream_key(seed, ARMSP_STREAM_VAL), rowptr, colind, val);
+    return 0;
+}
+
+int armsp_gen_powerlaw_graph(int n, int64_t pairs, double gamma, uint64_t seed, int* rowptr, int* colind) {
+    if (n < 2 || pairs < 0 || gamma <= 1.0 || !rowptr || (pairs > 0 && !colind)) return 1;
+    if ((uint64_t)pairs > (uint64_t)n * (uint64_t)(n - 1) / 4) return 1;
+    std::vector<double> cdf(n);
+    double acc = 0.0, expo = -1.0 / (gamma - 1.0);
+    for (int i = 0; i < n; ++i) {
+        acc += std::pow((double)(i + 1), expo);
+        cdf[i] = acc;
+    }
+    for (int i = 0; i < n; ++i) cdf[i] /= acc;
+    auto pick = [&](uint64_t r) {
+        double u = (double)(r >> 11) * 0x1p-53;
+        int v = (int)(std::upper_bound(cdf.begin(), cdf.end(), u) - cdf.begin());
+        return v < n ? v : n - 1;
+    };
+    const uint64_t gkey = stream_key(seed, ARMSP_STREAM_GRAPH);
+    KeySet set(pairs);
+    std::vector<uint64_t> keys;
+    keys.reserve(2 * pairs);
+    int64_t found = 0;
+    for (uint64_t i = 0; found < pairs; ++i) {
+        int u = pick(draw(gkey, 2 * i)), v = pick(draw(gkey, 2 * i + 1));
+        if (u == v) continue;
+        uint64_t lo = (uint64_t)std::min(u, v), hi = (uint64_t)std::max(u, v);
+        if (set.insert((lo << 32) | hi)) {
+            keys.push_back((lo << 32) | hi);
+            keys.push_back((hi << 32) | lo);
+            ++found;
+        }
+    }
+    keys_to_csr(keys, n, 0, rowptr, colind, nullptr);
+    return 0;
+}
+
+int64_t armsp_gen_corpus_csr(int kind, int M, double avg_row, double param, uint64_t seed, int64_t cap, int* rowptr,
+                             int* colind, float* val) {
+    if (M <= 0 || !rowptr || !colind || avg_row < 0) return -1;
+    const uint64_t key = stream_key(seed, 100 + (uint64_t)kind);
+    const uint64_t vkey = stream_key(seed, ARMSP_STREAM_VAL);
+    std::vector<std::vector<int>> rows(M);
+    if (kind == 1) {
+        int scale = 0;
+        while ((1 << scale) < M) ++scale;
+        if ((1 << scale) != M) return -1;
+        double a = param, b = 0.4 * (1.0 - a), c = b;
+        const uint32_t ta = (uint32_t)std::lround(a * 65536.0);
+        const uint32_t tb = (uint32_t)std::lround((a + b) * 65536.0);
+        const uint32_t tc = (uint32_t)std::lround((a + b + c) * 65536.0);
+        const int64_t E = (int64_t)std::llround(avg_row * M);
+        std::vector<uint64_t> keys(E);
+#pragma omp parallel for schedule(static)
+        for (int64_t e = 0; e < E; ++e) {
+            uint64_t row = 0, col = 0, q = 0;
+            for (int t = 0; t < scale; ++t) {
+                if ((t & 3) == 0) q = draw(key, (uint64_t)e * 8 + (t >> 2));
+                uint32_t u = (uint32_t)((q >> (16 * (t & 3))) & 0xffff);
+                uint64_t rb = (u >= tb) ? 1 : 0;
+                uint64_t cb = (u >= ta && u < tb) || (u >= tc) ? 1 : 0;
+                row = (row << 1) | rb;
+                col = (col << 1) | cb;
+            }
+            keys[e] = (row << 32) | col;
+        }
+        std::sort(keys.begin(), keys.end());
+        keys.erase(std::unique(keys.begin(), keys.end()), keys.end());
+        if ((int64_t)keys.size() > cap) return -1;
+        keys_to_csr(keys, M, vkey, rowptr, colind, val);
+        return (int64_t)keys.size();
+    }
+#pragma omp parallel for schedule(dynamic, 256)
+    for (int r = 0; r < M; ++r) {
+        std::vector<int>& out = rows[r];
+        const uint64_t base = (uint64_t)r * 4096;
+        if (kind == 3) {
+            const int nb = std::max(1, (int)param), nblk = (M + 31) / 32, br = r / 32;
+            std::vector<int> bc{br};
+            for (int t = 1; t < nb; ++t) bc.push_back((int)(draw(key, (uint64_t)br * 4096 + t) % (uint64_t)nblk));
+            std::sort(bc.begin(), bc.end());
+            bc.erase(std::unique(bc.begin(), bc.end()), bc.end());
+            for (int blk : bc)
+                for (int t = 0; t < 32 && blk * 32 + t < M; ++t) out.push_back(blk * 32 + t);
+            continue;
+        }
+        double fl = std::floor(avg_row);
+        int d = (int)fl + ((double)(draw(key, base) >> 11) * 0x1p-53 < avg_row - fl ? 1 : 0);
+        int lo = 0, hi = M;
+        if (kind == 2) {
+            int bw = std::max(1, (int)param);
+            lo = std::max(0, r - bw);
+            hi = std::min(M, r + bw + 1);
+        }
+        const int width = hi - lo;
+        if (d >= width) {
+            for (int cidx = lo; cidx < hi; ++cidx) out.push_back(cidx);
+            continue;
+        }
+        for (int t = 0; t < d; ++t) out.push_back(lo + (int)(draw(key, base + 1 + t) % (uint64_t)width));
+        std::sort(out.begin(), out.end());
+        out.erase(std::unique(out.begin(), out.end()), out.end());
+    }
+    int64_t total = 0;
+    rowptr[0] = 0;
+    for (int r = 0; r < M; ++r) {
+        total += (int64_t)rows[r].size();
+        if (total > cap || total > 0x7fffffff) return -1;
+        rowptr[r + 1] = (int)total;
+    }
+#pragma omp parallel for schedule(dynamic, 256)
+    for (int r = 0; r < M; ++r) {
+        int64_t p = rowptr[r];
+        for (int cidx : rows[r]) {
+            colind[p] = cidx;
+            if (val) val[p] = nonzero_value(vkey, (uint64_t)p);
+            ++p;
+        }
+    }
+    return total;
+}
+
+int64_t armsp_csr_to_vcsr(int M, int K, const int* rowptr, const int* colind, const float* val, float threshold,
+                          int* out_rowptr, int* out_colind, float* out_val) {
+    if (M < 0 || K < 0 || !rowptr || !out_rowptr) return -1;
+    auto trig = [&](float v) { return threshold > 0.0f ? std::fabs(v) > threshold : v != 0.0f; };
+    int64_t size = 0;
+    out_rowptr[0] = 0;
+    for (int r = 0; r < M; ++r) {
+        int p = rowptr[r], e = rowptr[r + 1];
+        while (p < e) {
+            int blk = colind[p] >> 2;
+            bool keep = false;
+            int q = p;
+            while (q < e && (colind[q] >> 2) == blk) {
+                keep = keep || trig(val ? val[q] : 1.0f);
+                ++q;
+            }
+            if (keep) {
+                if (out_colind) {
+                    for (int t = 0; t < 4; ++t) {
+                        out_colind[size + t] = blk * 4 + t;
+                        if (out_val) out_val[size + t] = 0.0f;
+                    }
+                    if (out_val)
+                        for (int s = p; s < q; ++s) out_val[size + (colind[s] & 3)] = val ? val[s] : 1.0f;
+                }
+                size += 4;
+            }
+            p = q;
+        }
+        out_rowptr[r + 1] = (int)size;
+    }
+    return size;
+}
+
+int armsp_csr_check(int M, int K, int64_t nnz, const int* rowptr, const int* colind) {
+    if (!rowptr) return 1;
+    if (rowptr[0] != 0) return 1;
+    for (int r = 0; r < M; ++r)
+        if (rowptr[r + 1] < rowptr[r]) return 2;
+    if (rowptr[M] != nnz) return 3;
+    for (int r = 0; r < M; ++r)
+        for (int p = rowptr[r]; p < rowptr[r + 1]; ++p) {
+            if (colind[p] < 0 || colind[p] >= K) return 4;
+            if (p > rowptr[r] && colind[p] <= colind[p - 1]) return 5;
+        }
+    return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,178 @@
+// Library runtime: status strings, HIP error capture, stream-ordered
+// workspace, optional event timing of kernel launches.
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "common.hpp"
+
+namespace armsp {
+
+namespace {
+thread_local int g_last_hip_error = 0;
+
+std::mutex g_pool_mu;
+bool g_pool_ready[64] = {};
+
+struct TimingRecord {
+    std::string family;
+    hipEvent_t start, stop;
+};
+std::mutex g_timing_mu;
+bool g_timing = false;
+std::vector<TimingRecord*> g_records;
+
+struct FamilyTotal {
+    std::string family;
+    double ms;
+    int64_t launches;
+};
+std::vector<FamilyTotal> g_totals;
+
+void resolve_records_locked() {
+    for (TimingRecord* r : g_records) {
+        if (r->stop) {
+            hipEventSynchronize(r->stop);
+            float ms = 0.f;
+            if (hipEventElapsedTime(&ms, r->start, r->stop) == hipSuccess) {
+                bool found = false;
+                for (auto& t : g_totals)
+                    if (t.family == r->family) {
+                        t.ms += ms;
+                        t.launches += 1;
+                        found = true;
+                    }
+                if (!found) g_totals.push_back({r->family, (double)ms, 1});
+            }
+            hipEventDestroy(r->stop);
+        }
+        hipEventDestroy(r->start);
+        delete r;
+    }
+    g_records.clear();
+}
+}  // namespace
+
+int hip_fail(hipError_t e) {
+    g_last_hip_error = (int)e;
+    if (e == hipErrorOutOfMemory) return ARMSP_ERROR_ALLOC;
+    if (e == hipErrorNoDevice || e == hipErrorInvalidDevice) return ARMSP_ERROR_NO_DEVICE;
+    return ARMSP_ERROR_HIP;
+}
+
+int ws_alloc(void** p, size_t bytes, hipStream_t s) {
+    *p = nullptr;
+    if (bytes == 0) return ARMSP_SUCCESS;
+    int dev = 0;
+    ARMSP_HIP(hipGetDevice(&dev));
+    if (dev >= 0 && dev < 64) {
+        std::lock_guard<std::mutex> g(g_pool_mu);
+        if (!g_pool_ready[dev]) {
+            hipMemPool_t pool;
+            if (hipDeviceGetDefaultMemPool(&pool, dev) == hipSuccess) {
+                uint64_t thr = ~0ULL;
+                hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &thr);
+            }
+            g_pool_ready[dev] = true;
+        }
+    }
+    ARMSP_HIP(hipMallocAsync(p, bytes, s));
+    return ARMSP_SUCCESS;
+}
+
+void ws_free(void* p, hipStream_t s) {
+    if (p) hipFreeAsync(p, s);
+}
+
+bool timing_on() { return g_timing; }
+
+TimedLaunch::TimedLaunch(const char* family, hipStream_t s) {
+    if (!g_timing) return;
+    TimingRecord* r = new TimingRecord{family, nullptr, nullptr};
+    if (hipEventCreate(&r->start) != hipSuccess) {
+        delete r;
+        return;
+    }
+    hipEventRecord(r->start, s);
+    rec = r;
+    stream_ = s;
+}
+
+void TimedLaunch::stop() {
+    if (!rec) return;
+    TimingRecord* r = static_cast<TimingRecord*>(rec);
+    if (hipEventCreate(&r->stop) == hipSuccess) hipEventRecord(r->stop, stream_);
+    std::lock_guard<std::mutex> g(g_timing_mu);
+    g_records.push_back(r);
+    rec = nullptr;
+}
+
+int device_cus() {
+    static int cus = 0;
+    if (!cus) {
+        int dev = 0;
+        hipGetDevice(&dev);
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+        if (cus <= 0) cus = 256;
+    }
+    return cus;
+}
+
+}  // namespace armsp
+
+extern "C" {
+
+int armsp_version(void) { return ARMSP_VERSION; }
+
+const char* armsp_status_string(int status) {
+    switch (status) {
+        case ARMSP_SUCCESS: return "success";
+        case ARMSP_ERROR_INVALID_VALUE: return "invalid value";
+        case ARMSP_ERROR_NOT_SUPPORTED: return "not supported";
+        case ARMSP_ERROR_HIP: return "HIP runtime error";
+        case ARMSP_ERROR_ALLOC: return "device allocation failed";
+        case ARMSP_ERROR_NO_DEVICE: return "no usable gfx950 device";
+        default: return "unknown status";
+    }
+}
+
+int armsp_last_hip_error(void) { return armsp::g_last_hip_error; }
+
+int armsp_device_available(void) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return 0;
+    hipDeviceProp_t prop;
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return 0;
+    if (hipGetDeviceProperties(&prop, dev) != hipSuccess) return 0;
+    return std::string(prop.gcnArchName).rfind("gfx950", 0) == 0 ? 1 : 0;
+}
+
+int armsp_timing_enable(int on) {
+    std::lock_guard<std::mutex> g(armsp::g_timing_mu);
+    armsp::g_timing = on != 0;
+    return ARMSP_SUCCESS;
+}
+
+int armsp_timing_reset(void) {
+    std::lock_guard<std::mutex> g(armsp::g_timing_mu);
+    armsp::resolve_records_locked();
+    armsp::g_totals.clear();
+    return ARMSP_SUCCESS;
+}
+
+int armsp_timing_read(const char* family, double* total_ms, int64_t* launches) {
+    if (!family || !total_ms || !launches) return ARMSP_ERROR_INVALID_VALUE;
+    std::lock_guard<std::mutex> g(armsp::g_timing_mu);
+    armsp::resolve_records_locked();
+    *total_ms = 0.0;
+    *launches = 0;
+    for (auto& t : armsp::g_totals)
+        if (t.family == family) {
+            *total_ms = t.ms;
+            *launches = t.launches;
+        }
+    return ARMSP_SUCCESS;
+}
+
+}  // extern "C"

@@ -1,0 +1,222 @@
+// SDDMM for gfx950: out[e] = s_e · <U[row(e)], V[col(e)]> on a CSR or COO pattern.
+//
+// Reference semantics: gcn/pytorch-custom/sddmm.cu:7-424 (COO kernels with 16
+// edges per block and an 8/16/32-lane shuffle all-reduce per edge; CSR kernels
+// that binary-search the row of every edge, computeUtil.h:11-28).  Here each
+// wave owns a contiguous range of nonzeros; a group of G lanes computes one
+// dot product with W-element vector loads, each group takes G consecutive
+// nonzeros, and a transpose-reduce (G-1 shuffles for G dot products) leaves
+// lane L holding nonzero base+L, so the output store is one coalesced line
+// per wave.  The row of a CSR nonzero is found by a per-lane binary search
+// restricted to the wave's row range.
+#include <climits>
+
+#include "common.hpp"
+
+namespace armsp {
+namespace {
+
+template <typename T, int W>
+__device__ __forceinline__ void load_vec(const T* __restrict__ p, float (&o)[W]) {
+    if constexpr (sizeof(T) == 4) {
+        if constexpr (W == 4) {
+            float4 t = *reinterpret_cast<const float4*>(p);
+            o[0] = t.x; o[1] = t.y; o[2] = t.z; o[3] = t.w;
+        } else {
+#pragma unroll
+            for (int i = 0; i < W; ++i) o[i] = p[i];
+        }
+    } else {
+        if constexpr (W == 8) {
+            uint4 t = *reinterpret_cast<const uint4*>(p);
+            uint32_t w[4] = {t.x, t.y, t.z, t.w};
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                o[2 * i] = __uint_as_float(w[i] << 16);
+                o[2 * i + 1] = __uint_as_float(w[i] & 0xffff0000u);
+            }
+        } else if constexpr (W == 4) {
+            uint2 t = *reinterpret_cast<const uint2*>(p);
+            o[0] = __uint_as_float(t.x << 16); o[1] = __uint_as_float(t.x & 0xffff0000u);
+            o[2] = __uint_as_float(t.y << 16); o[3] = __uint_as_float(t.y & 0xffff0000u);
+        } else {
+#pragma unroll
+            for (int i = 0; i < W; ++i) o[i] = bf16_to_f32(p[i]);
+        }
+    }
+}
+
+// G lanes per nonzero, W elements per load, T loads per lane (Kd ≤ G·W·T).
+template <typename T, int G, int W, bool COO>
+__global__ __launch_bounds__(256) void k_sddmm(int M, int Kd, int nnz, int chunk, const int* __restrict__ rowidx,
+                                               const int* __restrict__ colind, const float* __restrict__ scale,
+                                               const T* __restrict__ U, int ldu, const T* __restrict__ V, int ldv,
+                                               float* __restrict__ out) {
+    const int lane = threadIdx.x & 63;
+    const int g = lane / G, l = lane & (G - 1);
+    const int wave = uniform((int)(blockIdx.x * 4u + (threadIdx.x >> 6)));
+    const int64_t j0l = (int64_t)wave * chunk;
+    if (j0l >= nnz) return;
+    const int j0 = (int)j0l;
+    const int j1 = (int)(j0l + chunk < nnz ? j0l + chunk : nnz);
+    int rlo = 0, rhi = 0;
+    if constexpr (!COO) {
+        rlo = uniform(rows_ending_by(j0, rowidx, M));
+        rhi = uniform(rows_ending_by(j1 - 1, rowidx, M));
+    }
+    const int steps = (Kd + G * W - 1) / (G * W);
+    const bool full = (Kd % (G * W)) == 0;
+
+    for (int base = j0; base < j1; base += 64) {
+        const int e = base + lane;
+        int myrow = 0, mycol = 0;
+        if (e < j1) {
+            mycol = colind[e];
+            if constexpr (COO) {
+                myrow = rowidx[e];
+            } else {
+                int lo = rlo, hi = rhi;  // row r with rowptr[r] <= e < rowptr[r+1]
+                while (lo < hi) {
+                    const int mid = (lo + hi + 1) >> 1;
+                    if (rowidx[mid] <= e)
+                        lo = mid;
+                    else
+                        hi = mid - 1;
+                }
+                myrow = lo;
+            }
+        }
+        if constexpr (!COO) {
+            if (base + 64 < j1) rlo = uniform(__shfl(myrow, 63));  // later nonzeros are in rows ≥ this one
+        }
+        float p[G];
+#pragma unroll
+        for (int q = 0; q < G; ++q) {
+            const int src = g * G + q;
+            const int r = __shfl(myrow, src);
+            const int c = __shfl(mycol, src);
+            float acc = 0.0f;
+            if (base + src < j1) {
+                const T* up = U + (size_t)r * ldu;
+                const T* vp = V + (size_t)c * ldv;
+                for (int t = 0; t < steps; ++t) {
+                    const int k = (t * G + l) * W;
+                    float a[W], b[W];
+                    if (full || k + W <= Kd) {
+                        load_vec<T, W>(up + k, a);
+                        load_vec<T, W>(vp + k, b);
+                    } else {
+#pragma unroll
+                        for (int i = 0; i < W; ++i) {
+                            const bool in = k + i < Kd;
+                            if constexpr (sizeof(T) == 4) {
+                                a[i] = in ? (float)up[k + i] : 0.0f;
+                                b[i] = in ? (float)vp[k + i] : 0.0f;
+                            } else {
+                                a[i] = in ? bf16_to_f32(up[k + i]) : 0.0f;
+                                b[i] = in ? bf16_to_f32(vp[k + i]) : 0.0f;
+                            }
+                        }
+                    }
+#pragma unroll
+                    for (int i = 0; i < W; ++i) acc = fmaf(a[i], b[i], acc);
+                }
+            }
+            p[q] = acc;
+        }
+        // transpose-reduce: after the stage with mask m, lane keeps the half of
+        // its G partials selected by (l & m); finally lane l holds nonzero l.
+#pragma unroll
+        for (int m = G / 2; m >= 1; m >>= 1) {
+            const bool upper = (l & m) != 0;
+#pragma unroll
+            for (int i = 0; i < m; ++i) {
+                const float send = upper ? p[i] : p[i + m];
+                const float keep = upper ? p[i + m] : p[i];
+                p[i] = keep + __shfl_xor(send, m);
+            }
+        }
+        if (e < j1) out[e] = scale ? scale[e] * p[0] : p[0];
+    }
+}
+
+template <typename T, int G, int W>
+void launch_sddmm(bool coo, dim3 grid, hipStream_t s, int M, int Kd, int nnz, int chunk, const int* ri, const int* ci,
+                  const float* sc, const T* U, int ldu, const T* V, int ldv, float* out) {
+    if (coo)
+        hipLaunchKernelGGL((k_sddmm<T, G, W, true>), grid, dim3(256), 0, s, M, Kd, nnz, chunk, ri, ci, sc, U, ldu, V,
+                           ldv, out);
+    else
+        hipLaunchKernelGGL((k_sddmm<T, G, W, false>), grid, dim3(256), 0, s, M, Kd, nnz, chunk, ri, ci, sc, U, ldu, V,
+                           ldv, out);
+}
+
+inline bool aligned(const void* p, int bytes) { return ((uintptr_t)p % (uintptr_t)bytes) == 0; }
+
+template <typename T>
+int sddmm_impl(bool coo, int M, int K, int Kd, int nnz, const int* rowidx, const int* colind, const float* scale,
+               const T* U, int ldu, const T* V, int ldv, float* out, const armsp_config* cfg, void* stream) {
+    if (M < 0 || K < 0 || Kd < 0 || nnz < 0) return ARMSP_ERROR_INVALID_VALUE;
+    if (nnz == 0) return ARMSP_SUCCESS;
+    if (!rowidx || !colind || !out || Kd == 0 || !U || !V || ldu < Kd || ldv < Kd) return ARMSP_ERROR_INVALID_VALUE;
+    hipStream_t s = as_stream(stream);
+    armsp_config c = cfg ? *cfg : armsp_config{0, 0, 0, 0, 0};
+    if (c.schedule != ARMSP_SCHED_AUTO && c.schedule != ARMSP_SCHED_NNZ) return ARMSP_ERROR_NOT_SUPPORTED;
+    // vector width: 16-byte loads when every row start is aligned
+    constexpr int WMAX = sizeof(T) == 4 ? 4 : 8;
+    int W = c.vec > 0 ? c.vec : WMAX;
+    if (W != 1 && W != WMAX) W = 1;
+    if (W > 1 && (ldu % W || ldv % W || !aligned(U, 16) || !aligned(V, 16))) W = 1;
+    int G = c.group;
+    if (G <= 0) {
+        G = 4;
+        while (G < 16 && G * W < Kd) G <<= 1;
+    }
+    if (G != 4 && G != 8 && G != 16 && G != 32) return ARMSP_ERROR_INVALID_VALUE;
+    const int chunk = c.chunk > 0 ? c.chunk : 256;
+    if (chunk % 64) return ARMSP_ERROR_INVALID_VALUE;
+    const int64_t nwaves = ((int64_t)nnz + chunk - 1) / chunk;
+    if (nwaves > INT_MAX / 4) return ARMSP_ERROR_NOT_SUPPORTED;
+    const dim3 grid((unsigned)((nwaves + 3) / 4));
+    TimedLaunch t("sddmm", s);
+#define ARMSP_SDDMM_G(WW)                                                                                    \
+    switch (G) {                                                                                             \
+        case 4: launch_sddmm<T, 4, WW>(coo, grid, s, M, Kd, nnz, chunk, rowidx, colind, scale, U, ldu, V, ldv, out); break; \
+        case 8: launch_sddmm<T, 8, WW>(coo, grid, s, M, Kd, nnz, chunk, rowidx, colind, scale, U, ldu, V, ldv, out); break; \
+        case 16: launch_sddmm<T, 16, WW>(coo, grid, s, M, Kd, nnz, chunk, rowidx, colind, scale, U, ldu, V, ldv, out); break; \
+        default: launch_sddmm<T, 32, WW>(coo, grid, s, M, Kd, nnz, chunk, rowidx, colind, scale, U, ldu, V, ldv, out); break; \
+    }
+    if (W == 1) {
+        ARMSP_SDDMM_G(1)
+    } else {
+        ARMSP_SDDMM_G(WMAX)
+    }
+#undef ARMSP_SDDMM_G
+    ARMSP_LAUNCHED();
+    t.stop();
+    return ARMSP_SUCCESS;
+}
+
+}  // namespace
+}  // namespace armsp
+
+extern "C" {
+
+int armsp_sddmm_csr_f32(int M, int K, int Kd, int nnz, const int* rowptr, const int* colind, const float* scale,
+                        const float* U, int ldu, const float* V, int ldv, float* out, const armsp_config* cfg,
+                        void* stream) {
+    return armsp::sddmm_impl<float>(false, M, K, Kd, nnz, rowptr, colind, scale, U, ldu, V, ldv, out, cfg, stream);
+}
+
+int armsp_sddmm_coo_f32(int M, int K, int Kd, int nnz, const int* rowind, const int* colind, const float* scale,
+                        const float* U, int ldu, const float* V, int ldv, float* out, void* stream) {
+    return armsp::sddmm_impl<float>(true, M, K, Kd, nnz, rowind, colind, scale, U, ldu, V, ldv, out, nullptr, stream);
+}
+
+int armsp_sddmm_csr_bf16(int M, int K, int Kd, int nnz, const int* rowptr, const int* colind, const float* scale,
+                         const uint16_t* U, int ldu, const uint16_t* V, int ldv, float* out, const armsp_config* cfg,
+                         void* stream) {
+    return armsp::sddmm_impl<uint16_t>(false, M, K, Kd, nnz, rowptr, colind, scale, U, ldu, V, ldv, out, cfg, stream);
+}
+
+}  // extern "C"

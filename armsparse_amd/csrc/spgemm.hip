@@ -1,0 +1,84 @@
+// Sparse × sparse → dense for gfx950: C[M×N] += A·B, A and B CSR.
+//
+// Reference: sparse_matrix_mul3 (SpMM.cpp:309-327) — per row m of A, a
+// Gustavson scatter C[m, colB[t]] += valA[idx]·valB[t] directly into the dense
+// output row.  Here one wave owns (row m, a tile of up to TN columns): the C
+// tile is staged in LDS, the nonzeros of A's row are taken in CSR order and
+// each B row is scattered by the lanes (its columns are distinct, so one pass
+// has no conflicts), then the tile is written back.  Every output element
+// sees the same fmaf sequence as the reference, so the result is bit-exact.
+#include "common.hpp"
+
+namespace armsp {
+namespace {
+
+constexpr int kTN = 4096;  // columns per wave tile (16 KiB of LDS per wave)
+
+__device__ __forceinline__ void wave_sync_lds() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__global__ __launch_bounds__(256) void k_spgemm_dense(int M, int N, int ntiles, const int* __restrict__ rpA,
+                                                      const int* __restrict__ ciA, const float* __restrict__ vA,
+                                                      const int* __restrict__ rpB, const int* __restrict__ ciB,
+                                                      const float* __restrict__ vB, float* __restrict__ C, int ldc) {
+    __shared__ float tile[4][kTN];
+    const int lane = threadIdx.x & 63;
+    const int wid = threadIdx.x >> 6;
+    const int item = uniform((int)(blockIdx.x * 4u + wid));
+    if (item >= M * ntiles) return;
+    const int m = item / ntiles;
+    const int c0 = (item % ntiles) * kTN;
+    const int tn = min(kTN, N - c0);
+    float* t = tile[wid];
+    float* crow = C + (size_t)m * ldc + c0;
+    for (int k = lane; k < tn; k += 64) t[k] = crow[k];
+    wave_sync_lds();
+
+    const int beg = rpA[m], end = rpA[m + 1];
+    for (int wbase = beg; wbase < end; wbase += 64) {
+        // A's nonzeros for this window, broadcast with readlane
+        int wk = 0;
+        float wa = 0.0f;
+        if (wbase + lane < end) {
+            wk = ciA[wbase + lane];
+            wa = vA ? vA[wbase + lane] : 1.0f;
+        }
+        const int n = min(64, end - wbase);
+        for (int q = 0; q < n; ++q) {
+            const int k = read_lane(wk, q);
+            const float a = read_lane(wa, q);
+            const int bb = rpB[k], be = rpB[k + 1];
+            for (int p = bb + lane; p < be; p += 64) {
+                const int j = ciB[p] - c0;
+                if (j >= 0 && j < tn) t[j] = fmaf(a, vB ? vB[p] : 1.0f, t[j]);
+            }
+            wave_sync_lds();
+        }
+    }
+    for (int k = lane; k < tn; k += 64) crow[k] = t[k];
+}
+
+}  // namespace
+}  // namespace armsp
+
+extern "C" int armsp_spgemm_dense_f32(int M, int K, int N, int nnzA, const int* rowptrA, const int* colindA,
+                                      const float* valA, int nnzB, const int* rowptrB, const int* colindB,
+                                      const float* valB, float* C, int ldc, void* stream) {
+    using namespace armsp;
+    if (M < 0 || K < 0 || N < 0 || nnzA < 0 || nnzB < 0) return ARMSP_ERROR_INVALID_VALUE;
+    if (M == 0 || N == 0 || nnzA == 0 || nnzB == 0) return ARMSP_SUCCESS;
+    if (!rowptrA || !colindA || !rowptrB || !colindB || !C || ldc < N) return ARMSP_ERROR_INVALID_VALUE;
+    const int ntiles = (N + kTN - 1) / kTN;
+    const int64_t items = (int64_t)M * ntiles;
+    if (items > 0x7fffffffLL - 4) return ARMSP_ERROR_NOT_SUPPORTED;
+    hipStream_t s = as_stream(stream);
+    TimedLaunch t("spgemm", s);
+    hipLaunchKernelGGL(k_spgemm_dense, dim3((unsigned)((items + 3) / 4)), dim3(256), 0, s, M, N, ntiles, rowptrA,
+                       colindA, valA, rowptrB, colindB, valB, C, ldc);
+    ARMSP_LAUNCHED();
+    t.stop();
+    return ARMSP_SUCCESS;
+}

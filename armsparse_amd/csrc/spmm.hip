@@ -1,0 +1,526 @@
+// CSR SpMM for gfx950: C (=|+=) A·B with B in fp32, bf16 or fp8-e4m3.
+//
+// The reference kernel family (SDMM.cpp:21-180, 422-448;
+// gcn/pytorch-custom/spmm.cpp:49-93) runs an OpenMP loop over rows and, per
+// nonzero, an axpy of a B row into the C row (num_vector_neon, SDMM.cpp:449).
+// Here the C row lives in registers instead: lanes cover the columns, and the
+// nonzeros are streamed in CSR order, so the per-element fmaf sequence is the
+// reference's one and the C row is written once.
+//
+// Schedules:
+//  ROW   — a group of G lanes owns a row (G = 64 for wide N, down to 4 for
+//          narrow N with 64/G rows per wave).  Bit-exact with the CPU order.
+//  NNZ   — each wave owns a fixed range of nonzeros.
+//  MERGE — each wave owns a fixed range of the merge path over
+//          (row ends, nonzeros), so empty rows and power-law hub rows cost
+//          the same per item (Merrill & Garland's merge-based CSR).
+//  NNZ and MERGE keep the partial sum of a row cut by a wave boundary in a
+//  carry buffer; a second kernel adds each run of carries into its row in
+//  wave order, so results are deterministic run to run.
+#include <climits>
+
+#include "common.hpp"
+
+namespace armsp {
+namespace {
+
+// ----------------------------------------------------------- dense loads
+template <typename TB, int V>
+__device__ __forceinline__ void load_b(const TB* __restrict__ p, float (&o)[V]) {
+    if constexpr (sizeof(TB) == 4) {
+        if constexpr (V == 1) {
+            o[0] = p[0];
+        } else if constexpr (V == 2) {
+            float2 t = *reinterpret_cast<const float2*>(p);
+            o[0] = t.x; o[1] = t.y;
+        } else if constexpr (V == 4) {
+            float4 t = *reinterpret_cast<const float4*>(p);
+            o[0] = t.x; o[1] = t.y; o[2] = t.z; o[3] = t.w;
+        }
+    } else if constexpr (sizeof(TB) == 2) {  // bf16 bits
+        if constexpr (V == 1) {
+            o[0] = bf16_to_f32(p[0]);
+        } else if constexpr (V == 2) {
+            uint32_t t = *reinterpret_cast<const uint32_t*>(p);
+            o[0] = __uint_as_float(t << 16); o[1] = __uint_as_float(t & 0xffff0000u);
+        } else if constexpr (V == 4) {
+            uint2 t = *reinterpret_cast<const uint2*>(p);
+            o[0] = __uint_as_float(t.x << 16); o[1] = __uint_as_float(t.x & 0xffff0000u);
+            o[2] = __uint_as_float(t.y << 16); o[3] = __uint_as_float(t.y & 0xffff0000u);
+        } else if constexpr (V == 8) {
+            uint4 t = *reinterpret_cast<const uint4*>(p);
+            uint32_t w[4] = {t.x, t.y, t.z, t.w};
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                o[2 * i] = __uint_as_float(w[i] << 16);
+                o[2 * i + 1] = __uint_as_float(w[i] & 0xffff0000u);
+            }
+        }
+    } else {  // fp8-e4m3 bytes
+        uint32_t w[2] = {0u, 0u};
+        if constexpr (V == 1) w[0] = p[0];
+        else if constexpr (V == 2) w[0] = *reinterpret_cast<const uint16_t*>(p);
+        else if constexpr (V == 4) w[0] = *reinterpret_cast<const uint32_t*>(p);
+        else if constexpr (V == 8) {
+            uint2 t = *reinterpret_cast<const uint2*>(p);
+            w[0] = t.x; w[1] = t.y;
+        }
+#pragma unroll
+        for (int i = 0; i < V; ++i) o[i] = e4m3_to_f32((w[i >> 2] >> (8 * (i & 3))) & 0xffu);
+    }
+}
+
+template <int V>
+__device__ __forceinline__ void load_c(const float* __restrict__ p, float (&o)[V]) {
+    load_b<float, V>(p, o);
+}
+
+template <int V>
+__device__ __forceinline__ void store_c(float* __restrict__ p, const float (&o)[V]) {
+    if constexpr (V == 1) {
+        __builtin_nontemporal_store(o[0], p);
+    } else if constexpr (V == 2) {
+        __builtin_nontemporal_store(o[0], p);
+        __builtin_nontemporal_store(o[1], p + 1);
+    } else {
+#pragma unroll
+        for (int i = 0; i < V; ++i) __builtin_nontemporal_store(o[i], p + i);
+    }
+}
+
+template <typename TB>
+__device__ __forceinline__ float coef(const float* __restrict__ val, const float* __restrict__ bscale, int idx, int c) {
+    float a = val ? val[idx] : 1.0f;
+    if constexpr (sizeof(TB) == 1) a *= bscale[c];
+    return a;
+}
+
+// ------------------------------------------------------------- ROW split
+template <typename TB, int V, int G>
+__global__ __launch_bounds__(256) void k_spmm_row(int M, int N, const int* __restrict__ rowptr,
+                                                  const int* __restrict__ colind, const float* __restrict__ val,
+                                                  const TB* __restrict__ B, int ldb, const float* __restrict__ bscale,
+                                                  float* __restrict__ C, int ldc, int flags) {
+    constexpr int U = G < 4 ? G : 4;
+    const int lane = threadIdx.x & (G - 1);
+    int row = (int)((blockIdx.x * 256u + threadIdx.x) / G);
+    if constexpr (G == 64) row = uniform(row);
+    if (row >= M) return;
+    const int col = (blockIdx.y * G + lane) * V;
+    const bool active = col < N;
+    const int beg = rowptr[row];
+    int end = rowptr[row + 1];
+    if (flags & ARMSP_TRUNC4) end = beg + ((end - beg) & ~3);
+    float acc[V];
+#pragma unroll
+    for (int v = 0; v < V; ++v) acc[v] = 0.0f;
+    if ((flags & ARMSP_ACCUMULATE) && active) load_c<V>(C + (size_t)row * ldc + col, acc);
+
+    for (int base = beg; base < end; base += G) {
+        const int n = min(G, end - base);
+        int mc = 0;
+        float mv = 0.0f;
+        if (lane < n) {
+            mc = colind[base + lane];
+            mv = coef<TB>(val, bscale, base + lane, mc);
+        }
+        for (int j = 0; j < n; j += U) {
+            float b[U][V];
+            float a[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                int c;
+                if constexpr (G == 64) {
+                    c = read_lane(mc, j + u);
+                    a[u] = read_lane(mv, j + u);
+                } else {
+                    c = __shfl(mc, j + u, G);
+                    a[u] = __shfl(mv, j + u, G);
+                }
+#pragma unroll
+                for (int v = 0; v < V; ++v) b[u][v] = 0.0f;
+                if (j + u < n && active) load_b<TB, V>(B + (size_t)c * ldb + col, b[u]);
+            }
+#pragma unroll
+            for (int u = 0; u < U; ++u)
+                if (j + u < n) {
+#pragma unroll
+                    for (int v = 0; v < V; ++v) acc[v] = fmaf(a[u], b[u][v], acc[v]);
+                }
+        }
+    }
+    if (active) store_c<V>(C + (size_t)row * ldc + col, acc);
+}
+
+// ----------------------------------------------------- NNZ / MERGE split
+// One wave per `chunk` items.  Lanes cover 64·V columns of the tile
+// blockIdx.y.  colind/val of the wave's range are read 64 at a time into a
+// register window and broadcast with readlane; U dense rows are fetched
+// before any of them is consumed so U gathers are in flight per wave.
+template <typename TB, int V, int U, bool MERGE>
+__global__ __launch_bounds__(256) void k_spmm_split(int M, int N, int nnz, int chunk, const int* __restrict__ rowptr,
+                                                    const int* __restrict__ colind, const float* __restrict__ val,
+                                                    const TB* __restrict__ B, int ldb, const float* __restrict__ bscale,
+                                                    float* __restrict__ C, int ldc, int flags,
+                                                    int* __restrict__ carry_row, float* __restrict__ carry_val) {
+    const int lane = threadIdx.x & 63;
+    const int wave = uniform((int)(blockIdx.x * 4u + (threadIdx.x >> 6)));
+    const int64_t total = MERGE ? (int64_t)M + nnz : (int64_t)nnz;
+    const int64_t d0l = (int64_t)wave * chunk;
+    if (d0l >= total) return;
+    const int d0 = (int)d0l;
+    const int d1 = (int)min(d0l + chunk, total);
+    int i0, j0, i1, j1;
+    if constexpr (MERGE) {
+        i0 = uniform(merge_path_search(d0, rowptr, M, nnz));
+        i1 = uniform(merge_path_search(d1, rowptr, M, nnz));
+        j0 = d0 - i0;
+        j1 = d1 - i1;
+    } else {
+        j0 = d0;
+        j1 = d1;
+        i0 = d0 == 0 ? 0 : uniform(rows_ending_by(j0, rowptr, M));
+        i1 = uniform(rows_ending_by(j1, rowptr, M));
+    }
+    const bool acc_mode = (flags & ARMSP_ACCUMULATE) != 0;
+    const int col = (int)(blockIdx.y * 64u + lane) * V;
+    const bool active = col < N;
+
+    // register windows: nonzeros [jw, jw+64), row ends of rows [rw, rw+64)
+    int jw = j0, wc = 0;
+    float wv = 0.0f;
+    if (jw + lane < j1) {
+        wc = colind[jw + lane];
+        wv = coef<TB>(val, bscale, jw + lane, wc);
+    }
+    int r = i0, rw = i0;
+    int wre = (rw + lane < M) ? rowptr[rw + 1 + lane] : INT_MAX;
+    int re = read_lane(wre, 0);
+
+    float acc[V];
+#pragma unroll
+    for (int v = 0; v < V; ++v) acc[v] = 0.0f;
+
+    auto emit = [&]() {
+        if (active) {
+            float* cp = C + (size_t)r * ldc + col;
+            float o[V];
+            if (acc_mode) {
+                load_c<V>(cp, o);
+#pragma unroll
+                for (int v = 0; v < V; ++v) o[v] += acc[v];
+            } else {
+#pragma unroll
+                for (int v = 0; v < V; ++v) o[v] = acc[v];
+            }
+            store_c<V>(cp, o);
+        }
+#pragma unroll
+        for (int v = 0; v < V; ++v) acc[v] = 0.0f;
+        ++r;
+        if (r - rw == 64) {
+            rw = r;
+            wre = (rw + lane < M) ? rowptr[rw + 1 + lane] : INT_MAX;
+        }
+        re = read_lane(wre, r - rw);
+    };
+
+    for (int jb = j0; jb < j1; jb += U) {
+        if (jb - jw == 64) {
+            jw = jb;
+            wc = 0;
+            wv = 0.0f;
+            if (jw + lane < j1) {
+                wc = colind[jw + lane];
+                wv = coef<TB>(val, bscale, jw + lane, wc);
+            }
+        }
+        float b[U][V];
+        float a[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int c = read_lane(wc, jb - jw + u);
+            a[u] = read_lane(wv, jb - jw + u);
+#pragma unroll
+            for (int v = 0; v < V; ++v) b[u][v] = 0.0f;
+            if (jb + u < j1 && active) load_b<TB, V>(B + (size_t)c * ldb + col, b[u]);
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int jj = jb + u;
+            if (jj < j1) {
+                while (r < i1 && re <= jj) emit();
+#pragma unroll
+                for (int v = 0; v < V; ++v) acc[v] = fmaf(a[u], b[u][v], acc[v]);
+            }
+        }
+    }
+    while (r < i1) emit();
+
+    // partial sum of row i1 (nonzeros [max(j0, rowptr[i1]), j1)) goes to the carry
+    bool has = false;
+    if (i1 < M) {
+        const int rs = rowptr[i1];
+        has = j1 > (rs > j0 ? rs : j0);
+    }
+    if (blockIdx.y == 0 && lane == 0) carry_row[wave] = has ? i1 : -1;
+    if (has && active) {
+        float* cp = carry_val + (size_t)wave * N + col;
+#pragma unroll
+        for (int v = 0; v < V; ++v) cp[v] = acc[v];
+    }
+}
+
+// Adds each run of carries (consecutive waves cut inside the same row) into
+// that row, summing the run in wave order.
+template <int V>
+__global__ __launch_bounds__(256) void k_carry_fixup(int nwaves, int N, const int* __restrict__ carry_row,
+                                                     const float* __restrict__ carry_val, float* __restrict__ C,
+                                                     int ldc) {
+    const int lane = threadIdx.x & 63;
+    const int w = uniform((int)(blockIdx.x * 4u + (threadIdx.x >> 6)));
+    if (w >= nwaves) return;
+    const int r = carry_row[w];
+    if (r < 0) return;
+    if (w > 0 && carry_row[w - 1] == r) return;
+    // run length
+    int run = 0;
+    for (int base = w;; base += 64) {
+        const int cr = (base + lane < nwaves) ? carry_row[base + lane] : -2;
+        const uint64_t m = __ballot(cr == r);
+        const int cnt = (~m == 0ull) ? 64 : __builtin_ctzll(~m);
+        run += cnt;
+        if (cnt < 64) break;
+    }
+    for (int col = (int)(blockIdx.y * 64u + lane) * V; col < N; col += (int)gridDim.y * 64 * V) {
+        float s[V];
+#pragma unroll
+        for (int v = 0; v < V; ++v) s[v] = 0.0f;
+        int k = 0;
+        for (; k + 4 <= run; k += 4) {
+            float t[4][V];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) load_c<V>(carry_val + (size_t)(w + k + q) * N + col, t[q]);
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+#pragma unroll
+                for (int v = 0; v < V; ++v) s[v] += t[q][v];
+        }
+        for (; k < run; ++k) {
+            float t[V];
+            load_c<V>(carry_val + (size_t)(w + k) * N + col, t);
+#pragma unroll
+            for (int v = 0; v < V; ++v) s[v] += t[v];
+        }
+        float* cp = C + (size_t)r * ldc + col;
+        float o[V];
+        load_c<V>(cp, o);
+#pragma unroll
+        for (int v = 0; v < V; ++v) o[v] += s[v];
+#pragma unroll
+        for (int v = 0; v < V; ++v) cp[v] = o[v];
+    }
+}
+
+// ------------------------------------------------------------- dispatch
+template <typename TB>
+constexpr int max_vec() {
+    return sizeof(TB) == 4 ? 4 : 8;
+}
+
+inline bool aligned(const void* p, int bytes) { return ((uintptr_t)p % (uintptr_t)bytes) == 0; }
+
+template <typename TB>
+int pick_vec(int N, int ldb, int ldc, const TB* B, const float* C, int want, int lanes) {
+    int v = want;
+    if (v <= 0) {
+        v = 1;
+        while (v < max_vec<TB>() && lanes * v < N) v <<= 1;
+    }
+    while (v > 1 && (N % v || ldb % v || ldc % v || !aligned(B, v * (int)sizeof(TB)) || !aligned(C, v * 4))) v >>= 1;
+    return v;
+}
+
+template <typename TB, int V>
+int launch_row_v(int G, dim3 grid, hipStream_t s, int M, int N, const int* rp, const int* ci, const float* va,
+                 const TB* B, int ldb, const float* bs, float* C, int ldc, int flags) {
+    switch (G) {
+#define ARMSP_ROW_CASE(GG)                                                                                   \
+    case GG:                                                                                                 \
+        hipLaunchKernelGGL((k_spmm_row<TB, V, GG>), grid, dim3(256), 0, s, M, N, rp, ci, va, B, ldb, bs, C, ldc, \
+                           flags);                                                                           \
+        break;
+        ARMSP_ROW_CASE(4)
+        ARMSP_ROW_CASE(8)
+        ARMSP_ROW_CASE(16)
+        ARMSP_ROW_CASE(32)
+        ARMSP_ROW_CASE(64)
+#undef ARMSP_ROW_CASE
+        default: return ARMSP_ERROR_INVALID_VALUE;
+    }
+    return ARMSP_SUCCESS;
+}
+
+template <typename TB, int V, bool MERGE>
+int launch_split_vu(int U, dim3 grid, hipStream_t s, int M, int N, int nnz, int chunk, const int* rp, const int* ci,
+                    const float* va, const TB* B, int ldb, const float* bs, float* C, int ldc, int flags, int* crow,
+                    float* cval) {
+    switch (U) {
+#define ARMSP_SPLIT_CASE(UU)                                                                                \
+    case UU:                                                                                                \
+        hipLaunchKernelGGL((k_spmm_split<TB, V, UU, MERGE>), grid, dim3(256), 0, s, M, N, nnz, chunk, rp, ci, va, \
+                           B, ldb, bs, C, ldc, flags, crow, cval);                                          \
+        break;
+        ARMSP_SPLIT_CASE(4)
+        ARMSP_SPLIT_CASE(8)
+        ARMSP_SPLIT_CASE(16)
+#undef ARMSP_SPLIT_CASE
+        default: return ARMSP_ERROR_INVALID_VALUE;
+    }
+    return ARMSP_SUCCESS;
+}
+
+template <typename TB>
+int spmm_impl(int M, int K, int N, int nnz, const int* rowptr, const int* colind, const float* val, const TB* B,
+              const float* bscale, int ldb, float* C, int ldc, int flags, const armsp_config* cfg, void* stream) {
+    if (M < 0 || K < 0 || N < 0 || nnz < 0) return ARMSP_ERROR_INVALID_VALUE;
+    if (M == 0 || N == 0) return ARMSP_SUCCESS;
+    if (!rowptr || !C || ldc < N) return ARMSP_ERROR_INVALID_VALUE;
+    if (nnz > 0 && (!colind || !B || ldb < N || K == 0)) return ARMSP_ERROR_INVALID_VALUE;
+    if (sizeof(TB) == 1 && nnz > 0 && !bscale) return ARMSP_ERROR_INVALID_VALUE;
+    if (flags & ~(ARMSP_ACCUMULATE | ARMSP_TRUNC4)) return ARMSP_ERROR_INVALID_VALUE;
+    if ((int64_t)M + nnz > INT_MAX) return ARMSP_ERROR_NOT_SUPPORTED;
+    hipStream_t s = as_stream(stream);
+
+    armsp_config c = cfg ? *cfg : armsp_config{0, 0, 0, 0, 0};
+    int sched = c.schedule;
+    if (sched == ARMSP_SCHED_AUTO) sched = (N >= 32) ? ARMSP_SCHED_MERGE : ARMSP_SCHED_ROW;
+    if (flags & ARMSP_TRUNC4) sched = ARMSP_SCHED_ROW;  // truncation is per row
+    if (nnz == 0) sched = ARMSP_SCHED_ROW;               // writes (or keeps) every row
+
+    if (sched == ARMSP_SCHED_ROW) {
+        int G = c.group;
+        int V = pick_vec<TB>(N, ldb, ldc, B, C, c.vec, G > 0 ? G : 64);
+        if (G <= 0) {
+            G = 4;
+            while (G < 64 && G * V < N) G <<= 1;
+        }
+        if (G < 4 || G > 64 || (G & (G - 1))) return ARMSP_ERROR_INVALID_VALUE;
+        dim3 grid((unsigned)(((int64_t)M * G + 255) / 256), (unsigned)((N + G * V - 1) / (G * V)));
+        if (grid.y > 65535) return ARMSP_ERROR_NOT_SUPPORTED;
+        TimedLaunch t("spmm", s);
+        int st;
+        switch (V) {
+            case 1: st = launch_row_v<TB, 1>(G, grid, s, M, N, rowptr, colind, val, B, ldb, bscale, C, ldc, flags); break;
+            case 2: st = launch_row_v<TB, 2>(G, grid, s, M, N, rowptr, colind, val, B, ldb, bscale, C, ldc, flags); break;
+            case 4: st = launch_row_v<TB, 4>(G, grid, s, M, N, rowptr, colind, val, B, ldb, bscale, C, ldc, flags); break;
+            case 8:
+                if constexpr (sizeof(TB) < 4) {
+                    st = launch_row_v<TB, 8>(G, grid, s, M, N, rowptr, colind, val, B, ldb, bscale, C, ldc, flags);
+                    break;
+                }
+                [[fallthrough]];
+            default: return ARMSP_ERROR_INVALID_VALUE;
+        }
+        if (st) return st;
+        ARMSP_LAUNCHED();
+        t.stop();
+        return ARMSP_SUCCESS;
+    }
+    if (sched != ARMSP_SCHED_NNZ && sched != ARMSP_SCHED_MERGE) return ARMSP_ERROR_INVALID_VALUE;
+
+    const bool merge = sched == ARMSP_SCHED_MERGE;
+    const int V = pick_vec<TB>(N, ldb, ldc, B, C, c.vec, 64);
+    int U = c.unroll > 0 ? c.unroll : (V <= 2 ? 8 : 4);
+    if (U != 4 && U != 8 && U != 16) return ARMSP_ERROR_INVALID_VALUE;
+    int chunk = c.chunk > 0 ? c.chunk : 512;
+    if (chunk % 64) return ARMSP_ERROR_INVALID_VALUE;
+    const int64_t total = merge ? (int64_t)M + nnz : (int64_t)nnz;
+    const int64_t nwaves = (total + chunk - 1) / chunk;
+    if (nwaves > INT_MAX / 4) return ARMSP_ERROR_NOT_SUPPORTED;
+    const unsigned tiles = (unsigned)((N + 64 * V - 1) / (64 * V));
+    if (tiles > 65535) return ARMSP_ERROR_NOT_SUPPORTED;
+
+    void* ws = nullptr;
+    const size_t row_bytes = ((size_t)nwaves * sizeof(int) + 255) & ~(size_t)255;
+    int st = ws_alloc(&ws, row_bytes + (size_t)nwaves * N * sizeof(float), s);
+    if (st) return st;
+    int* crow = static_cast<int*>(ws);
+    float* cval = reinterpret_cast<float*>(static_cast<char*>(ws) + row_bytes);
+
+    dim3 grid((unsigned)((nwaves + 3) / 4), tiles);
+    {
+        TimedLaunch t("spmm", s);
+#define ARMSP_SPLIT_V(VV)                                                                                          \
+    case VV:                                                                                                       \
+        st = merge ? launch_split_vu<TB, VV, true>(U, grid, s, M, N, nnz, chunk, rowptr, colind, val, B, ldb, bscale, \
+                                                   C, ldc, flags, crow, cval)                                      \
+                   : launch_split_vu<TB, VV, false>(U, grid, s, M, N, nnz, chunk, rowptr, colind, val, B, ldb,      \
+                                                    bscale, C, ldc, flags, crow, cval);                            \
+        break;
+        switch (V) {
+            ARMSP_SPLIT_V(1)
+            ARMSP_SPLIT_V(2)
+            ARMSP_SPLIT_V(4)
+            case 8:
+                if constexpr (sizeof(TB) < 4) {
+                    st = merge ? launch_split_vu<TB, 8, true>(U, grid, s, M, N, nnz, chunk, rowptr, colind, val, B, ldb,
+                                                              bscale, C, ldc, flags, crow, cval)
+                               : launch_split_vu<TB, 8, false>(U, grid, s, M, N, nnz, chunk, rowptr, colind, val, B,
+                                                               ldb, bscale, C, ldc, flags, crow, cval);
+                    break;
+                }
+                [[fallthrough]];
+            default: st = ARMSP_ERROR_INVALID_VALUE;
+        }
+#undef ARMSP_SPLIT_V
+        if (st) {
+            ws_free(ws, s);
+            return st;
+        }
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) {
+            ws_free(ws, s);
+            return hip_fail(e);
+        }
+        t.stop();
+    }
+    {
+        TimedLaunch t("spmm_fixup", s);
+        const int fv = (N % 4 == 0 && ldc % 4 == 0 && aligned(C, 16)) ? 4 : 1;
+        dim3 fgrid((unsigned)((nwaves + 3) / 4), 1);
+        if (fv == 4)
+            hipLaunchKernelGGL((k_carry_fixup<4>), fgrid, dim3(256), 0, s, (int)nwaves, N, crow, cval, C, ldc);
+        else
+            hipLaunchKernelGGL((k_carry_fixup<1>), fgrid, dim3(256), 0, s, (int)nwaves, N, crow, cval, C, ldc);
+        hipError_t e = hipGetLastError();
+        ws_free(ws, s);
+        if (e != hipSuccess) return hip_fail(e);
+        t.stop();
+    }
+    return ARMSP_SUCCESS;
+}
+
+}  // namespace
+}  // namespace armsp
+
+extern "C" {
+
+int armsp_spmm_f32(int M, int K, int N, int nnz, const int* rowptr, const int* colind, const float* val,
+                   const float* B, int ldb, float* C, int ldc, int flags, const armsp_config* cfg, void* stream) {
+    return armsp::spmm_impl<float>(M, K, N, nnz, rowptr, colind, val, B, nullptr, ldb, C, ldc, flags, cfg, stream);
+}
+
+int armsp_spmm_bf16(int M, int K, int N, int nnz, const int* rowptr, const int* colind, const float* val,
+                    const uint16_t* B, int ldb, float* C, int ldc, int flags, const armsp_config* cfg, void* stream) {
+    return armsp::spmm_impl<uint16_t>(M, K, N, nnz, rowptr, colind, val, B, nullptr, ldb, C, ldc, flags, cfg, stream);
+}
+
+int armsp_spmm_fp8(int M, int K, int N, int nnz, const int* rowptr, const int* colind, const float* val,
+                   const uint8_t* Bq, const float* b_row_scale, int ldb, float* C, int ldc, int flags,
+                   const armsp_config* cfg, void* stream) {
+    return armsp::spmm_impl<uint8_t>(M, K, N, nnz, rowptr, colind, val, Bq, b_row_scale, ldb, C, ldc, flags, cfg,
+                                     stream);
+}
+
+}  // extern "C"

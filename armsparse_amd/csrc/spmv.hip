@@ -1,0 +1,263 @@
+// CSR SpMV for gfx950.
+//
+// Reference: sparse_dense_vector (SpMV.cpp:155-179) runs an OpenMP loop over
+// rows of a zero-padded 1×4-blocked CSR; numaAware/numaAwareSpMV.cpp:102-125
+// splits the rows into four NUMA quarters.  Here:
+//  ROW   — G lanes per row, lanes stride the row, butterfly reduction.
+//  MERGE — merge-path over (row ends, nonzeros): each wave owns 64·IPT items,
+//          stages row ends and products val·x in LDS, every lane walks IPT
+//          consecutive items, a wave-level segmented scan joins rows cut
+//          between lanes, and a carry per wave is added by a fixup kernel.
+//  VCSR  — the reference's blocked format with its exact summation order:
+//          four lanes per row keep the four partial sums s0..s3 and the row
+//          total is (s0+s2)+(s1+s3), as vadd/vpadd compute it.
+#include <climits>
+
+#include "common.hpp"
+
+namespace armsp {
+namespace {
+
+__device__ __forceinline__ void wave_sync_lds() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+template <int G>
+__global__ __launch_bounds__(256) void k_spmv_row(int M, const int* __restrict__ rowptr, const int* __restrict__ colind,
+                                                  const float* __restrict__ val, const float* __restrict__ x,
+                                                  float* __restrict__ y, int flags) {
+    const int lane = threadIdx.x & (G - 1);
+    const int row = (int)((blockIdx.x * 256u + threadIdx.x) / G);
+    if (row >= M) return;
+    const int beg = rowptr[row], end = rowptr[row + 1];
+    float s = 0.0f;
+    for (int p = beg + lane; p < end; p += G) s = fmaf(val ? val[p] : 1.0f, x[colind[p]], s);
+#pragma unroll
+    for (int m = G / 2; m >= 1; m >>= 1) s += __shfl_xor(s, m, G);
+    if (lane == 0) y[row] = (flags & ARMSP_ACCUMULATE) ? y[row] + s : s;
+}
+
+template <int IPT>
+__global__ __launch_bounds__(256) void k_spmv_merge(int M, int nnz, const int* __restrict__ rowptr,
+                                                    const int* __restrict__ colind, const float* __restrict__ val,
+                                                    const float* __restrict__ x, float* __restrict__ y, int flags,
+                                                    int* __restrict__ carry_row, float* __restrict__ carry_val) {
+    constexpr int TILE = 64 * IPT;
+    __shared__ int s_re[4][TILE + 1];
+    __shared__ float s_prod[4][TILE];
+    __shared__ float s_out[4][TILE];
+    const int lane = threadIdx.x & 63;
+    const int wid = threadIdx.x >> 6;
+    const int wave = uniform((int)(blockIdx.x * 4u + wid));
+    const int64_t total = (int64_t)M + nnz;
+    const int64_t d0l = (int64_t)wave * TILE;
+    if (d0l >= total) return;
+    const int d0 = (int)d0l;
+    const int d1 = (int)(d0l + TILE < total ? d0l + TILE : total);
+    const int i0 = uniform(merge_path_search(d0, rowptr, M, nnz));
+    const int i1 = uniform(merge_path_search(d1, rowptr, M, nnz));
+    const int j0 = d0 - i0, j1 = d1 - i1;
+    const int nrows = i1 - i0, nz = j1 - j0;
+    int* re = s_re[wid];
+    float* prod = s_prod[wid];
+    float* out = s_out[wid];
+    for (int k = lane; k < nrows; k += 64) re[k] = rowptr[i0 + k + 1];
+    for (int k = lane; k < nz; k += 64) {
+        const int p = j0 + k;
+        prod[k] = (val ? val[p] : 1.0f) * x[colind[p]];
+    }
+    wave_sync_lds();
+
+    // this lane's start on the tile's merge path
+    const int items = nrows + nz;
+    const int ld = lane * IPT < items ? lane * IPT : items;
+    int lo = ld - nz > 0 ? ld - nz : 0, hi = ld < nrows ? ld : nrows;
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (re[mid] - j0 <= ld - mid - 1)
+            lo = mid + 1;
+        else
+            hi = mid;
+    }
+    int ri = lo, rj = ld - lo;
+    const int ri_start = ri;
+    float sum = 0.0f;
+#pragma unroll
+    for (int q = 0; q < IPT; ++q) {
+        if (ri + rj < items) {
+            if (ri < nrows && re[ri] - j0 <= rj) {
+                out[ri] = sum;
+                sum = 0.0f;
+                ++ri;
+            } else {
+                sum += prod[rj];
+                ++rj;
+            }
+        }
+    }
+    // segmented inclusive scan of (row, partial) across lanes; rows are non-decreasing in lane order
+    int key = ri;
+    float S = sum;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const float os = __shfl_up(S, d);
+        const int ok = __shfl_up(key, d);
+        if (lane >= d && ok == key) S += os;
+    }
+    const float prevS = __shfl_up(S, 1);
+    if (lane > 0 && ri > ri_start) out[ri_start] += prevS;
+    wave_sync_lds();
+    const bool acc = (flags & ARMSP_ACCUMULATE) != 0;
+    for (int k = lane; k < nrows; k += 64) y[i0 + k] = acc ? y[i0 + k] + out[k] : out[k];
+
+    const float tail = read_lane(S, 63);
+    if (lane == 0) {
+        bool has = false;
+        if (i1 < M) {
+            const int rs = rowptr[i1];
+            has = j1 > (rs > j0 ? rs : j0);
+        }
+        carry_row[wave] = has ? i1 : -1;
+        carry_val[wave] = tail;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_spmv_fixup(int nwaves, const int* __restrict__ carry_row,
+                                                    const float* __restrict__ carry_val, float* __restrict__ y) {
+    const int w = (int)(blockIdx.x * 256u + threadIdx.x);
+    if (w >= nwaves) return;
+    const int r = carry_row[w];
+    if (r < 0 || (w > 0 && carry_row[w - 1] == r)) return;
+    float s = 0.0f;
+    for (int k = w; k < nwaves && carry_row[k] == r; ++k) s += carry_val[k];
+    y[r] += s;
+}
+
+__global__ __launch_bounds__(256) void k_spmv_vcsr(int M, const int* __restrict__ rowptr,
+                                                   const int* __restrict__ colind, const float* __restrict__ val,
+                                                   const float* __restrict__ x, float* __restrict__ y, int fused) {
+    const int l = threadIdx.x & 3;
+    const int row = (int)((blockIdx.x * 256u + threadIdx.x) >> 2);
+    if (row >= M) return;
+    const int beg = rowptr[row], end = rowptr[row + 1];
+    float s = 0.0f;
+    if (fused) {
+        for (int idx = beg; idx < end; idx += 4) s = fmaf(val[idx + l], x[colind[idx] + l], s);
+    } else {
+        for (int idx = beg; idx < end; idx += 4) s = __fadd_rn(s, __fmul_rn(val[idx + l], x[colind[idx] + l]));
+    }
+    const float t = __fadd_rn(s, __shfl_xor(s, 2, 4));   // lanes 0/2: s0+s2, lanes 1/3: s1+s3
+    const float tot = __fadd_rn(t, __shfl_xor(t, 1, 4));  // (s0+s2)+(s1+s3)
+    if (l == 0) y[row] = __fadd_rn(y[row], tot);
+}
+
+template <int IPT>
+int launch_merge(int M, int nnz, const int* rp, const int* ci, const float* va, const float* x, float* y, int flags,
+                 hipStream_t s) {
+    constexpr int TILE = 64 * IPT;
+    const int64_t total = (int64_t)M + nnz;
+    const int64_t nwaves = (total + TILE - 1) / TILE;
+    if (nwaves > INT_MAX / 4) return ARMSP_ERROR_NOT_SUPPORTED;
+    void* ws = nullptr;
+    const size_t rb = ((size_t)nwaves * 4 + 255) & ~(size_t)255;
+    int st = ws_alloc(&ws, rb + (size_t)nwaves * 4, s);
+    if (st) return st;
+    int* crow = static_cast<int*>(ws);
+    float* cval = reinterpret_cast<float*>(static_cast<char*>(ws) + rb);
+    {
+        TimedLaunch t("spmv", s);
+        hipLaunchKernelGGL((k_spmv_merge<IPT>), dim3((unsigned)((nwaves + 3) / 4)), dim3(256), 0, s, M, nnz, rp, ci,
+                           va, x, y, flags, crow, cval);
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) {
+            ws_free(ws, s);
+            return hip_fail(e);
+        }
+        t.stop();
+    }
+    {
+        TimedLaunch t("spmv_fixup", s);
+        hipLaunchKernelGGL(k_spmv_fixup, dim3((unsigned)((nwaves + 255) / 256)), dim3(256), 0, s, (int)nwaves, crow,
+                           cval, y);
+        hipError_t e = hipGetLastError();
+        ws_free(ws, s);
+        if (e != hipSuccess) return hip_fail(e);
+        t.stop();
+    }
+    return ARMSP_SUCCESS;
+}
+
+}  // namespace
+}  // namespace armsp
+
+extern "C" {
+
+int armsp_spmv_f32(int M, int K, int nnz, const int* rowptr, const int* colind, const float* val, const float* x,
+                   float* y, int flags, const armsp_config* cfg, void* stream) {
+    using namespace armsp;
+    if (M < 0 || K < 0 || nnz < 0) return ARMSP_ERROR_INVALID_VALUE;
+    if (M == 0) return ARMSP_SUCCESS;
+    if (!rowptr || !y || (nnz > 0 && (!colind || !x))) return ARMSP_ERROR_INVALID_VALUE;
+    if (flags & ~ARMSP_ACCUMULATE) return ARMSP_ERROR_INVALID_VALUE;
+    if ((int64_t)M + nnz > INT_MAX) return ARMSP_ERROR_NOT_SUPPORTED;
+    hipStream_t s = as_stream(stream);
+    armsp_config c = cfg ? *cfg : armsp_config{0, 0, 0, 0, 0};
+    int sched = c.schedule == ARMSP_SCHED_AUTO ? ARMSP_SCHED_MERGE : c.schedule;
+    if (sched == ARMSP_SCHED_NNZ) sched = ARMSP_SCHED_MERGE;
+    if (sched == ARMSP_SCHED_ROW) {
+        int G = c.group;
+        if (G <= 0) {
+            const int avg = nnz / M;
+            G = 1;
+            while (G < 64 && G < avg) G <<= 1;
+        }
+        TimedLaunch t("spmv", s);
+        const dim3 grid((unsigned)(((int64_t)M * G + 255) / 256));
+        switch (G) {
+#define ARMSP_SPMV_ROW(GG)                                                                                          \
+    case GG:                                                                                                        \
+        hipLaunchKernelGGL((k_spmv_row<GG>), grid, dim3(256), 0, s, M, rowptr, colind, val, x, y, flags); \
+        break;
+            ARMSP_SPMV_ROW(1)
+            ARMSP_SPMV_ROW(2)
+            ARMSP_SPMV_ROW(4)
+            ARMSP_SPMV_ROW(8)
+            ARMSP_SPMV_ROW(16)
+            ARMSP_SPMV_ROW(32)
+            ARMSP_SPMV_ROW(64)
+#undef ARMSP_SPMV_ROW
+            default: return ARMSP_ERROR_INVALID_VALUE;
+        }
+        ARMSP_LAUNCHED();
+        t.stop();
+        return ARMSP_SUCCESS;
+    }
+    if (sched != ARMSP_SCHED_MERGE) return ARMSP_ERROR_INVALID_VALUE;
+    const int ipt = c.chunk > 0 ? c.chunk / 64 : 8;
+    switch (ipt) {
+        case 4: return launch_merge<4>(M, nnz, rowptr, colind, val, x, y, flags, s);
+        case 8: return launch_merge<8>(M, nnz, rowptr, colind, val, x, y, flags, s);
+        case 16: return launch_merge<16>(M, nnz, rowptr, colind, val, x, y, flags, s);
+        default: return ARMSP_ERROR_INVALID_VALUE;
+    }
+}
+
+int armsp_spmv_vcsr_f32(int M, int nnz_padded, const int* rowptr, const int* colind, const float* val, const float* x,
+                        float* y, int flags, void* stream) {
+    using namespace armsp;
+    if (M < 0 || nnz_padded < 0 || (nnz_padded & 3)) return ARMSP_ERROR_INVALID_VALUE;
+    if (M == 0) return ARMSP_SUCCESS;
+    if (!rowptr || !y || (nnz_padded > 0 && (!colind || !val || !x))) return ARMSP_ERROR_INVALID_VALUE;
+    if (flags & ~(ARMSP_ACCUMULATE | ARMSP_FUSED_MLA)) return ARMSP_ERROR_INVALID_VALUE;
+    hipStream_t s = as_stream(stream);
+    TimedLaunch t("spmv_vcsr", s);
+    hipLaunchKernelGGL(k_spmv_vcsr, dim3((unsigned)(((int64_t)M * 4 + 255) / 256)), dim3(256), 0, s, M, rowptr,
+                       colind, val, x, y, (flags & ARMSP_FUSED_MLA) ? 1 : 0);
+    ARMSP_LAUNCHED();
+    t.stop();
+    return ARMSP_SUCCESS;
+}
+
+}  // extern "C"

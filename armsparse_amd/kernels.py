@@ -1,0 +1,308 @@
+"""Torch-tensor front end of libarmsparse: every operator is a HIP kernel of
+the library launched on torch's current stream through the C ABI.
+
+PyTorch is used only for device memory and streams.  Tensors on the CPU are
+staged to the current ROCm device and the result is copied back (the same
+contract as the reference's CPU op, gcn/pytorch-custom/spmm.cpp:31-47);
+without a ROCm device every operator raises.
+"""
+from __future__ import annotations
+
+import ctypes
+from typing import Callable, Optional, Union
+
+import torch
+
+from . import _lib
+from ._lib import Config, call
+
+ConfigLike = Union[None, str, dict, Config]
+
+# Optional schedule selector: f(op, rowptr(host-or-device), n_rows, nnz, n_dense) -> Config | None.
+_selector: Optional[Callable[..., Optional[Config]]] = None
+
+
+def set_selector(fn: Optional[Callable[..., Optional[Config]]]) -> None:
+    """Install (or clear) the automatic schedule selector used when config is None."""
+    global _selector
+    _selector = fn
+
+
+def _config(cfg: ConfigLike, op: str, rowptr, m: int, nnz: int, n: int):
+    if cfg is None and _selector is not None:
+        cfg = _selector(op, rowptr, m, nnz, n)
+    if cfg is None:
+        return None
+    if isinstance(cfg, Config):
+        return ctypes.byref(cfg)
+    if isinstance(cfg, str):
+        return ctypes.byref(Config.make(cfg))
+    if isinstance(cfg, dict):
+        return ctypes.byref(Config.make(**cfg))
+    raise TypeError(f"config must be None, a schedule name, a dict or a Config, got {type(cfg)}")
+
+
+def _require(cond: bool, msg: str) -> None:
+    if not cond:
+        raise RuntimeError(msg)
+
+
+def _device() -> torch.device:
+    _require(torch.cuda.is_available(), "armsparse_amd: no ROCm device available (no CPU fallback)")
+    return torch.device("cuda", torch.cuda.current_device())
+
+
+def _on_device(t: Optional[torch.Tensor], dev: torch.device) -> Optional[torch.Tensor]:
+    if t is None:
+        return None
+    return t if t.device == dev else t.to(dev)
+
+
+def _ptr(t: Optional[torch.Tensor]):
+    return ctypes.c_void_p(t.data_ptr()) if t is not None else None
+
+
+def _stream(dev: torch.device):
+    return ctypes.c_void_p(torch.cuda.current_stream(dev).cuda_stream)
+
+
+def _check_csr(fn: str, rowptr: torch.Tensor, colind: torch.Tensor, values: Optional[torch.Tensor]) -> None:
+    _require(rowptr.dtype == torch.int32, f"{fn}: rowptr must be int32")
+    _require(colind.dtype == torch.int32, f"{fn}: colind must be int32")
+    _require(rowptr.dim() == 1 and rowptr.numel() >= 1, f"{fn}: rowptr must be 1-D with M+1 entries")
+    _require(rowptr.is_contiguous() and colind.is_contiguous(), f"{fn}: index arrays must be contiguous")
+    if values is not None:
+        _require(values.dtype == torch.float32, f"{fn}: values must be float32")
+        _require(values.is_contiguous(), f"{fn}: values must be contiguous")
+        _require(values.numel() == colind.numel(), f"{fn}: values and colind differ in length")
+
+
+# ----------------------------------------------------------------- SpMM
+def spmm(rowptr: torch.Tensor, colind: torch.Tensor, values: Optional[torch.Tensor], dense: torch.Tensor, *,
+         dense_scale: Optional[torch.Tensor] = None, out: Optional[torch.Tensor] = None, accumulate: bool = False,
+         trunc4: bool = False, ncols: Optional[int] = None, config: ConfigLike = None) -> torch.Tensor:
+    """out[M×N] (=, or += with accumulate) A·dense.
+
+    dense: float32, bfloat16, or fp8-e4m3 (torch.float8_e4m3fn or raw uint8
+    bytes) with ``dense_scale`` holding one float32 scale per row.  values=None
+    means all-ones edge values.  ``ncols`` restricts the update to the first
+    ncols columns (the reference's ``irregular``/``rowBaseSpmm`` tails).
+    """
+    fn = "spmm"
+    _check_csr(fn, rowptr, colind, values)
+    _require(dense.dim() == 2, f"{fn}: dense must be 2-D")
+    _require(dense.stride(1) == 1, f"{fn}: dense rows must be contiguous")
+    m = rowptr.numel() - 1
+    k, n = dense.shape
+    nnz = colind.numel()
+    host = not dense.is_cuda
+    dev = _device() if host else dense.device
+    rp, ci, va = _on_device(rowptr, dev), _on_device(colind, dev), _on_device(values, dev)
+    d = _on_device(dense, dev)
+    if out is None:
+        _require(not accumulate, f"{fn}: accumulate needs an output tensor")
+        c = torch.empty((m, n), dtype=torch.float32, device=dev)
+    else:
+        _require(out.shape == (m, n) and out.dtype == torch.float32, f"{fn}: out must be float32 [M, N]")
+        _require(out.stride(1) == 1, f"{fn}: out rows must be contiguous")
+        c = _on_device(out, dev)
+    flags = (_lib.ACCUMULATE if accumulate else 0) | (_lib.TRUNC4 if trunc4 else 0)
+    nc = n if ncols is None else int(ncols)
+    _require(0 <= nc <= n, f"{fn}: ncols out of range")
+    cfg = _config(config, "spmm", rowptr, m, nnz, nc)
+    s = _stream(dev)
+    if d.dtype == torch.float32:
+        call("armsp_spmm_f32", m, k, nc, nnz, _ptr(rp), _ptr(ci), _ptr(va), _ptr(d), d.stride(0), _ptr(c),
+             c.stride(0), flags, cfg, s)
+    elif d.dtype == torch.bfloat16:
+        call("armsp_spmm_bf16", m, k, nc, nnz, _ptr(rp), _ptr(ci), _ptr(va), _ptr(d), d.stride(0), _ptr(c),
+             c.stride(0), flags, cfg, s)
+    elif d.dtype in (torch.uint8, torch.float8_e4m3fn):
+        _require(dense_scale is not None and dense_scale.dtype == torch.float32 and dense_scale.numel() == k,
+                 f"{fn}: fp8 dense needs a float32 per-row scale of length K")
+        sc = _on_device(dense_scale.contiguous(), dev)
+        call("armsp_spmm_fp8", m, k, nc, nnz, _ptr(rp), _ptr(ci), _ptr(va), _ptr(d), _ptr(sc), d.stride(0),
+             _ptr(c), c.stride(0), flags, cfg, s)
+    else:
+        raise RuntimeError(f"{fn}: unsupported dense dtype {d.dtype}")
+    if host:
+        c = c.cpu()
+        if out is not None:
+            out.copy_(c)
+            return out
+    return c
+
+
+# ----------------------------------------------------------------- SpMV
+def spmv(rowptr: torch.Tensor, colind: torch.Tensor, values: Optional[torch.Tensor], x: torch.Tensor, *,
+         out: Optional[torch.Tensor] = None, accumulate: bool = False, config: ConfigLike = None) -> torch.Tensor:
+    """y[M] (=, or += with accumulate) A·x on plain CSR."""
+    fn = "spmv"
+    _check_csr(fn, rowptr, colind, values)
+    _require(x.dim() == 1 and x.dtype == torch.float32 and x.is_contiguous(), f"{fn}: x must be contiguous float32")
+    m, nnz = rowptr.numel() - 1, colind.numel()
+    host = not x.is_cuda
+    dev = _device() if host else x.device
+    rp, ci, va, xd = (_on_device(t, dev) for t in (rowptr, colind, values, x))
+    if out is None:
+        _require(not accumulate, f"{fn}: accumulate needs an output tensor")
+        y = torch.empty(m, dtype=torch.float32, device=dev)
+    else:
+        _require(out.shape == (m,) and out.dtype == torch.float32 and out.is_contiguous(), f"{fn}: bad out")
+        y = _on_device(out, dev)
+    cfg = _config(config, "spmv", rowptr, m, nnz, 1)
+    call("armsp_spmv_f32", m, x.numel(), nnz, _ptr(rp), _ptr(ci), _ptr(va), _ptr(xd), _ptr(y),
+         _lib.ACCUMULATE if accumulate else 0, cfg, _stream(dev))
+    if host:
+        y = y.cpu()
+        if out is not None:
+            out.copy_(y)
+            return out
+    return y
+
+
+def vcsr_spmv(rowptr: torch.Tensor, colind: torch.Tensor, values: torch.Tensor, x: torch.Tensor, y: torch.Tensor, *,
+              fused: bool = False) -> torch.Tensor:
+    """y += A·x on the zero-padded 1×4-blocked CSR (sparse_dense_vector, SpMV.cpp:155-179). In place."""
+    fn = "vcsr_spmv"
+    _check_csr(fn, rowptr, colind, values)
+    _require(values is not None, f"{fn}: values required")
+    _require(x.dtype == torch.float32 and y.dtype == torch.float32 and x.is_contiguous() and y.is_contiguous(),
+             f"{fn}: x and y must be contiguous float32")
+    m, nnz = rowptr.numel() - 1, colind.numel()
+    _require(y.numel() == m, f"{fn}: y must have M entries")
+    host = not y.is_cuda
+    dev = _device() if host else y.device
+    rp, ci, va, xd, yd = (_on_device(t, dev) for t in (rowptr, colind, values, x, y))
+    call("armsp_spmv_vcsr_f32", m, nnz, _ptr(rp), _ptr(ci), _ptr(va), _ptr(xd), _ptr(yd),
+         _lib.FUSED_MLA if fused else 0, _stream(dev))
+    if host:
+        y.copy_(yd.cpu())
+    return y
+
+
+# ---------------------------------------------------------------- SDDMM
+def sddmm(rows: torch.Tensor, colind: torch.Tensor, U: torch.Tensor, V: torch.Tensor, *,
+          scale: Optional[torch.Tensor] = None, coo: bool = False, config: ConfigLike = None) -> torch.Tensor:
+    """out[e] = scale[e]·<U[row(e)], V[colind[e]]>; rows is rowptr (CSR) or rowind (COO)."""
+    fn = "sddmm"
+    _require(rows.dtype == torch.int32 and colind.dtype == torch.int32, f"{fn}: indices must be int32")
+    _require(rows.is_contiguous() and colind.is_contiguous(), f"{fn}: indices must be contiguous")
+    _require(U.dim() == 2 and V.dim() == 2 and U.shape[1] == V.shape[1], f"{fn}: U, V must be 2-D with equal width")
+    _require(U.dtype == V.dtype and U.dtype in (torch.float32, torch.bfloat16), f"{fn}: U, V must be float32 or bf16")
+    _require(U.stride(1) == 1 and V.stride(1) == 1, f"{fn}: U, V rows must be contiguous")
+    nnz, kd = colind.numel(), U.shape[1]
+    m = U.shape[0]
+    if coo:
+        _require(rows.numel() == nnz, f"{fn}: rowind and colind differ in length")
+    else:
+        _require(rows.numel() == m + 1, f"{fn}: rowptr must have U.shape[0]+1 entries")
+    if scale is not None:
+        _require(scale.dtype == torch.float32 and scale.numel() == nnz and scale.is_contiguous(), f"{fn}: bad scale")
+    host = not U.is_cuda
+    dev = _device() if host else U.device
+    rd, ci, ud, vd, sd = (_on_device(t, dev) for t in (rows, colind, U, V, scale))
+    out = torch.empty(nnz, dtype=torch.float32, device=dev)
+    s = _stream(dev)
+    if coo:
+        _require(U.dtype == torch.float32, f"{fn}: COO form takes float32")
+        call("armsp_sddmm_coo_f32", m, V.shape[0], kd, nnz, _ptr(rd), _ptr(ci), _ptr(sd), _ptr(ud), ud.stride(0),
+             _ptr(vd), vd.stride(0), _ptr(out), s)
+    else:
+        cfg = _config(config, "sddmm", rows, m, nnz, kd)
+        name = "armsp_sddmm_csr_f32" if U.dtype == torch.float32 else "armsp_sddmm_csr_bf16"
+        call(name, m, V.shape[0], kd, nnz, _ptr(rd), _ptr(ci), _ptr(sd), _ptr(ud), ud.stride(0), _ptr(vd),
+             vd.stride(0), _ptr(out), cfg, s)
+    return out.cpu() if host else out
+
+
+# ------------------------------------------------------------ SpGEMM
+def spgemm_dense(a: tuple, b: tuple, c: torch.Tensor) -> torch.Tensor:
+    """c[M×N] += A·B with A = (rowptr, colind, values) M×K and B K×N in CSR. In place."""
+    fn = "spgemm_dense"
+    (rp1, ci1, v1), (rp2, ci2, v2) = a, b
+    _check_csr(fn, rp1, ci1, v1)
+    _check_csr(fn, rp2, ci2, v2)
+    _require(c.dim() == 2 and c.dtype == torch.float32 and c.stride(1) == 1, f"{fn}: c must be float32 2-D")
+    m, n = c.shape
+    k = rp2.numel() - 1
+    _require(rp1.numel() == m + 1, f"{fn}: A has {rp1.numel() - 1} rows, c has {m}")
+    host = not c.is_cuda
+    dev = _device() if host else c.device
+    t = [_on_device(x, dev) for x in (rp1, ci1, v1, rp2, ci2, v2, c)]
+    call("armsp_spgemm_dense_f32", m, k, n, ci1.numel(), *(_ptr(x) for x in t[:3]), ci2.numel(),
+         *(_ptr(x) for x in t[3:6]), _ptr(t[6]), t[6].stride(0), _stream(dev))
+    if host:
+        c.copy_(t[6].cpu())
+    return c
+
+
+# --------------------------------------------------------- transpose
+def csr_transpose(rowptr: torch.Tensor, colind: torch.Tensor, values: Optional[torch.Tensor], num_cols: int):
+    """(colptr, rowind, values_t): CSC of A, stable (rows ascend within a column)."""
+    fn = "csr_transpose"
+    _check_csr(fn, rowptr, colind, values)
+    m, nnz = rowptr.numel() - 1, colind.numel()
+    host = not rowptr.is_cuda
+    dev = _device() if host else rowptr.device
+    rp, ci, va = (_on_device(t, dev) for t in (rowptr, colind, values))
+    colptr = torch.empty(num_cols + 1, dtype=torch.int32, device=dev)
+    rowind = torch.empty(nnz, dtype=torch.int32, device=dev)
+    vt = torch.empty(nnz, dtype=torch.float32, device=dev) if values is not None else None
+    call("armsp_csr_transpose", m, num_cols, nnz, _ptr(rp), _ptr(ci), _ptr(va), _ptr(colptr), _ptr(rowind), _ptr(vt),
+         _stream(dev))
+    res = (colptr, rowind, vt)
+    return tuple(None if x is None else x.cpu() for x in res) if host else res
+
+
+# ------------------------------------------------------- dense helpers
+def gemm_bf16(a: torch.Tensor, b: torch.Tensor, *, out: Optional[torch.Tensor] = None,
+              accumulate: bool = False) -> torch.Tensor:
+    """out (=|+=) a @ b for 2-D bfloat16 tensors of any strides, fp32 result (MFMA)."""
+    fn = "gemm_bf16"
+    _require(a.dim() == 2 and b.dim() == 2 and a.shape[1] == b.shape[0], f"{fn}: shape mismatch")
+    _require(a.dtype == torch.bfloat16 and b.dtype == torch.bfloat16, f"{fn}: operands must be bfloat16")
+    _require(a.is_cuda and b.is_cuda, f"{fn}: operands must be device tensors")
+    m, kd = a.shape
+    n = b.shape[1]
+    if out is None:
+        _require(not accumulate, f"{fn}: accumulate needs an output tensor")
+        out = torch.empty((m, n), dtype=torch.float32, device=a.device)
+    _require(out.dtype == torch.float32 and out.shape == (m, n) and out.stride(1) == 1, f"{fn}: bad out")
+    call("armsp_gemm_bf16", m, n, kd, _ptr(a), a.stride(0), a.stride(1), _ptr(b), b.stride(0), b.stride(1),
+         _ptr(out), out.stride(0), 1 if accumulate else 0, _stream(a.device))
+    return out
+
+
+def to_bf16(x: torch.Tensor) -> torch.Tensor:
+    """float32 → bfloat16 (round to nearest even) with the library kernel."""
+    _require(x.is_cuda and x.dtype == torch.float32, "to_bf16: needs a float32 device tensor")
+    x = x.contiguous()
+    y = torch.empty(x.shape, dtype=torch.bfloat16, device=x.device)
+    call("armsp_convert_f32_to_bf16", x.numel(), _ptr(x), _ptr(y), _stream(x.device))
+    return y
+
+
+def quantize_fp8_rows(b: torch.Tensor):
+    """(bytes uint8 [K×N], scale float32 [K]) — per-row OCP e4m3 quantisation."""
+    _require(b.is_cuda and b.dtype == torch.float32 and b.dim() == 2 and b.stride(1) == 1,
+             "quantize_fp8_rows: needs a float32 device matrix with contiguous rows")
+    k, n = b.shape
+    q = torch.empty((k, n), dtype=torch.uint8, device=b.device)
+    sc = torch.empty(k, dtype=torch.float32, device=b.device)
+    if k and n:
+        _require(b.stride(0) == n, "quantize_fp8_rows: rows must be packed")
+        call("armsp_quantize_fp8_rows", k, n, _ptr(b), b.stride(0), _ptr(q), _ptr(sc), _stream(b.device))
+    return q, sc
+
+
+def fill_uniform_(t: torch.Tensor, seed: int, stream_id: int, lo: float = -1.0, hi: float = 1.0) -> torch.Tensor:
+    _require(t.is_cuda and t.dtype == torch.float32 and t.is_contiguous(), "fill_uniform_: float32 device tensor")
+    call("armsp_fill_uniform_f32", t.numel(), seed, stream_id, lo, hi, _ptr(t), _stream(t.device))
+    return t
+
+
+def fill_normal_(t: torch.Tensor, seed: int, stream_id: int) -> torch.Tensor:
+    _require(t.is_cuda and t.dtype == torch.float32 and t.is_contiguous(), "fill_normal_: float32 device tensor")
+    call("armsp_fill_normal_f32", t.numel(), seed, stream_id, _ptr(t), _stream(t.device))
+    return t

@@ -1,0 +1,155 @@
+"""Drop-in for gcn/pytorch-custom/op.py: ``SPMMFunction`` and ``GCNConv``
+with the reference's signatures, running on libarmsparse.
+
+Differences from the reference, all deliberate:
+- the forward output is zero-initialised (spmm.cpp:41 leaves it undefined);
+- backward with value-less edges computes Aᵀ·G with all-ones values (the
+  reference returns a stale global, op.py:21,32);
+- the edge-weight gradient, returned as None by the reference (op.py:31),
+  is computed by SDDMM when the CSR edge weights require grad;
+- no "[I] Treat edge weight as no_grad." print per backward call;
+- ``precision="bf16"`` (extension) stores the dense features in bfloat16:
+  x·W runs on the MFMA GEMM and the aggregation reads bf16 features,
+  accumulating in fp32.
+"""
+from __future__ import annotations
+
+import math
+from typing import Optional
+
+import torch
+from torch.nn import Parameter
+
+from . import kernels
+
+
+def glorot(t: Optional[torch.Tensor]) -> None:
+    """Uniform(-a, a), a = sqrt(6 / (fan_in + fan_out)) — the init op.py:98 takes from PyG."""
+    if t is not None:
+        a = math.sqrt(6.0 / (t.size(-2) + t.size(-1)))
+        with torch.no_grad():
+            t.uniform_(-a, a)
+
+
+def zeros(t: Optional[torch.Tensor]) -> None:
+    if t is not None:
+        with torch.no_grad():
+            t.zero_()
+
+
+class SPMMFunction(torch.autograd.Function):
+    """out = A·feat with A given by (rowptr, colind, edge_weight_csr); backward
+    dfeat = Aᵀ·grad via (colptr, rowind, edge_weight_csc) (op.py:8-37)."""
+
+    @staticmethod
+    def forward(ctx, rowptr, colind, colptr, rowind, feat, edge_weight_csr=None, edge_weight_csc=None):
+        out = kernels.spmm(rowptr, colind, edge_weight_csr, feat)
+        ctx.backward_csc = (rowptr, colind, colptr, rowind, feat, edge_weight_csr, edge_weight_csc)
+        return out
+
+    @staticmethod
+    def backward(ctx, grad_out):
+        rowptr, colind, colptr, rowind, feat, ew_csr, ew_csc = ctx.backward_csc
+        if ew_csr is not None and ew_csc is None:
+            raise RuntimeError(
+                "Backward of SPMM require edge values in both src-first and dst-first order. "
+                "Call with SPMMFunction.apply(rowptr, colind, colptr, rowind, in_feat, "
+                "edge_value_row_first, edge_value_col_first)")
+        grad = grad_out.contiguous()
+        grad_feat = grad_ew = None
+        if ctx.needs_input_grad[4]:
+            g = grad if feat.dtype == torch.float32 else grad.to(feat.dtype)
+            grad_feat = kernels.spmm(colptr, rowind, ew_csc, g).to(feat.dtype)
+        if ctx.needs_input_grad[5] and ew_csr is not None:
+            x = feat if feat.dtype == torch.float32 else feat.float()
+            grad_ew = kernels.sddmm(rowptr, colind, grad.float(), x.contiguous())
+        return None, None, None, None, grad_feat, grad_ew, None
+
+
+class _BF16Linear(torch.autograd.Function):
+    """y = x·W with x bfloat16 [M×F] and W float32 master weights [F×H]; MFMA GEMM, fp32 y."""
+
+    @staticmethod
+    def forward(ctx, x, weight):
+        wb = kernels.to_bf16(weight.detach())
+        ctx.save_for_backward(x, wb)
+        return kernels.gemm_bf16(x, wb)
+
+    @staticmethod
+    def backward(ctx, gy):
+        x, wb = ctx.saved_tensors
+        gyb = kernels.to_bf16(gy.contiguous())
+        gx = gw = None
+        if ctx.needs_input_grad[0]:
+            gx = kernels.gemm_bf16(gyb, wb.t()).to(x.dtype)  # [M×H]·[H×F]
+        if ctx.needs_input_grad[1]:
+            gw = kernels.gemm_bf16(x.t(), gyb)  # [F×M]·[M×H]
+        return gx, gw
+
+
+class GCNConv(torch.nn.Module):
+    """GCN layer of gcn/pytorch-custom/op.py:78-153: x·W, scale by
+    1/sqrt(out-degree) (counts from colptr), aggregate with SPMMFunction,
+    scale by 1/sqrt(in-degree) (counts from rowptr), add bias."""
+
+    def __init__(self, in_channels, out_channels, improved=False, cached=False, bias=True, normalize=True,
+                 precision: str = "fp32", **kwargs):
+        super().__init__()
+        if precision not in ("fp32", "bf16"):
+            raise ValueError("precision must be 'fp32' or 'bf16'")
+        self.in_channels = in_channels
+        self.out_channels = out_channels
+        self.improved = improved
+        self.cached = cached
+        self.normalize = normalize
+        self.precision = precision
+        self.weight = Parameter(torch.empty(in_channels, out_channels))
+        if bias:
+            self.bias = Parameter(torch.empty(out_channels))
+        else:
+            self.register_parameter("bias", None)
+        self.reset_parameters()
+
+    def reset_parameters(self):
+        glorot(self.weight)
+        zeros(self.bias)
+        self.cached_result = None
+        self.cached_num_edges = None
+
+    @staticmethod
+    def in_deg_sqrt(indptr):
+        return (1 / torch.sqrt((indptr[1:] - indptr[:-1]).float())).unsqueeze(dim=1)
+
+    @staticmethod
+    def out_deg_sqrt(indptr):
+        return (1 / torch.sqrt((indptr[1:] - indptr[:-1]).float())).unsqueeze(dim=1)
+
+    def forward(self, x, rowptr, colind, colptr, rowind, edge_weight_csr=None, edge_weight_csc=None):
+        if self.precision == "bf16":
+            xb = x if x.dtype == torch.bfloat16 else x.to(torch.bfloat16)
+            x = _BF16Linear.apply(xb.contiguous(), self.weight)
+        else:
+            x = torch.matmul(x, self.weight)
+        if not self.cached or self.cached_result is None:
+            if self.normalize:
+                in_deg_norm = self.in_deg_sqrt(rowptr)
+                out_deg_norm = self.out_deg_sqrt(colptr)
+            else:
+                in_deg_norm = torch.ones(rowptr.shape[0] - 1, 1, dtype=x.dtype, device=x.device)
+                out_deg_norm = torch.ones(colptr.shape[0] - 1, 1, dtype=x.dtype, device=x.device)
+            self.cached_result = in_deg_norm, out_deg_norm
+        in_deg_norm, out_deg_norm = self.cached_result
+        if self.normalize:
+            x = x * out_deg_norm
+        if self.precision == "bf16":
+            x = x.to(torch.bfloat16)
+        aggr_out = SPMMFunction.apply(rowptr, colind, colptr, rowind, x.contiguous(), edge_weight_csr,
+                                      edge_weight_csc)
+        if self.normalize:
+            aggr_out = aggr_out * in_deg_norm
+        if self.bias is not None:
+            aggr_out = aggr_out + self.bias
+        return aggr_out
+
+    def __repr__(self):
+        return "{}({}, {})".format(self.__class__.__name__, self.in_channels, self.out_channels)

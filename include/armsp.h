@@ -1,0 +1,190 @@
+/*
+ * armsp.h — C ABI of libarmsparse.so, the MI355X (gfx950) sparse-kernel library.
+ *
+ * Every entry point takes plain device pointers, plain sizes and a HIP stream
+ * (passed as `void*`, NULL = the null stream), launches asynchronously on that
+ * stream and returns an armsp_status.  No torch or HIP types appear in the
+ * signatures.  Index arrays are int32 CSR/COO as in the reference; row-major
+ * dense operands carry a leading dimension.
+ *
+ * Which reference interface each entry point replaces is stated beside it
+ * (paths are relative to the ARMSparse source tree).  The C++-linkage drop-ins
+ * with the reference's own names and host-pointer signatures are declared in
+ * armsparse_compat.h; they are thin shims over these functions.
+ *
+ * Workspace: entry points whose schedule needs a carry buffer (nnz-split and
+ * merge-path SpMM / SpMV, split-K GEMM, transpose) take it from the device's
+ * stream-ordered memory pool on the caller's stream (hipMallocAsync /
+ * hipFreeAsync), so concurrent calls on different streams never share it.
+ */
+#ifndef ARMSP_H
+#define ARMSP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ARMSP_VERSION 100 /* 1.0.0 */
+
+typedef enum armsp_status {
+    ARMSP_SUCCESS = 0,
+    ARMSP_ERROR_INVALID_VALUE = 1, /* bad size, null pointer, malformed CSR metadata */
+    ARMSP_ERROR_NOT_SUPPORTED = 2, /* valid request this build has no kernel for */
+    ARMSP_ERROR_HIP = 3,           /* a HIP runtime call failed (see armsp_last_hip_error) */
+    ARMSP_ERROR_ALLOC = 4,         /* device workspace allocation failed */
+    ARMSP_ERROR_NO_DEVICE = 5      /* no usable gfx950 device */
+} armsp_status;
+
+/* Work decomposition of a sparse kernel. */
+typedef enum armsp_schedule {
+    ARMSP_SCHED_AUTO = 0,  /* library heuristic (or the learned selector, set by the host layer) */
+    ARMSP_SCHED_ROW = 1,   /* row-split: a group of `group` lanes owns one row, CSR order kept */
+    ARMSP_SCHED_NNZ = 2,   /* nnz-split: each wave owns `chunk` nonzeros, carries fixed up after */
+    ARMSP_SCHED_MERGE = 3  /* merge-path: each wave owns `chunk` items of (rows + nnz), carries fixed up */
+} armsp_schedule;
+
+/* Schedule and tile shape.  Zero fields mean "choose automatically". */
+typedef struct armsp_config {
+    int32_t schedule; /* armsp_schedule */
+    int32_t group;    /* lanes per row (ROW) or per nonzero (SDDMM): 1,2,4,...,64 */
+    int32_t vec;      /* dense elements per lane: 1, 2, 4 (8 for bf16/fp8 B) */
+    int32_t chunk;    /* items per wave for NNZ / MERGE (multiple of 64) */
+    int32_t unroll;   /* nonzeros whose dense rows are fetched together: 1,2,4,8,16 */
+} armsp_config;
+
+/* flags for the SpMM / SpMV entry points */
+#define ARMSP_ACCUMULATE 0x1   /* C += A·B (reference rowBaseSpmm2/3, csr_spmm_kernel); else C = A·B (origin) */
+#define ARMSP_TRUNC4     0x2   /* rowBaseSpmm semantics: only the first 4*floor(nnz_row/4) nonzeros of each row */
+#define ARMSP_FUSED_MLA  0x4   /* VCSR SpMV: fused multiply-add instead of the unfused vmlaq_f32 reading */
+
+/* ---------------------------------------------------------------- runtime */
+int armsp_version(void);
+const char* armsp_status_string(int status);
+/* Last HIP error code seen by the library on this thread (0 if none). */
+int armsp_last_hip_error(void);
+/* 1 if a gfx950 device is visible, else 0.  Never fails. */
+int armsp_device_available(void);
+/* Kernel timing: when enabled, every main kernel launch is bracketed by HIP
+ * events on its own stream; totals are read per kernel family. */
+int armsp_timing_enable(int on);
+int armsp_timing_reset(void);
+/* Synchronises the recorded events.  family: "spmm", "spmm_fixup", "spmv",
+ * "spmv_fixup", "sddmm", "spgemm", "gemm", "transpose", ... */
+int armsp_timing_read(const char* family, double* total_ms, int64_t* launches);
+
+/* ------------------------------------------------------------------- SpMM */
+/* C[M×N] (=|+=) A[M×K] · B[K×N].  A: CSR rowptr[M+1], colind[nnz], val[nnz]
+ * (val may be NULL: all ones — the value-less aggregation of
+ * gcn/pytorch-custom/spmm_kernel.cu:31-207).  B row-major with ldb, C with ldc.
+ * Replaces rowBaseSpmm3/rowBaseSpmm2/rowWiseBlocking/rowWiseRegisterBlocking
+ * (SDMM.cpp:65-160) with flags=ACCUMULATE, origin (SDMM.cpp:422-448) with
+ * flags=0, rowBaseSpmm (SDMM.cpp:21-64) with ACCUMULATE|TRUNC4 and N
+ * rounded down to a multiple of 4, irregular (SDMM.cpp:161-180) with N
+ * rounded down to a multiple of 32, and csr_spmm_kernel
+ * (gcn/pytorch-custom/spmm.cpp:49-71) on a zeroed C.  ROW schedule keeps the
+ * reference's per-element fmaf order (bit-exact); NNZ/MERGE re-associate
+ * across the carry boundary. */
+int armsp_spmm_f32(int M, int K, int N, int nnz,
+                   const int* rowptr, const int* colind, const float* val,
+                   const float* B, int ldb, float* C, int ldc,
+                   int flags, const armsp_config* cfg, void* stream);
+
+/* Same with B stored as bfloat16 bits; fp32 accumulate, fp32 C.  No
+ * reference counterpart (the reference is fp32 only, spmm.cpp:112). */
+int armsp_spmm_bf16(int M, int K, int N, int nnz,
+                    const int* rowptr, const int* colind, const float* val,
+                    const uint16_t* B, int ldb, float* C, int ldc,
+                    int flags, const armsp_config* cfg, void* stream);
+
+/* Same with B stored as OCP fp8-e4m3 bytes and one fp32 scale per row of B:
+ * B[k][n] = scale[k] * e4m3(Bq[k][n]).  No reference counterpart. */
+int armsp_spmm_fp8(int M, int K, int N, int nnz,
+                   const int* rowptr, const int* colind, const float* val,
+                   const uint8_t* Bq, const float* b_row_scale, int ldb,
+                   float* C, int ldc, int flags, const armsp_config* cfg, void* stream);
+
+/* ------------------------------------------------------------------- SpMV */
+/* y[M] (=|+=) A·x on plain CSR.  Replaces the row split of
+ * numaAware/numaAwareSpMV.cpp:102-125 (one launch, no per-node copies). */
+int armsp_spmv_f32(int M, int K, int nnz,
+                   const int* rowptr, const int* colind, const float* val,
+                   const float* x, float* y, int flags, const armsp_config* cfg, void* stream);
+
+/* y[M] += A·x on the zero-padded 1×4-blocked "vector-aware" CSR built by
+ * toVectorAwareCSR (SpMV.cpp:15-38): every row holds whole aligned 4-blocks,
+ * colind[idx] of a block start is a multiple of 4.  Replaces
+ * sparse_dense_vector (SpMV.cpp:155-179): per row four partial sums
+ * s_l += val*x (unfused, or fused with ARMSP_FUSED_MLA) reduced as
+ * (s0+s2)+(s1+s3), bit-exact with the reference's order.  x must hold
+ * K rounded up to a multiple of 4 entries. */
+int armsp_spmv_vcsr_f32(int M, int nnz_padded,
+                        const int* rowptr, const int* colind, const float* val,
+                        const float* x, float* y, int flags, void* stream);
+
+/* ------------------------------------------------------------------ SDDMM */
+/* out[e] = s_e · <U[row(e),:Kd], V[colind[e],:Kd]> over the pattern of A.
+ * scale NULL means s_e = 1 (the semantics of csr_sddmm,
+ * gcn/pytorch-custom/sddmm.cpp:42-61 / sddmm.cu:255-424); scale = A's values
+ * gives the A_ij-scaled form.  U is M×Kd (ldu), V is K×Kd (ldv), row-major. */
+int armsp_sddmm_csr_f32(int M, int K, int Kd, int nnz,
+                        const int* rowptr, const int* colind, const float* scale,
+                        const float* U, int ldu, const float* V, int ldv,
+                        float* out, const armsp_config* cfg, void* stream);
+/* COO form: replaces coo_sddmm (sddmm.cpp:21-40 / sddmm.cu:7-252). */
+int armsp_sddmm_coo_f32(int M, int K, int Kd, int nnz,
+                        const int* rowind, const int* colind, const float* scale,
+                        const float* U, int ldu, const float* V, int ldv,
+                        float* out, void* stream);
+/* bfloat16 U and V, fp32 accumulate and output.  No reference counterpart. */
+int armsp_sddmm_csr_bf16(int M, int K, int Kd, int nnz,
+                         const int* rowptr, const int* colind, const float* scale,
+                         const uint16_t* U, int ldu, const uint16_t* V, int ldv,
+                         float* out, const armsp_config* cfg, void* stream);
+
+/* ------------------------------------------------------- SpGEMM to dense */
+/* C[M×N] += A[M×K]·B[K×N] with A and B both CSR and C dense row-major.
+ * Replaces sparse_matrix_mul3 (SpMM.cpp:309-327, numaAware/numaAwareSpMM.cpp
+ * :452-470): per output element the fmaf sequence follows A's CSR order, as
+ * in the reference, so the result is bit-exact. */
+int armsp_spgemm_dense_f32(int M, int K, int N,
+                           int nnzA, const int* rowptrA, const int* colindA, const float* valA,
+                           int nnzB, const int* rowptrB, const int* colindB, const float* valB,
+                           float* C, int ldc, void* stream);
+
+/* -------------------------------------------------------------- transpose */
+/* CSC of A (= CSR of Aᵀ): colptr[K+1], rowind[nnz], valT[nnz] (valT and val
+ * may be NULL together).  Stable: rows ascend inside each column, exactly as
+ * toCSC (SpMM.cpp:80-95) / scipy tocsc produce; replaces the cuSPARSE call in
+ * csr2cscKernel (gcn/pytorch-custom/spmm_kernel.cu:381-423). */
+int armsp_csr_transpose(int M, int K, int nnz,
+                        const int* rowptr, const int* colind, const float* val,
+                        int* colptr, int* rowind, float* valT, void* stream);
+
+/* -------------------------------------------------------- dense helpers */
+/* C[M×N] (=|+=) op(A)[M×Kd] · op(B)[Kd×N] in bfloat16 on MFMA with fp32
+ * accumulation; element (i,k) of op(A) is A[i*sam + k*sak], element (k,j) of
+ * op(B) is B[k*sbk + j*sbn].  Replaces torch.matmul(x, weight) of GCNConv
+ * (gcn/pytorch-custom/op.py:114) and its weight gradient. */
+int armsp_gemm_bf16(int M, int N, int Kd,
+                    const uint16_t* A, int64_t sam, int64_t sak,
+                    const uint16_t* B, int64_t sbk, int64_t sbn,
+                    float* C, int64_t ldc, int accumulate, void* stream);
+/* fp32 → bf16 (round to nearest even), n elements. */
+int armsp_convert_f32_to_bf16(int64_t n, const float* x, uint16_t* y, void* stream);
+/* Per-row fp8-e4m3 quantisation: scale[k] = max|B[k,:]| / 448 (1 if the row
+ * is zero), Bq = RNE(B / scale) saturated to ±448. */
+int armsp_quantize_fp8_rows(int K, int N, const float* B, int ldb,
+                            uint8_t* Bq, float* scale, void* stream);
+/* Counter-based fills (same streams as the host generators in armsp_gen.h). */
+int armsp_fill_uniform_f32(int64_t n, uint64_t seed, uint64_t stream_id,
+                           float lo, float hi, float* out, void* stream);
+int armsp_fill_normal_f32(int64_t n, uint64_t seed, uint64_t stream_id,
+                          float* out, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* ARMSP_H */

@@ -1,0 +1,114 @@
+"""The C-ABI library loads, exports every symbol the public headers declare,
+and rejects bad arguments on the host before any GPU work (no GPU needed)."""
+import os
+import re
+import subprocess
+import textwrap
+
+import pytest
+
+from conftest import ROOT
+
+INCLUDE = os.path.join(ROOT, "include")
+LIB = os.path.join(ROOT, "armsparse_amd", "libarmsparse.so")
+
+
+def _declared_c_functions():
+    names = set()
+    for h in ("armsp.h", "armsp_gen.h"):
+        src = open(os.path.join(INCLUDE, h)).read()
+        src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+        names |= set(re.findall(r"\b(armsp_\w+)\s*\(", src))
+    return names
+
+
+def _exported():
+    out = subprocess.run(["nm", "-D", "--defined-only", "-C", LIB], check=True, capture_output=True, text=True)
+    return {line.split(" T ", 1)[1].split("(")[0].strip() for line in out.stdout.splitlines() if " T " in line}
+
+
+def test_every_declared_symbol_is_exported():
+    exported = _exported()
+    declared = _declared_c_functions()
+    assert len(declared) >= 30
+    missing = declared - exported
+    assert not missing, f"declared but not exported: {sorted(missing)}"
+    compat = re.findall(r"^(?:void)\s+(\w+)\(", open(os.path.join(INCLUDE, "armsparse_compat.h")).read(), re.M)
+    assert set(compat) == {"rowBaseSpmm", "rowBaseSpmm2", "rowBaseSpmm3", "rowWiseBlocking",
+                           "rowWiseRegisterBlocking", "irregular", "origin", "sparse_matrix_mul3",
+                           "sparse_dense_vector", "csr_spmm_kernel"}
+    assert set(compat) <= exported
+    # helpers the reference drivers define themselves must not collide
+    for helper in ("toCSR", "prepareSparse", "getHighResolutionTime", "num_vector_neon"):
+        assert helper not in exported
+
+
+def test_python_binding_covers_header():
+    from armsparse_amd import _lib
+
+    assert _declared_c_functions() == set(_lib.EXPORTED)
+
+
+def test_status_and_invalid_arguments_without_gpu():
+    from armsparse_amd import _lib
+
+    lib = _lib.lib
+    assert lib.armsp_version() == 100
+    assert lib.armsp_status_string(0) == b"success"
+    assert lib.armsp_status_string(2) == b"not supported"
+    # invalid sizes are rejected before any HIP call
+    assert lib.armsp_spmm_f32(-1, 4, 4, 0, None, None, None, None, 4, None, 4, 0, None, None) == 1
+    assert lib.armsp_spmm_f32(4, 4, 4, 8, None, None, None, None, 4, None, 4, 0, None, None) == 1
+    assert lib.armsp_spmm_f32(0, 4, 4, 0, None, None, None, None, 4, None, 4, 0, None, None) == 0  # empty
+    assert lib.armsp_spmv_vcsr_f32(4, 6, None, None, None, None, None, 0, None) == 1  # not a multiple of 4
+    assert lib.armsp_sddmm_csr_f32(4, 4, -1, 1, None, None, None, None, 1, None, 1, None, None, None) == 1
+    assert lib.armsp_gemm_bf16(4, 4, 4, None, 1, 1, None, 1, 1, None, 2, 0, None) == 1  # ldc < N
+    cfg = _lib.Config.make("row", group=3)
+    import ctypes
+
+    # group must be a power of two in [4, 64]: rejected on the host
+    assert lib.armsp_spmm_f32(4, 4, 4, 0, ctypes.c_void_p(8), None, None, None, 4, ctypes.c_void_p(8), 4, 0,
+                              ctypes.byref(cfg), None) == 1
+
+
+def test_no_gpu_fails_loudly():
+    import torch
+
+    if torch.cuda.is_available():
+        pytest.skip("a GPU is present")
+    from armsparse_amd import kernels
+
+    rp = torch.tensor([0, 1], dtype=torch.int32)
+    ci = torch.tensor([0], dtype=torch.int32)
+    with pytest.raises(RuntimeError, match="no ROCm device"):
+        kernels.spmm(rp, ci, torch.ones(1), torch.ones(1, 4))
+
+
+_SHIM_MAIN = textwrap.dedent(r"""
+    #include "armsparse_compat.h"
+    #include <cstring>
+    int main(int argc, char** argv) {
+        int rp[] = {0, 1, 0}, ci[] = {0, 0}; float v[] = {1, 1}, B[64] = {0}, C[64] = {0};
+        if (!std::strcmp(argv[1], "blocking")) rowWiseBlocking(rp, ci, v, B, C, 7, 2, 8, 1);
+        if (!std::strcmp(argv[1], "register")) rowWiseRegisterBlocking(rp, ci, v, B, C, 2, 2, 6, 1);
+        if (!std::strcmp(argv[1], "decreasing")) rowBaseSpmm3(rp, ci, v, B, C, 2, 2, 4, 1);
+        int rp2[] = {0, 1, 2}, ci2[] = {0, 5};
+        if (!std::strcmp(argv[1], "range")) rowBaseSpmm3(rp2, ci2, v, B, C, 2, 2, 4, 1);
+        return 0;
+    }
+""")
+
+
+@pytest.mark.parametrize("case,msg", [("blocking", "requires M % 8 == 0"), ("register", "requires N % 4 == 0"),
+                                      ("decreasing", "rowPtr is decreasing"),
+                                      ("range", "column index out of range")])
+def test_compat_shims_reject_bad_input(tmp_path, case, msg):
+    """The reference-name shims validate on the host and exit(EXIT_FAILURE) with a message."""
+    src = tmp_path / "shim.cpp"
+    src.write_text(_SHIM_MAIN)
+    exe = tmp_path / "shim"
+    subprocess.run(["g++", "-O1", f"-I{INCLUDE}", str(src), "-o", str(exe), LIB,
+                    f"-Wl,-rpath,{os.path.dirname(LIB)}"], check=True)
+    r = subprocess.run([str(exe), case], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 1
+    assert msg in r.stderr and r.stderr.startswith("armsparse: ")
