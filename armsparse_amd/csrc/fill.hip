@@ -22,10 +22,11 @@ uint64_t host_mix64(uint64_t z) {
 }
 
 __global__ void k_fill_uniform(int64_t n, uint64_t key, float lo, float hi, float* __restrict__ out) {
+#pragma clang fp contract(off)  // lo + (hi-lo)·u unfused, as on the host
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
         const uint64_t r = mix64(key + kGolden * (uint64_t)(i + 1));
         const float u = (float)(uint32_t)(r >> 40) * 0x1p-24f;
-        out[i] = __fadd_rn(lo, __fmul_rn(hi - lo, u));
+        out[i] = lo + (hi - lo) * u;
     }
 }
 

@@ -7,11 +7,56 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <atomic>
+#include <cstdlib>
+#include <thread>
 #include <vector>
 
 namespace {
 
 constexpr uint64_t kGolden = 0x9E3779B97F4A7C15ULL;
+
+// Host parallelism with std::thread (no OpenMP runtime in the library, so it
+// never brings a second one into a process that already has torch's).
+int num_threads() {
+    static const int n = [] {
+        const char* e = std::getenv("OMP_NUM_THREADS");
+        int v = e ? std::atoi(e) : 0;
+        if (v <= 0) v = (int)std::thread::hardware_concurrency();
+        return std::max(1, std::min(v, 64));
+    }();
+    return n;
+}
+
+// body(tid, nthreads) on nt threads.
+template <class F>
+void run_threads(int nt, F body) {
+    std::vector<std::thread> th;
+    for (int t = 1; t < nt; ++t) th.emplace_back(body, t, nt);
+    body(0, nt);
+    for (auto& x : th) x.join();
+}
+
+// body(i) for i in [0, n), static contiguous chunks.
+template <class F>
+void parallel_for(int64_t n, F body) {
+    const int nt = n < 65536 ? 1 : num_threads();
+    run_threads(nt, [&](int t, int T) {
+        const int64_t lo = n * t / T, hi = n * (t + 1) / T;
+        for (int64_t i = lo; i < hi; ++i) body(i);
+    });
+}
+
+// body(i) for i in [0, n), handed out `grain` at a time.
+template <class F>
+void parallel_dynamic(int64_t n, int64_t grain, F body) {
+    std::atomic<int64_t> next{0};
+    const int nt = n <= grain ? 1 : num_threads();
+    run_threads(nt, [&](int, int) {
+        for (int64_t lo; (lo = next.fetch_add(grain)) < n;)
+            for (int64_t i = lo; i < std::min(n, lo + grain); ++i) body(i);
+    });
+}
 
 inline uint64_t mix64(uint64_t z) {
     z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
@@ -33,38 +78,85 @@ inline float nonzero_value(uint64_t key, uint64_t i) {
     return v == 0.0f ? 0x1p-24f : v;
 }
 
-// Open-addressing set of 64-bit keys (UINT64_MAX reserved as "empty").
-struct KeySet {
-    std::vector<uint64_t> slots;
-    uint64_t mask;
-    explicit KeySet(int64_t expected) {
-        uint64_t cap = 1024;
-        while (cap < (uint64_t)expected * 2) cap <<= 1;
-        slots.assign(cap, ~0ULL);
-        mask = cap - 1;
-    }
-    // true if newly inserted
-    bool insert(uint64_t k) {
-        uint64_t h = mix64(k) & mask;
-        while (true) {
-            uint64_t s = slots[h];
-            if (s == ~0ULL) { slots[h] = k; return true; }
-            if (s == k) return false;
-            h = (h + 1) & mask;
+// The first `want` distinct values of draw(e), e = 0, 1, 2, ... (draw returns
+// ~0 for a rejected draw), sorted ascending.  Draws are generated in parallel,
+// partitioned into NB buckets by bucket_of(key) (monotone in the key) with
+// each bucket keeping draw order, sorted and deduplicated per bucket keeping
+// the earliest draw of every key, and cut at the want-th smallest first-draw
+// index — the same set a sequential scan would collect, for any thread count.
+template <class Draw, class Bucket>
+std::vector<uint64_t> first_distinct(int64_t want, Draw draw, Bucket bucket_of, int NB) {
+    if (want <= 0) return {};
+    int64_t T = want + want / 4 + 1024;
+    while (true) {
+        std::vector<uint64_t> keys(T);
+        parallel_for(T, [&](int64_t e) { keys[e] = draw((uint64_t)e); });
+        const int nt = T < 65536 ? 1 : num_threads();
+        std::vector<int64_t> cnt((size_t)nt * NB, 0);
+        run_threads(nt, [&](int tid, int T_) {
+            const int64_t lo = T * tid / T_, hi = T * (tid + 1) / T_;
+            int64_t* c = &cnt[(size_t)tid * NB];
+            for (int64_t e = lo; e < hi; ++e)
+                if (keys[e] != ~0ULL) c[bucket_of(keys[e])]++;
+        });
+        std::vector<int64_t> boff(NB + 1, 0), off((size_t)nt * NB);
+        int64_t acc = 0;
+        for (int b = 0; b < NB; ++b) {
+            boff[b] = acc;
+            for (int t = 0; t < nt; ++t) {
+                off[(size_t)t * NB + b] = acc;
+                acc += cnt[(size_t)t * NB + b];
+            }
         }
+        boff[NB] = acc;
+        std::vector<std::pair<uint64_t, int64_t>> pairs(acc);
+        run_threads(nt, [&](int tid, int T_) {
+            const int64_t lo = T * tid / T_, hi = T * (tid + 1) / T_;
+            int64_t* o = &off[(size_t)tid * NB];
+            for (int64_t e = lo; e < hi; ++e)
+                if (keys[e] != ~0ULL) pairs[o[bucket_of(keys[e])]++] = {keys[e], e};
+        });
+        std::vector<uint64_t>().swap(keys);
+        std::vector<int64_t> bcount(NB, 0);
+        parallel_dynamic(NB, 4, [&](int64_t b) {
+            auto first = pairs.begin() + boff[b], last = pairs.begin() + boff[b + 1];
+            std::sort(first, last);  // by key, then by draw index
+            auto w = first;
+            for (auto it = first; it != last; ++it)
+                if (w == first || (w - 1)->first != it->first) *w++ = *it;
+            bcount[b] = w - first;
+        });
+        int64_t D = 0;
+        for (int b = 0; b < NB; ++b) D += bcount[b];
+        if (D < want) {
+            T = T + T / 2;
+            continue;
+        }
+        std::vector<int64_t> firsts;
+        firsts.reserve(D);
+        for (int b = 0; b < NB; ++b)
+            for (int64_t i = 0; i < bcount[b]; ++i) firsts.push_back(pairs[boff[b] + i].second);
+        std::nth_element(firsts.begin(), firsts.begin() + (want - 1), firsts.end());
+        const int64_t tstar = firsts[want - 1];
+        std::vector<uint64_t> out;
+        out.reserve(want);
+        for (int b = 0; b < NB; ++b)
+            for (int64_t i = 0; i < bcount[b]; ++i)
+                if (pairs[boff[b] + i].second <= tstar) out.push_back(pairs[boff[b] + i].first);
+        return out;
     }
-};
+}
 
 // Sorted unique keys (row<<32 | col) → canonical CSR.
 void keys_to_csr(std::vector<uint64_t>& keys, int M, uint64_t vkey, int* rowptr, int* colind, float* val) {
-    std::sort(keys.begin(), keys.end());
+    if (!std::is_sorted(keys.begin(), keys.end())) std::sort(keys.begin(), keys.end());
     std::fill(rowptr, rowptr + M + 1, 0);
     for (uint64_t k : keys) rowptr[(k >> 32) + 1]++;
     for (int r = 0; r < M; ++r) rowptr[r + 1] += rowptr[r];
-    for (size_t i = 0; i < keys.size(); ++i) {
+    parallel_for((int64_t)keys.size(), [&](int64_t i) {
         colind[i] = (int)(keys[i] & 0xffffffffULL);
-        if (val) val[i] = nonzero_value(vkey, i);
-    }
+        if (val) val[i] = nonzero_value(vkey, (uint64_t)i);
+    });
 }
 
 }  // namespace
@@ -80,20 +172,18 @@ float armsp_rng_uniform(uint64_t seed, uint64_t stream, uint64_t i, float lo, fl
 int armsp_host_fill_uniform(int64_t n, uint64_t seed, uint64_t stream, float lo, float hi, float* out) {
     if (n < 0 || (n > 0 && !out)) return 1;
     const uint64_t key = stream_key(seed, stream);
-#pragma omp parallel for schedule(static)
-    for (int64_t i = 0; i < n; ++i) out[i] = to_uniform(draw(key, (uint64_t)i), lo, hi);
+    parallel_for(n, [&](int64_t i) { out[i] = to_uniform(draw(key, (uint64_t)i), lo, hi); });
     return 0;
 }
 
 int armsp_host_fill_normal(int64_t n, uint64_t seed, uint64_t stream, float* out) {
     if (n < 0 || (n > 0 && !out)) return 1;
     const uint64_t key = stream_key(seed, stream);
-#pragma omp parallel for schedule(static)
-    for (int64_t i = 0; i < n; ++i) {
+    parallel_for(n, [&](int64_t i) {
         double u1 = ((double)(draw(key, 2 * (uint64_t)i) >> 11) + 1.0) * 0x1p-53;  // (0,1]
         double u2 = (double)(draw(key, 2 * (uint64_t)i + 1) >> 11) * 0x1p-53;      // [0,1)
         out[i] = (float)(std::sqrt(-2.0 * std::log(u1)) * std::cos(2.0 * M_PI * u2));
-    }
+    });
     return 0;
 }
 
@@ -102,14 +192,16 @@ int armsp_gen_uniform_csr(int M, int K, int64_t nnz, uint64_t seed, int* rowptr,
     const uint64_t cells = (uint64_t)M * (uint64_t)K;
     if ((uint64_t)nnz > cells) return 1;
     const uint64_t pkey = stream_key(seed, ARMSP_STREAM_POS);
-    KeySet set(nnz);
-    std::vector<uint64_t> keys;
-    keys.reserve(nnz);
-    for (uint64_t i = 0; (int64_t)keys.size() < nnz; ++i) {
-        uint64_t p = draw(pkey, i) % cells;
-        uint64_t k = ((p / K) << 32) | (p % K);
-        if (set.insert(k)) keys.push_back(k);
-    }
+    const uint64_t Ku = (uint64_t)K;
+    int rshift = 0;
+    while ((M >> rshift) > 4096) ++rshift;
+    std::vector<uint64_t> keys = first_distinct(
+        nnz,
+        [&](uint64_t i) {
+            uint64_t p = draw(pkey, i) % cells;
+            return ((p / Ku) << 32) | (p % Ku);
+        },
+        [&](uint64_t k) { return (int)((k >> 32) >> rshift); }, (M >> rshift) + 1);
     keys_to_csr(keys, M, stream_key(seed, ARMSP_STREAM_VAL), rowptr, colind, val);
     return 0;
 }
@@ -125,22 +217,22 @@ int armsp_gen_rmat_csr(int scale, int64_t nnz, double a, double b, double c, uin
     const uint32_t tb = (uint32_t)std::lround((a + b) * 65536.0);
     const uint32_t tc = (uint32_t)std::lround((a + b + c) * 65536.0);
     const uint64_t rkey = stream_key(seed, ARMSP_STREAM_RMAT);
-    KeySet set(nnz);
-    std::vector<uint64_t> keys;
-    keys.reserve(nnz);
-    for (uint64_t e = 0; (int64_t)keys.size() < nnz; ++e) {
-        uint64_t row = 0, col = 0, q = 0;
-        for (int t = 0; t < scale; ++t) {
-            if ((t & 3) == 0) q = draw(rkey, e * 8 + (t >> 2));
-            uint32_t u = (uint32_t)((q >> (16 * (t & 3))) & 0xffff);
-            uint64_t rb = (u >= tb) ? 1 : 0;                       // quadrants c, d
-            uint64_t cb = (u >= ta && u < tb) || (u >= tc) ? 1 : 0;  // quadrants b, d
-            row = (row << 1) | rb;
-            col = (col << 1) | cb;
-        }
-        uint64_t k = (row << 32) | col;
-        if (set.insert(k)) keys.push_back(k);
-    }
+    const int rshift = scale > 12 ? scale - 12 : 0;
+    std::vector<uint64_t> keys = first_distinct(
+        nnz,
+        [&](uint64_t e) {
+            uint64_t row = 0, col = 0, q = 0;
+            for (int t = 0; t < scale; ++t) {
+                if ((t & 3) == 0) q = draw(rkey, e * 8 + (t >> 2));
+                uint32_t u = (uint32_t)((q >> (16 * (t & 3))) & 0xffff);
+                uint64_t rb = (u >= tb) ? 1 : 0;                         // quadrants c, d
+                uint64_t cb = (u >= ta && u < tb) || (u >= tc) ? 1 : 0;  // quadrants b, d
+                row = (row << 1) | rb;
+                col = (col << 1) | cb;
+            }
+            return (row << 32) | col;
+        },
+        [&](uint64_t k) { return (int)((k >> 32) >> rshift); }, (M >> rshift) + 1);
     keys_to_csr(keys, M, stream_key(seed, ARMSP_STREAM_VAL), rowptr, colind, val);
     return 0;
 }
@@ -161,19 +253,22 @@ int armsp_gen_powerlaw_graph(int n, int64_t pairs, double gamma, uint64_t seed, 
         return v < n ? v : n - 1;
     };
     const uint64_t gkey = stream_key(seed, ARMSP_STREAM_GRAPH);
-    KeySet set(pairs);
+    int rshift = 0;
+    while ((n >> rshift) > 4096) ++rshift;
+    std::vector<uint64_t> und = first_distinct(
+        pairs,
+        [&](uint64_t i) -> uint64_t {
+            int u = pick(draw(gkey, 2 * i)), v = pick(draw(gkey, 2 * i + 1));
+            if (u == v) return (uint64_t)~0ULL;
+            uint64_t lo = (uint64_t)std::min(u, v), hi = (uint64_t)std::max(u, v);
+            return (lo << 32) | hi;
+        },
+        [&](uint64_t k) { return (int)((k >> 32) >> rshift); }, (n >> rshift) + 1);
     std::vector<uint64_t> keys;
-    keys.reserve(2 * pairs);
-    int64_t found = 0;
-    for (uint64_t i = 0; found < pairs; ++i) {
-        int u = pick(draw(gkey, 2 * i)), v = pick(draw(gkey, 2 * i + 1));
-        if (u == v) continue;
-        uint64_t lo = (uint64_t)std::min(u, v), hi = (uint64_t)std::max(u, v);
-        if (set.insert((lo << 32) | hi)) {
-            keys.push_back((lo << 32) | hi);
-            keys.push_back((hi << 32) | lo);
-            ++found;
-        }
+    keys.reserve(2 * und.size());
+    for (uint64_t k : und) {
+        keys.push_back(k);
+        keys.push_back((k << 32) | (k >> 32));
     }
     keys_to_csr(keys, n, 0, rowptr, colind, nullptr);
     return 0;
@@ -195,8 +290,7 @@ int64_t armsp_gen_corpus_csr(int kind, int M, double avg_row, double param, uint
         const uint32_t tc = (uint32_t)std::lround((a + b + c) * 65536.0);
         const int64_t E = (int64_t)std::llround(avg_row * M);
         std::vector<uint64_t> keys(E);
-#pragma omp parallel for schedule(static)
-        for (int64_t e = 0; e < E; ++e) {
+        parallel_for(E, [&](int64_t e) {
             uint64_t row = 0, col = 0, q = 0;
             for (int t = 0; t < scale; ++t) {
                 if ((t & 3) == 0) q = draw(key, (uint64_t)e * 8 + (t >> 2));
@@ -207,15 +301,15 @@ int64_t armsp_gen_corpus_csr(int kind, int M, double avg_row, double param, uint
                 col = (col << 1) | cb;
             }
             keys[e] = (row << 32) | col;
-        }
+        });
         std::sort(keys.begin(), keys.end());
         keys.erase(std::unique(keys.begin(), keys.end()), keys.end());
         if ((int64_t)keys.size() > cap) return -1;
         keys_to_csr(keys, M, vkey, rowptr, colind, val);
         return (int64_t)keys.size();
     }
-#pragma omp parallel for schedule(dynamic, 256)
-    for (int r = 0; r < M; ++r) {
+    parallel_dynamic(M, 256, [&](int64_t ri) {
+        const int r = (int)ri;
         std::vector<int>& out = rows[r];
         const uint64_t base = (uint64_t)r * 4096;
         if (kind == 3) {
@@ -226,7 +320,7 @@ int64_t armsp_gen_corpus_csr(int kind, int M, double avg_row, double param, uint
             bc.erase(std::unique(bc.begin(), bc.end()), bc.end());
             for (int blk : bc)
                 for (int t = 0; t < 32 && blk * 32 + t < M; ++t) out.push_back(blk * 32 + t);
-            continue;
+            return;
         }
         double fl = std::floor(avg_row);
         int d = (int)fl + ((double)(draw(key, base) >> 11) * 0x1p-53 < avg_row - fl ? 1 : 0);
@@ -239,12 +333,12 @@ int64_t armsp_gen_corpus_csr(int kind, int M, double avg_row, double param, uint
         const int width = hi - lo;
         if (d >= width) {
             for (int cidx = lo; cidx < hi; ++cidx) out.push_back(cidx);
-            continue;
+            return;
         }
         for (int t = 0; t < d; ++t) out.push_back(lo + (int)(draw(key, base + 1 + t) % (uint64_t)width));
         std::sort(out.begin(), out.end());
         out.erase(std::unique(out.begin(), out.end()), out.end());
-    }
+    });
     int64_t total = 0;
     rowptr[0] = 0;
     for (int r = 0; r < M; ++r) {
@@ -252,15 +346,14 @@ int64_t armsp_gen_corpus_csr(int kind, int M, double avg_row, double param, uint
         if (total > cap || total > 0x7fffffff) return -1;
         rowptr[r + 1] = (int)total;
     }
-#pragma omp parallel for schedule(dynamic, 256)
-    for (int r = 0; r < M; ++r) {
+    parallel_dynamic(M, 256, [&](int64_t r) {
         int64_t p = rowptr[r];
         for (int cidx : rows[r]) {
             colind[p] = cidx;
             if (val) val[p] = nonzero_value(vkey, (uint64_t)p);
             ++p;
         }
-    }
+    });
     return total;
 }
 

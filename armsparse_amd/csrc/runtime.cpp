@@ -13,6 +13,7 @@ thread_local int g_last_hip_error = 0;
 
 std::mutex g_pool_mu;
 bool g_pool_ready[64] = {};
+hipMemPool_t g_pools[64] = {};
 
 struct TimingRecord {
     std::string family;
@@ -60,23 +61,31 @@ int hip_fail(hipError_t e) {
     return ARMSP_ERROR_HIP;
 }
 
+// The library's own stream-ordered pool per device (the device's default pool,
+// which other libraries in the process share, is left untouched).
 int ws_alloc(void** p, size_t bytes, hipStream_t s) {
     *p = nullptr;
     if (bytes == 0) return ARMSP_SUCCESS;
     int dev = 0;
     ARMSP_HIP(hipGetDevice(&dev));
-    if (dev >= 0 && dev < 64) {
+    if (dev < 0 || dev >= 64) return ARMSP_ERROR_NOT_SUPPORTED;
+    hipMemPool_t pool;
+    {
         std::lock_guard<std::mutex> g(g_pool_mu);
         if (!g_pool_ready[dev]) {
-            hipMemPool_t pool;
-            if (hipDeviceGetDefaultMemPool(&pool, dev) == hipSuccess) {
-                uint64_t thr = ~0ULL;
-                hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &thr);
-            }
+            hipMemPoolProps props;
+            std::memset(&props, 0, sizeof(props));
+            props.allocType = hipMemAllocationTypePinned;
+            props.location.type = hipMemLocationTypeDevice;
+            props.location.id = dev;
+            ARMSP_HIP(hipMemPoolCreate(&g_pools[dev], &props));
+            uint64_t thr = ~0ULL;
+            ARMSP_HIP(hipMemPoolSetAttribute(g_pools[dev], hipMemPoolAttrReleaseThreshold, &thr));
             g_pool_ready[dev] = true;
         }
+        pool = g_pools[dev];
     }
-    ARMSP_HIP(hipMallocAsync(p, bytes, s));
+    ARMSP_HIP(hipMallocFromPoolAsync(p, bytes, pool, s));
     return ARMSP_SUCCESS;
 }
 

@@ -146,7 +146,12 @@ __global__ __launch_bounds__(256) void k_spmv_vcsr(int M, const int* __restrict_
     if (fused) {
         for (int idx = beg; idx < end; idx += 4) s = fmaf(val[idx + l], x[colind[idx] + l], s);
     } else {
-        for (int idx = beg; idx < end; idx += 4) s = __fadd_rn(s, __fmul_rn(val[idx + l], x[colind[idx] + l]));
+        // vmlaq_f32 is a multiply then an add: forbid contraction into an FMA here
+#pragma clang fp contract(off)
+        for (int idx = beg; idx < end; idx += 4) {
+            const float p = val[idx + l] * x[colind[idx] + l];
+            s = s + p;
+        }
     }
     const float t = __fadd_rn(s, __shfl_xor(s, 2, 4));   // lanes 0/2: s0+s2, lanes 1/3: s1+s3
     const float tot = __fadd_rn(t, __shfl_xor(t, 1, 4));  // (s0+s2)+(s1+s3)

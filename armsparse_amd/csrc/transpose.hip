@@ -1,9 +1,234 @@
-// CSR → CSC transpose (placeholder until the radix-sort kernels land).
+// Stable CSR → CSC transpose for gfx950.
+//
+// Reference: toCSC's column-major scan (SpMM.cpp:80-95) and scipy's tocsc in
+// the GCN script (gcn_custom.py:48-55) produce, for every column, its entries
+// with ascending rows; csr2cscKernel (gcn/pytorch-custom/spmm_kernel.cu:381-423)
+// would call cuSPARSE for it.  Here: an LSD radix sort of (column, position)
+// pairs, 8 bits per pass.  The input is already in row order, and every pass
+// is stable (per-block digit histograms, a device-wide exclusive scan, and a
+// scatter whose in-block ranks come from wave ballots in item order), so rows
+// stay ascending inside each column.  The last pass gathers row ids and
+// values; colptr is read off the sorted column keys.
+#include <climits>
+
 #include "common.hpp"
+
+namespace armsp {
+namespace {
+
+constexpr int TPB = 256, IPT = 16, TILE = TPB * IPT;  // items per block: 4 waves × 16 rounds × 64
+constexpr int NB = 256;                               // 8-bit digits
+constexpr int SCAN_CHUNK = 2048;                       // 256 threads × 8
+
+__global__ __launch_bounds__(256) void k_expand_rows(int M, const int* __restrict__ rowptr, int* __restrict__ rows) {
+    const int lane = threadIdx.x & 63;
+    const int r = uniform((int)(blockIdx.x * 4u + (threadIdx.x >> 6)));
+    if (r >= M) return;
+    const int b = rowptr[r], e = rowptr[r + 1];
+    for (int p = b + lane; p < e; p += 64) rows[p] = r;
+}
+
+__global__ __launch_bounds__(256) void k_radix_hist(int n, int shift, const int* __restrict__ keys,
+                                                    int* __restrict__ hist, int nblocks) {
+    __shared__ int cnt[NB];
+    cnt[threadIdx.x] = 0;
+    __syncthreads();
+    const int64_t base = (int64_t)blockIdx.x * TILE;
+    for (int j = 0; j < IPT; ++j) {
+        const int64_t e = base + (int64_t)j * TPB + threadIdx.x;
+        if (e < n) atomicAdd(&cnt[(keys[e] >> shift) & (NB - 1)], 1);
+    }
+    __syncthreads();
+    hist[(int64_t)threadIdx.x * nblocks + blockIdx.x] = cnt[threadIdx.x];
+}
+
+// idx_in == nullptr: item e carries its own position.  final_pass: instead of
+// idx_out, write rowind[dst] = rows[idx], valT[dst] = val[idx].
+__global__ __launch_bounds__(256) void k_radix_scatter(int n, int shift, const int* __restrict__ keys_in,
+                                                       const int* __restrict__ idx_in, const int* __restrict__ offs,
+                                                       int nblocks, int* __restrict__ keys_out,
+                                                       int* __restrict__ idx_out, const int* __restrict__ rows,
+                                                       const float* __restrict__ val, int* __restrict__ rowind,
+                                                       float* __restrict__ valT, int final_pass) {
+    __shared__ int wcnt[4][NB];
+    __shared__ int wpre[4][NB];
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    for (int i = tid; i < 4 * NB; i += TPB) wcnt[i / NB][i % NB] = 0;
+    __syncthreads();
+    const int64_t base = (int64_t)blockIdx.x * TILE + (int64_t)w * (IPT * 64);
+    const uint64_t lt = (1ull << lane) - 1ull;
+    int key[IPT], idx[IPT], dig[IPT], pos[IPT];
+#pragma unroll
+    for (int r = 0; r < IPT; ++r) {
+        const int64_t e = base + r * 64 + lane;
+        const bool valid = e < n;
+        key[r] = valid ? keys_in[e] : 0;
+        idx[r] = valid ? (idx_in ? idx_in[e] : (int)e) : -1;
+        const int d = (key[r] >> shift) & (NB - 1);
+        dig[r] = d;
+        uint64_t m = __ballot(valid);
+#pragma unroll
+        for (int bit = 0; bit < 8; ++bit) {
+            const bool on = (d >> bit) & 1;
+            const uint64_t bal = __ballot(on);
+            m &= on ? bal : ~bal;
+        }
+        int before = 0;
+        if (valid) before = wcnt[w][d];
+        pos[r] = before + __popcll(m & lt);
+        __builtin_amdgcn_wave_barrier();
+        if (valid && (m >> lane) == 1ull) wcnt[w][d] = before + __popcll(m);  // highest peer updates
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    }
+    __syncthreads();
+    {
+        int run = 0;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            wpre[q][tid] = run;
+            run += wcnt[q][tid];
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < IPT; ++r) {
+        if (idx[r] < 0) continue;
+        const int d = dig[r];
+        const int dst = offs[(int64_t)d * nblocks + blockIdx.x] + wpre[w][d] + pos[r];
+        keys_out[dst] = key[r];
+        if (final_pass) {
+            rowind[dst] = rows[idx[r]];
+            if (valT) valT[dst] = val[idx[r]];
+        } else {
+            idx_out[dst] = idx[r];
+        }
+    }
+}
+
+// In-place exclusive scan of 2048-element chunks; sums[b] = chunk total.
+__global__ __launch_bounds__(256) void k_scan_chunks(int64_t n, int* __restrict__ data, int* __restrict__ sums) {
+    __shared__ int wsum[4];
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const int64_t base = (int64_t)blockIdx.x * SCAN_CHUNK + (int64_t)tid * 8;
+    int v[8], s = 0;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+        v[i] = base + i < n ? data[base + i] : 0;
+        s += v[i];
+    }
+    int inc = s;  // inclusive wave scan of per-thread sums
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const int o = __shfl_up(inc, d);
+        if (lane >= d) inc += o;
+    }
+    if (lane == 63) wsum[w] = inc;
+    __syncthreads();
+    int wofs = 0;
+    for (int q = 0; q < w; ++q) wofs += wsum[q];
+    int run = wofs + inc - s;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+        if (base + i < n) data[base + i] = run;
+        run += v[i];
+    }
+    if (tid == TPB - 1) sums[blockIdx.x] = run;
+}
+
+__global__ void k_scan_add(int64_t n, int* __restrict__ data, const int* __restrict__ sums) {
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i < n) data[i] += sums[i / SCAN_CHUNK];
+}
+
+// colptr[c] = first sorted position whose column is ≥ c.
+__global__ void k_colptr(int nnz, int K, const int* __restrict__ keys, int* __restrict__ colptr) {
+    const int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (p > nnz) return;
+    const int k = p < nnz ? keys[p] : K;
+    const int kp = p > 0 ? keys[p - 1] : -1;
+    for (int c = kp + 1; c <= k; ++c) colptr[c] = (int)p;
+}
+
+int64_t scan_ws_ints(int64_t n) {
+    int64_t total = 0;
+    while (n > 1) {
+        n = (n + SCAN_CHUNK - 1) / SCAN_CHUNK;
+        total += n;
+    }
+    return total + 1;
+}
+
+void exclusive_scan(int64_t n, int* data, int* ws, hipStream_t s) {
+    const int64_t nb = (n + SCAN_CHUNK - 1) / SCAN_CHUNK;
+    hipLaunchKernelGGL(k_scan_chunks, dim3((unsigned)nb), dim3(TPB), 0, s, n, data, ws);
+    if (nb > 1) {
+        exclusive_scan(nb, ws, ws + nb, s);
+        hipLaunchKernelGGL(k_scan_add, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, n, data, ws);
+    }
+}
+
+}  // namespace
+}  // namespace armsp
 
 extern "C" int armsp_csr_transpose(int M, int K, int nnz, const int* rowptr, const int* colind, const float* val,
                                    int* colptr, int* rowind, float* valT, void* stream) {
-    (void)M; (void)K; (void)nnz; (void)rowptr; (void)colind; (void)val; (void)colptr; (void)rowind; (void)valT;
-    (void)stream;
-    return ARMSP_ERROR_NOT_SUPPORTED;
+    using namespace armsp;
+    if (M < 0 || K < 0 || nnz < 0) return ARMSP_ERROR_INVALID_VALUE;
+    if (!colptr || (nnz > 0 && (!rowptr || !colind || !rowind))) return ARMSP_ERROR_INVALID_VALUE;
+    if ((val == nullptr) != (valT == nullptr)) return ARMSP_ERROR_INVALID_VALUE;
+    hipStream_t s = as_stream(stream);
+    TimedLaunch t("transpose", s);
+    if (nnz == 0 || M == 0) {
+        hipLaunchKernelGGL(k_colptr, dim3(1), dim3(64), 0, s, 0, K, (const int*)nullptr, colptr);
+        ARMSP_LAUNCHED();
+        t.stop();
+        return ARMSP_SUCCESS;
+    }
+    const int nblocks = (int)(((int64_t)nnz + TILE - 1) / TILE);
+    const int64_t nhist = (int64_t)NB * nblocks;
+    if (nhist > INT_MAX) return ARMSP_ERROR_NOT_SUPPORTED;
+    int passes = 1;
+    while (K > 1 && ((K - 1) >> (8 * passes)) > 0) ++passes;
+    const size_t n4 = ((size_t)nnz * 4 + 255) & ~(size_t)255;
+    const size_t h4 = ((size_t)nhist * 4 + 255) & ~(size_t)255;
+    const size_t s4 = ((size_t)scan_ws_ints(nhist) * 4 + 255) & ~(size_t)255;
+    void* ws = nullptr;
+    int st = ws_alloc(&ws, 5 * n4 + h4 + s4, s);
+    if (st) return st;
+    char* p = static_cast<char*>(ws);
+    int* rows = reinterpret_cast<int*>(p);
+    int* keysA = reinterpret_cast<int*>(p + n4);
+    int* keysB = reinterpret_cast<int*>(p + 2 * n4);
+    int* idxA = reinterpret_cast<int*>(p + 3 * n4);
+    int* idxB = reinterpret_cast<int*>(p + 4 * n4);
+    int* hist = reinterpret_cast<int*>(p + 5 * n4);
+    int* scanws = reinterpret_cast<int*>(p + 5 * n4 + h4);
+
+    hipLaunchKernelGGL(k_expand_rows, dim3((unsigned)((M + 3) / 4)), dim3(256), 0, s, M, rowptr, rows);
+    const int* kin = colind;
+    const int* iin = nullptr;
+    int* kout = keysA;
+    int* iout = idxA;
+    for (int pass = 0; pass < passes; ++pass) {
+        const int shift = 8 * pass;
+        const bool last = pass == passes - 1;
+        hipLaunchKernelGGL(k_radix_hist, dim3(nblocks), dim3(TPB), 0, s, nnz, shift, kin, hist, nblocks);
+        exclusive_scan(nhist, hist, scanws, s);
+        hipLaunchKernelGGL(k_radix_scatter, dim3(nblocks), dim3(TPB), 0, s, nnz, shift, kin, iin, hist, nblocks,
+                           kout, iout, rows, val, rowind, valT, last ? 1 : 0);
+        kin = kout;
+        iin = iout;
+        kout = (kout == keysA) ? keysB : keysA;
+        iout = (iout == idxA) ? idxB : idxA;
+    }
+    const int* sorted_keys = kin;
+    hipLaunchKernelGGL(k_colptr, dim3((unsigned)(((int64_t)nnz + 1 + 255) / 256)), dim3(256), 0, s, nnz, K,
+                       sorted_keys, colptr);
+    hipError_t e = hipGetLastError();
+    ws_free(ws, s);
+    if (e != hipSuccess) return hip_fail(e);
+    t.stop();
+    return ARMSP_SUCCESS;
 }

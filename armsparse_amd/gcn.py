@@ -1,0 +1,84 @@
+"""The 2-layer GCN of gcn/pytorch-custom/gcn_custom.py:72-126 on synthetic
+PubMed-shaped data, for the training-step benchmark and tests.
+
+Planetoid data cannot be downloaded here, so the graph is a seeded power-law
+graph of the same size (19,717 nodes, 44,324 undirected pairs + self loops)
+and the features are TF-IDF-like: 500 columns, ~1% nonzero U(0,1) entries,
+row-normalised as T.NormalizeFeatures does (gcn_custom.py:21).  Labels are
+random over 3 classes; 60 training nodes as in Planetoid's PubMed split.
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass
+
+import numpy as np
+import torch
+import torch.nn.functional as F
+
+from . import data
+from .op import GCNConv
+
+
+class Net(torch.nn.Module):
+    """conv1 → ReLU → dropout → conv2 → log_softmax (gcn_custom.py:72-90)."""
+
+    def __init__(self, in_feats: int, n_hidden: int, n_classes: int, precision: str = "fp32"):
+        super().__init__()
+        self.conv1 = GCNConv(in_feats, n_hidden, cached=True, normalize=True, precision=precision)
+        self.conv2 = GCNConv(n_hidden, n_classes, cached=True, normalize=True, precision=precision)
+        self.reg_params = self.conv1.parameters()
+        self.non_reg_params = self.conv2.parameters()
+
+    def forward(self, x, g):
+        x = F.relu(self.conv1(x, *g))
+        x = F.dropout(x, training=self.training)
+        x = self.conv2(x, *g)
+        return F.log_softmax(x, dim=1)
+
+
+@dataclass
+class GCNTask:
+    model: Net
+    optimizer: torch.optim.Optimizer
+    x: torch.Tensor
+    graph: tuple
+    y: torch.Tensor
+    train_mask: torch.Tensor
+    precision: str
+    nnz: int
+
+
+def make_task(precision: str = "fp32", seed: int = data.CFG_SEED + 3, n: int = 19_717, pairs: int = 44_324,
+              features: int = 500, hidden: int = 64, classes: int = 3, feat_density: float = 0.01,
+              device: str = "cuda") -> GCNTask:
+    g = data.gcn_graph(n, pairs, seed)
+    r = np.random.default_rng(seed)
+    x = np.zeros((n, features), np.float32)
+    mask = r.random((n, features)) < feat_density
+    x[mask] = r.random(int(mask.sum())).astype(np.float32)
+    rs = x.sum(1, keepdims=True)
+    x = np.divide(x, rs, out=np.zeros_like(x), where=rs > 0)  # NormalizeFeatures
+    y = r.integers(0, classes, n)
+    train = np.zeros(n, bool)
+    train[r.choice(n, 20 * classes, replace=False)] = True
+    dev = torch.device(device)
+    t = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev)  # noqa: E731
+    xt = t(x)
+    if precision == "bf16":
+        xt = xt.to(torch.bfloat16)
+    graph = (t(g.rowptr), t(g.colind), t(g.colptr), t(g.rowind), t(g.value_csr), t(g.value_csc))
+    model = Net(features, hidden, classes, precision).to(dev)
+    opt = torch.optim.Adam([dict(params=model.reg_params, weight_decay=5e-4),
+                            dict(params=model.non_reg_params, weight_decay=0)], lr=0.01)
+    return GCNTask(model, opt, xt, graph, t(y), t(train), precision, int(g.colind.size))
+
+
+def train_step(task: GCNTask, sync: bool = True):
+    """forward + nll loss + backward + Adam step (gcn_custom.py:122-126)."""
+    task.model.train()
+    task.optimizer.zero_grad()
+    out = task.model(task.x, task.graph)
+    loss = F.nll_loss(out[task.train_mask], task.y[task.train_mask])
+    loss.backward()
+    task.optimizer.step()
+    return float(loss) if sync else loss.detach()

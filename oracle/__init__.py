@@ -14,6 +14,11 @@ from ctypes import c_float, c_int, c_void_p
 
 import numpy as np
 
+# torch bundles its own OpenMP runtime under the soname libgomp.so.1; loading
+# torch first makes liboracle.so's (and _ref's) libgomp dependency resolve to
+# that same runtime, so the process never holds two OpenMP runtimes.
+import torch  # noqa: F401  (imported for its libgomp)
+
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "liboracle.so")
 REF_PATH = os.path.join(_HERE, "_ref", "libref.so")

@@ -11,6 +11,14 @@ if ROOT not in sys.path:
 GOLDEN = os.path.join(ROOT, "tests", "golden")
 
 
+@__import__("functools").lru_cache(maxsize=None)
+def rmat_1m_cached():
+    """The 1M R-MAT workload, generated once per test session."""
+    from armsparse_amd import data
+
+    return data.rmat_1m()
+
+
 def pytest_configure(config):
     config.addinivalue_line("markers", "gpu: needs an MI355X (gfx950) device")
 

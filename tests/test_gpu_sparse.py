@@ -1,0 +1,239 @@
+"""GPU parity of SpMV (CSR and blocked VCSR), SDDMM, SpGEMM-to-dense, the bf16
+MFMA GEMM and the device fills, against the oracle."""
+import numpy as np
+import pytest
+import torch
+
+import oracle
+from armsparse_amd import data, kernels
+from conftest import rmat_1m_cached, random_csr, read_mtx
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-5
+
+
+def dev(*arrays):
+    return [None if a is None else torch.from_numpy(np.ascontiguousarray(a)).cuda() for a in arrays]
+
+
+def case(msg):
+    print(f"[case] {msg}", flush=True)
+
+
+def abs_bound(rowptr, colind, val, x):
+    """Σ|a_i x_i| per row: the scale of the rounding error of a dot product."""
+    v = np.ones(len(colind)) if val is None else np.abs(val.astype(np.float64))
+    return oracle.spmv_f64(rowptr, colind, v.astype(np.float32), np.abs(x))
+
+
+# ------------------------------------------------------------------ SpMV
+@pytest.fixture(scope="module")
+def rmat14():
+    a = data.rmat_csr(14, 1 << 18, seed=99)
+    return a, dev(a.rowptr, a.colind, a.values)
+
+
+SPMV_CFGS = [dict(schedule="merge", chunk=c) for c in (256, 512, 1024)] + \
+            [dict(schedule="row", group=g) for g in (1, 4, 16, 64)]
+
+
+@pytest.mark.parametrize("cfg", SPMV_CFGS, ids=lambda c: f"{c['schedule']}-{c.get('chunk', c.get('group'))}")
+def test_spmv_csr(rmat14, cfg):
+    case(f"spmv {cfg}")
+    a, (d_rp, d_ci, d_va) = rmat14
+    x = np.random.default_rng(1).uniform(-1, 1, a.shape[1]).astype(np.float32)
+    y0 = np.random.default_rng(2).uniform(-1, 1, a.shape[0]).astype(np.float32)
+    d_x, d_y = dev(x, y0)
+    y = kernels.spmv(d_rp, d_ci, d_va, d_x, config=cfg).cpu().numpy()
+    ref = oracle.spmv_f64(a.rowptr, a.colind, a.values, x)
+    bound = abs_bound(a.rowptr, a.colind, a.values, x)
+    assert np.all(np.abs(y - ref) <= TOL * bound + 1e-30)
+    assert not y[np.diff(a.rowptr) == 0].any()
+    kernels.spmv(d_rp, d_ci, d_va, d_x, out=d_y, accumulate=True, config=cfg)
+    ref = oracle.spmv_f64(a.rowptr, a.colind, a.values, x, y0)
+    assert np.all(np.abs(d_y.cpu().numpy() - ref) <= TOL * (bound + np.abs(y0)))
+
+
+@pytest.mark.parametrize("fused", [False, True])
+def test_vcsr_bitexact(fused):
+    """sparse_dense_vector drop-in: same four partial sums and (s0+s2)+(s1+s3) order."""
+    A, _, x = data.spmm_small()
+    v = data.csr_to_vcsr(A)
+    y0 = np.random.default_rng(3).uniform(-1, 1, 4096).astype(np.float32)
+    d_rp, d_ci, d_va, d_x, d_y = dev(v.rowptr, v.colind, v.values, x, y0)
+    kernels.vcsr_spmv(d_rp, d_ci, d_va, d_x, d_y, fused=fused)
+    expect = oracle.sparse_dense_vector(x, y0.copy(), v.rowptr, v.colind, v.values, fused=fused)
+    assert np.array_equal(d_y.cpu().numpy().view(np.uint32), expect.view(np.uint32))
+
+
+def test_spmv_full_rmat_1m():
+    case("spmv full 1M R-MAT")
+    a = rmat_1m_cached()
+    x = data.host_uniform(a.shape[1], data.CFG_SEED + 1, data.STREAM_DENSE + 1)
+    d_rp, d_ci, d_va, d_x = dev(a.rowptr, a.colind, a.values, x)
+    y = kernels.spmv(d_rp, d_ci, d_va, d_x).cpu().numpy()
+    ref = oracle.spmv_f64(a.rowptr, a.colind, a.values, x)
+    assert np.all(np.abs(y - ref) <= TOL * abs_bound(a.rowptr, a.colind, a.values, x) + 1e-30)
+
+
+# ----------------------------------------------------------------- SDDMM
+@pytest.mark.parametrize("kd", [1, 3, 8, 32, 100, 128])
+@pytest.mark.parametrize("group", [0, 4, 8, 16, 32])
+def test_sddmm_csr(rmat14, kd, group):
+    case(f"sddmm kd={kd} G={group}")
+    a, (d_rp, d_ci, d_va) = rmat14
+    r = np.random.default_rng(kd)
+    U = r.standard_normal((a.shape[0], kd)).astype(np.float32)
+    V = r.standard_normal((a.shape[1], kd)).astype(np.float32)
+    d_U, d_V = dev(U, V)
+    cfg = None if group == 0 else dict(schedule="nnz", group=group)
+    ref = oracle.sddmm_f64(a.rowptr, a.colind, U, V, scale=a.values)
+    bound = oracle.sddmm_f64(a.rowptr, a.colind, np.abs(U), np.abs(V), scale=np.abs(a.values))
+    out = kernels.sddmm(d_rp, d_ci, d_U, d_V, scale=d_va, config=cfg).cpu().numpy()
+    assert np.all(np.abs(out - ref) <= TOL * bound + 1e-30)
+    out = kernels.sddmm(d_rp, d_ci, d_U, d_V, config=cfg).cpu().numpy()
+    ref1 = oracle.sddmm_f64(a.rowptr, a.colind, U, V)
+    assert np.all(np.abs(out - ref1) <= TOL * bound / np.maximum(np.abs(a.values), 1e-30) + 1e-30)
+
+
+def test_sddmm_coo_and_bf16(rmat14):
+    a, (d_rp, d_ci, d_va) = rmat14
+    r = np.random.default_rng(11)
+    U = r.standard_normal((a.shape[0], 32)).astype(np.float32)
+    V = r.standard_normal((a.shape[1], 32)).astype(np.float32)
+    rows = np.repeat(np.arange(a.shape[0], dtype=np.int32), np.diff(a.rowptr))
+    d_ri, d_U, d_V = dev(rows, U, V)
+    ref = oracle.sddmm_f64(a.rowptr, a.colind, U, V)
+    bound = oracle.sddmm_f64(a.rowptr, a.colind, np.abs(U), np.abs(V))
+    out = kernels.sddmm(d_ri, d_ci, d_U, d_V, coo=True).cpu().numpy()
+    assert np.all(np.abs(out - ref) <= TOL * bound + 1e-30)
+    Ub, Vb = d_U.to(torch.bfloat16), d_V.to(torch.bfloat16)
+    refb = oracle.sddmm_f64(a.rowptr, a.colind, Ub.float().cpu().numpy(), Vb.float().cpu().numpy())
+    out = kernels.sddmm(d_rp, d_ci, Ub, Vb).cpu().numpy()
+    assert np.all(np.abs(out - refb) <= TOL * bound * 1.01 + 1e-30)
+
+
+def test_sddmm_full_rmat_1m_kd32():
+    case("sddmm full 1M R-MAT Kd=32")
+    a = rmat_1m_cached()
+    U = data.host_normal(a.shape[0] * 32, data.CFG_SEED + 1, data.STREAM_DENSE + 2).reshape(-1, 32)
+    V = data.host_normal(a.shape[1] * 32, data.CFG_SEED + 1, data.STREAM_DENSE + 3).reshape(-1, 32)
+    d_rp, d_ci, d_va, d_U, d_V = dev(a.rowptr, a.colind, a.values, U, V)
+    out = kernels.sddmm(d_rp, d_ci, d_U, d_V, scale=d_va).cpu().numpy()
+    ref = oracle.sddmm_f64(a.rowptr, a.colind, U, V, scale=a.values)
+    bound = oracle.sddmm_f64(a.rowptr, a.colind, np.abs(U), np.abs(V), scale=np.abs(a.values))
+    assert np.all(np.abs(out - ref) <= TOL * bound + 1e-30)
+
+
+# ---------------------------------------------------------------- SpGEMM
+def test_spgemm_literal_config_bitexact():
+    """./SpMM 4096 4096 4096: A and B 4096² at 1%, dense C += A·B, bit-exact with
+    the restatement (itself bit-exact with the reference's own source)."""
+    case("spgemm 4096^3")
+    A = data.uniform_csr(4096, 4096, 167_772, seed=21)
+    B = data.uniform_csr(4096, 4096, 167_772, seed=22)
+    C0 = data.host_uniform(4096 * 4096, 23, 16).reshape(4096, 4096)
+    t = dev(A.rowptr, A.colind, A.values, B.rowptr, B.colind, B.values, C0)
+    kernels.spgemm_dense((t[0], t[1], t[2]), (t[3], t[4], t[5]), t[6])
+    expect = oracle.sparse_matrix_mul3((A.rowptr, A.colind, A.values), (B.rowptr, B.colind, B.values), C0.copy())
+    assert np.array_equal(t[6].cpu().numpy().view(np.uint32), expect.view(np.uint32))
+
+
+def test_spgemm_wide_tiles():
+    a = random_csr(50, 40, 0.2, 31)
+    b = random_csr(40, 9000, 0.01, 32)
+    C0 = np.zeros((50, 9000), np.float32)
+    t = dev(*a, *b, C0)
+    kernels.spgemm_dense((t[0], t[1], t[2]), (t[3], t[4], t[5]), t[6])
+    expect = oracle.sparse_matrix_mul3(a, b, C0.copy())
+    assert np.array_equal(t[6].cpu().numpy(), expect)
+
+
+def test_spgemm_mtx_fixture():
+    rp, ci, va, (M, K) = read_mtx("ibm32.mtx")
+    C0 = np.zeros((M, K), np.float32)
+    t = dev(rp, ci, va, rp, ci, va, C0)
+    kernels.spgemm_dense((t[0], t[1], t[2]), (t[3], t[4], t[5]), t[6])
+    expect = oracle.sparse_matrix_mul3((rp, ci, va), (rp, ci, va), C0.copy())
+    assert np.array_equal(t[6].cpu().numpy(), expect)
+
+
+# ------------------------------------------------------------------ GEMM
+@pytest.mark.parametrize("m,n,k", [(64, 64, 32), (37, 5, 3), (130, 70, 50), (19717, 64, 500), (500, 64, 19717)])
+def test_gemm_bf16_mfma_layout(m, n, k):
+    """Checks the MFMA operand/accumulator lane layout against an FP64 GEMM of the bf16 inputs."""
+    case(f"gemm {m}x{n}x{k}")
+    r = np.random.default_rng(m + n + k)
+    a = torch.from_numpy(r.standard_normal((m, k)).astype(np.float32)).to(torch.bfloat16)
+    b = torch.from_numpy(r.standard_normal((k, n)).astype(np.float32)).to(torch.bfloat16)
+    ref = a.double() @ b.double()
+    bound = a.double().abs() @ b.double().abs()
+    out = kernels.gemm_bf16(a.cuda(), b.cuda()).cpu().double()
+    assert bool(((out - ref).abs() <= TOL * bound + 1e-30).all())
+    # transposed operands (strided views) and accumulate
+    at = a.t().contiguous().cuda().t()
+    bt = b.t().contiguous().cuda().t()
+    c = torch.ones(m, n, device="cuda")
+    kernels.gemm_bf16(at, bt, out=c, accumulate=True)
+    assert bool(((c.cpu().double() - ref - 1).abs() <= TOL * (bound + 1) + 1e-30).all())
+
+
+# ----------------------------------------------------------------- fills
+def test_device_uniform_fill_matches_host():
+    t = torch.empty(100_003, device="cuda")
+    kernels.fill_uniform_(t, 77, 16)
+    assert np.array_equal(t.cpu().numpy(), data.host_uniform(100_003, 77, 16))
+    kernels.fill_uniform_(t, 77, 16, 0.0, 3.0)
+    assert np.array_equal(t.cpu().numpy(), data.host_uniform(100_003, 77, 16, 0.0, 3.0))
+
+
+def test_bf16_conversion_matches_torch():
+    x = torch.from_numpy(np.random.default_rng(0).standard_normal(100_000).astype(np.float32) * 1e3)
+    x[:4] = torch.tensor([float("inf"), -0.0, 1e-40, 65504.0])
+    y = kernels.to_bf16(x.cuda()).cpu()
+    assert torch.equal(y.view(torch.int16), x.to(torch.bfloat16).view(torch.int16))
+
+
+# ------------------------------------------------------------- transpose
+def _check_transpose(rp, ci, va, K):
+    d = dev(rp, ci, va)
+    cp, ri, vt = kernels.csr_transpose(d[0], d[1], d[2], K)
+    ecp, eri, evt = oracle.csr_to_csc(rp, ci, va, K)
+    assert np.array_equal(cp.cpu().numpy(), ecp)
+    assert np.array_equal(ri.cpu().numpy(), eri)
+    assert np.array_equal(vt.cpu().numpy().view(np.uint32), evt.view(np.uint32))
+    return cp, ri, vt
+
+
+@pytest.mark.parametrize("shape,density", [((8, 7), None), ((300, 5), 0.3), ((513, 1000), 0.01),
+                                           ((100, 70000), 0.001), ((1, 1), 1.0)])
+def test_transpose_stable(fix8, shape, density):
+    case(f"transpose {shape}")
+    if density is None:
+        rp, ci, va = fix8
+    else:
+        rp, ci, va = random_csr(shape[0], shape[1], density, shape[0] + shape[1])
+    _check_transpose(rp, ci, va, shape[1])
+
+
+def test_transpose_power_law_and_roundtrip(rmat14):
+    a, (d_rp, d_ci, d_va) = rmat14
+    cp, ri, vt = _check_transpose(a.rowptr, a.colind, a.values, a.shape[1])
+    rp2, ci2, v2 = kernels.csr_transpose(cp, ri, vt, a.shape[0])
+    assert torch.equal(rp2, d_rp) and torch.equal(ci2, d_ci) and torch.equal(v2, d_va)
+
+
+def test_transpose_empty_and_valueless():
+    rp = torch.zeros(6, dtype=torch.int32, device="cuda")
+    ci = torch.zeros(0, dtype=torch.int32, device="cuda")
+    cp, ri, vt = kernels.csr_transpose(rp, ci, None, 9)
+    assert vt is None and ri.numel() == 0 and not cp.any() and cp.numel() == 10
+
+
+def test_transpose_full_rmat_1m():
+    """BASELINE size: exact equality with the stable host transpose, and twice = identity."""
+    case("transpose full 1M R-MAT")
+    a = rmat_1m_cached()
+    cp, ri, vt = _check_transpose(a.rowptr, a.colind, a.values, a.shape[1])
+    rp2, ci2, v2 = kernels.csr_transpose(cp, ri, vt, a.shape[0])
+    assert np.array_equal(rp2.cpu().numpy(), a.rowptr) and np.array_equal(ci2.cpu().numpy(), a.colind)

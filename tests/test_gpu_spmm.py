@@ -1,0 +1,213 @@
+"""GPU parity of the SpMM kernels against the oracle.
+
+Bar: the ROW schedule is bit-exact with the restated reference kernels (same
+per-element fmaf order); NNZ/MERGE schedules and reduced-precision B are
+within 1e-5 norm-wise relative error per row of the restatement and of the
+FP64 oracle (for bf16/fp8 B the oracle takes the dequantised B, so only fp32
+accumulation error remains)."""
+import numpy as np
+import pytest
+import torch
+
+import oracle
+from armsparse_amd import data, kernels
+from conftest import rmat_1m_cached, FIX8_ROWSUMS, random_csr
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-5
+
+
+def dev(*arrays):
+    return [None if a is None else torch.from_numpy(np.ascontiguousarray(a)).cuda() for a in arrays]
+
+
+def bits(t):
+    return (t.cpu().numpy() if torch.is_tensor(t) else t).view(np.uint32)
+
+
+def case(msg):
+    print(f"[case] {msg}", flush=True)
+
+
+def check_rows(out, restated, f64, tag=""):
+    """Per-row gate: within TOL of FP64, and within TOL of the CPU-order
+    restatement beyond that restatement's own distance from FP64 (long rows
+    summed sequentially in fp32 drift from FP64 by ~sqrt(n)·eps themselves)."""
+    e64 = oracle.row_rel_err(out, f64)
+    e_ref = oracle.row_rel_err(out, restated)
+    e_ref64 = oracle.row_rel_err(restated, f64)
+    print(f"{tag} max err vs fp64 {e64.max():.2e}, vs restated {e_ref.max():.2e}, "
+          f"restated vs fp64 {e_ref64.max():.2e}")
+    assert e64.max() <= TOL
+    assert np.all(e_ref <= TOL + e_ref64)
+
+
+def test_fix8_all_semantics(fix8):
+    rp, ci, va = fix8
+    d_rp, d_ci, d_va = dev(rp, ci, va)
+    B = torch.ones(7, 6, device="cuda")
+    out = kernels.spmm(d_rp, d_ci, d_va, B, config="row")
+    assert torch.equal(out[:, 0].cpu(), torch.from_numpy(FIX8_ROWSUMS))
+    out = kernels.spmm(d_rp, d_ci, d_va, B, config="merge")
+    assert torch.equal(out[:, 0].cpu(), torch.from_numpy(FIX8_ROWSUMS))
+    C = torch.zeros(8, 6, device="cuda")
+    kernels.spmm(d_rp, d_ci, d_va, B, out=C, accumulate=True, trunc4=True, ncols=4)
+    expect = np.zeros((8, 6), np.float32)
+    oracle.rowBaseSpmm(rp, ci, va, np.ones((7, 6), np.float32), expect)
+    assert np.array_equal(C.cpu().numpy(), expect)
+
+
+@pytest.mark.parametrize("N", [1, 2, 3, 8, 31, 64, 100, 128, 256, 300])
+def test_row_schedule_bitexact(N):
+    case(f"row N={N}")
+    rp, ci, va = random_csr(600, 500, 0.02, 100 + N, empty_rows=0.1)
+    r = np.random.default_rng(N)
+    B = r.uniform(-1, 1, (500, N)).astype(np.float32)
+    C0 = r.uniform(-1, 1, (600, N)).astype(np.float32)
+    d_rp, d_ci, d_va, d_B, d_C = dev(rp, ci, va, B, C0)
+    kernels.spmm(d_rp, d_ci, d_va, d_B, out=d_C, accumulate=True, config="row")
+    assert np.array_equal(bits(d_C), bits(oracle.rowBaseSpmm3(rp, ci, va, B, C0.copy())))
+    out = kernels.spmm(d_rp, d_ci, d_va, d_B, config="row")
+    assert np.array_equal(bits(out), bits(oracle.origin(rp, ci, va, B, np.zeros_like(C0))))
+
+
+@pytest.mark.parametrize("group", [4, 8, 16, 32, 64])
+@pytest.mark.parametrize("vec", [1, 2, 4])
+def test_row_schedule_tiles_bitexact(group, vec):
+    case(f"row G={group} V={vec}")
+    rp, ci, va = random_csr(300, 256, 0.05, group * 10 + vec)
+    B = np.random.default_rng(1).uniform(-1, 1, (256, 96)).astype(np.float32)
+    d_rp, d_ci, d_va, d_B = dev(rp, ci, va, B)
+    out = kernels.spmm(d_rp, d_ci, d_va, d_B, config=dict(schedule="row", group=group, vec=vec))
+    assert np.array_equal(bits(out), bits(oracle.origin(rp, ci, va, B, np.zeros((300, 96), np.float32))))
+
+
+def test_reference_tail_semantics():
+    """rowBaseSpmm (trunc4 + N%4) and irregular (first 32·floor(N/32) columns) are bit-exact."""
+    rp, ci, va = random_csr(200, 150, 0.05, 77)
+    r = np.random.default_rng(77)
+    for N in (6, 45, 70):
+        B = r.uniform(-1, 1, (150, N)).astype(np.float32)
+        C0 = r.uniform(-1, 1, (200, N)).astype(np.float32)
+        d_rp, d_ci, d_va, d_B, d_C = dev(rp, ci, va, B, C0)
+        kernels.spmm(d_rp, d_ci, d_va, d_B, out=d_C, accumulate=True, trunc4=True, ncols=N & ~3)
+        assert np.array_equal(bits(d_C), bits(oracle.rowBaseSpmm(rp, ci, va, B, C0.copy())))
+        d_C = dev(C0)[0]
+        kernels.spmm(d_rp, d_ci, d_va, d_B, out=d_C, accumulate=True, ncols=N & ~31, config="row")
+        assert np.array_equal(bits(d_C), bits(oracle.irregular(rp, ci, va, B, C0.copy())))
+
+
+SPLIT_CFGS = [dict(schedule=s, chunk=c, unroll=u) for s in ("merge", "nnz") for c in (64, 512, 2048)
+              for u in (4, 8, 16)]
+
+
+@pytest.fixture(scope="module")
+def rmat14():
+    a = data.rmat_csr(14, 1 << 18, seed=1234)
+    return a, dev(a.rowptr, a.colind, a.values)
+
+
+@pytest.mark.parametrize("cfg", SPLIT_CFGS, ids=lambda c: f"{c['schedule']}-c{c['chunk']}-u{c['unroll']}")
+def test_split_schedules_power_law(rmat14, cfg):
+    case(f"split {cfg}")
+    a, (d_rp, d_ci, d_va) = rmat14
+    N = 128
+    B = np.random.default_rng(5).uniform(-1, 1, (a.shape[1], N)).astype(np.float32)
+    C0 = np.random.default_rng(6).uniform(-1, 1, (a.shape[0], N)).astype(np.float32)
+    d_B, d_C = dev(B, C0)
+    out = kernels.spmm(d_rp, d_ci, d_va, d_B, config=cfg).cpu().numpy()
+    restated = oracle.origin(a.rowptr, a.colind, a.values, B, np.zeros_like(C0))
+    f64 = oracle.spmm_f64(a.rowptr, a.colind, a.values, B)
+    check_rows(out, restated, f64, str(cfg))
+    empty = np.diff(a.rowptr) == 0
+    assert not out[empty].any()
+    kernels.spmm(d_rp, d_ci, d_va, d_B, out=d_C, accumulate=True, config=cfg)
+    f64acc = oracle.spmm_f64(a.rowptr, a.colind, a.values, B, C0)
+    assert np.max(oracle.row_rel_err(d_C.cpu().numpy(), f64acc)) <= TOL
+
+
+@pytest.mark.parametrize("N", [1, 16, 32, 64, 96, 256, 512, 520])
+@pytest.mark.parametrize("sched", ["merge", "nnz"])
+def test_split_widths(rmat14, N, sched):
+    case(f"split width N={N} {sched}")
+    a, (d_rp, d_ci, d_va) = rmat14
+    B = np.random.default_rng(N).uniform(-1, 1, (a.shape[1], N)).astype(np.float32)
+    out = kernels.spmm(d_rp, d_ci, d_va, dev(B)[0], config=sched).cpu().numpy()
+    ref = oracle.spmm_f64(a.rowptr, a.colind, a.values, B)
+    if N >= 16:
+        assert np.max(oracle.row_rel_err(out, ref)) <= TOL
+    else:  # a handful of columns: gate each element on its dot product's scale Σ|a·b|
+        scale = oracle.spmm_f64(a.rowptr, a.colind, np.abs(a.values), np.abs(B))
+        assert np.all(np.abs(out - ref) <= TOL * scale + 1e-30)
+
+
+def test_value_less_and_empty():
+    rp, ci, _ = random_csr(300, 200, 0.05, 9, empty_rows=0.3)
+    B = np.random.default_rng(9).uniform(-1, 1, (200, 64)).astype(np.float32)
+    d_rp, d_ci, d_B = dev(rp, ci, B)
+    ones = np.ones(len(ci), np.float32)
+    for cfg in ("row", "merge", "nnz"):
+        out = kernels.spmm(d_rp, d_ci, None, d_B, config=cfg).cpu().numpy()
+        assert np.max(oracle.row_rel_err(out, oracle.spmm_f64(rp, ci, ones, B))) <= TOL
+    # no nonzeros at all: overwrite gives zeros, accumulate leaves C
+    z_rp = torch.zeros(301, dtype=torch.int32, device="cuda")
+    z_ci = torch.zeros(0, dtype=torch.int32, device="cuda")
+    out = kernels.spmm(z_rp, z_ci, None, d_B)
+    assert out.shape == (300, 64) and not out.any()
+    C = torch.full((300, 64), 3.0, device="cuda")
+    kernels.spmm(z_rp, z_ci, None, d_B, out=C, accumulate=True)
+    assert bool((C == 3.0).all())
+
+
+@pytest.mark.parametrize("sched", ["row", "merge"])
+def test_bf16_dense(rmat14, sched):
+    a, (d_rp, d_ci, d_va) = rmat14
+    B = torch.from_numpy(np.random.default_rng(3).uniform(-1, 1, (a.shape[1], 128)).astype(np.float32))
+    Bb = B.to(torch.bfloat16)
+    out = kernels.spmm(d_rp, d_ci, d_va, Bb.cuda(), config=sched).cpu().numpy()
+    deq = Bb.float().numpy()
+    assert np.max(oracle.row_rel_err(out, oracle.spmm_f64(a.rowptr, a.colind, a.values, deq))) <= TOL
+    err32 = np.max(oracle.row_rel_err(out, oracle.spmm_f64(a.rowptr, a.colind, a.values, B.numpy())))
+    print(f"bf16 B: max row rel err vs fp32-B FP64 oracle = {err32:.3e}")
+    assert err32 < 1e-2
+
+
+def test_fp8_quantiser_matches_torch():
+    B = torch.from_numpy(np.random.default_rng(4).standard_normal((2000, 128)).astype(np.float32))
+    B[7] = 0.0  # zero row: scale 1
+    B[9, :5] = torch.tensor([1e-8, -3e-6, 448.0, 1e4, -1e4])
+    q, sc = kernels.quantize_fp8_rows(B.cuda())
+    q, sc = q.cpu(), sc.cpu()
+    expect_sc = B.abs().amax(1) / 448.0
+    expect_sc[expect_sc == 0] = 1.0
+    assert torch.equal(sc, expect_sc)
+    expect_q = (B / expect_sc[:, None]).to(torch.float8_e4m3fn).view(torch.uint8)
+    assert torch.equal(q, expect_q)
+
+
+@pytest.mark.parametrize("sched", ["row", "merge"])
+def test_fp8_dense(rmat14, sched):
+    a, (d_rp, d_ci, d_va) = rmat14
+    B = torch.from_numpy(np.random.default_rng(8).uniform(-1, 1, (a.shape[1], 128)).astype(np.float32)).cuda()
+    q, sc = kernels.quantize_fp8_rows(B)
+    out = kernels.spmm(d_rp, d_ci, d_va, q, dense_scale=sc, config=sched).cpu().numpy()
+    deq = (q.cpu().view(torch.float8_e4m3fn).float() * sc.cpu()[:, None]).numpy()
+    assert np.max(oracle.row_rel_err(out, oracle.spmm_f64(a.rowptr, a.colind, a.values, deq))) <= TOL
+    err32 = np.max(oracle.row_rel_err(out, oracle.spmm_f64(a.rowptr, a.colind, a.values, B.cpu().numpy())))
+    print(f"fp8 B: max row rel err vs fp32-B FP64 oracle = {err32:.3e}")
+    assert err32 < 0.1
+
+
+def test_full_size_rmat_1m_n128():
+    """BASELINE size: 1M×1M R-MAT, 2^24 nnz, N=128, every row: MERGE against the
+    FP64 oracle and against the CPU-order ROW kernel (bit-exact with the
+    restated reference by the tests above)."""
+    case("full 1M R-MAT N=128")
+    a = rmat_1m_cached()
+    d_rp, d_ci, d_va = dev(a.rowptr, a.colind, a.values)
+    B = torch.empty((a.shape[1], 128), device="cuda")
+    kernels.fill_uniform_(B, data.CFG_SEED + 2, data.STREAM_DENSE)
+    merged = kernels.spmm(d_rp, d_ci, d_va, B, config="merge").cpu().numpy()
+    rowwise = kernels.spmm(d_rp, d_ci, d_va, B, config="row").cpu().numpy()
+    f64 = oracle.spmm_f64(a.rowptr, a.colind, a.values, B.cpu().numpy())
+    check_rows(merged, rowwise, f64, "1M R-MAT merge")
