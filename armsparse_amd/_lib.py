@@ -88,6 +88,15 @@ _SIGS = {
     "armsp_quantize_fp8_rows": (c_int, [_I, _I, _P, _I, _P, _P, _P]),
     "armsp_fill_uniform_f32": (c_int, [c_int64, c_uint64, c_uint64, c_float, c_float, _P, _P]),
     "armsp_fill_normal_f32": (c_int, [c_int64, c_uint64, c_uint64, _P, _P]),
+    # learned schedule selector
+    "armsp_row_stats": (c_int, [_I, _P, _I, _P, _P, _P]),
+    "armsp_sgemm_f32": (c_int, [_I, _I, _I, _P, c_int64, c_int64, _P, c_int64, c_int64, _P, c_int64, c_float, _P,
+                                _I, _P]),
+    "armsp_softmax_xent": (c_int, [_I, _I, _P, _P, _P, _P, _P, _P]),
+    "armsp_relu_backward": (c_int, [c_int64, _P, _P, _P]),
+    "armsp_colsum_f32": (c_int, [_I, _I, _P, _P, _P]),
+    "armsp_adam_step": (c_int, [c_int64, _P, _P, _P, _P, c_float, c_float, c_float, c_float, c_float, _I, _P]),
+    "armsp_mlp_infer": (c_int, [_I, _I, _I, _I, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P]),
     # host-side generators and format helpers (armsp_gen.h)
     "armsp_rng_u64": (c_uint64, [c_uint64, c_uint64, c_uint64]),
     "armsp_rng_uniform": (c_float, [c_uint64, c_uint64, c_uint64, c_float, c_float]),

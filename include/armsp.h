@@ -13,9 +13,10 @@
  * armsparse_compat.h; they are thin shims over these functions.
  *
  * Workspace: entry points whose schedule needs a carry buffer (nnz-split and
- * merge-path SpMM / SpMV, split-K GEMM, transpose) take it from the device's
- * stream-ordered memory pool on the caller's stream (hipMallocAsync /
- * hipFreeAsync), so concurrent calls on different streams never share it.
+ * merge-path SpMM / SpMV, split-K GEMM, transpose) take it from the library's
+ * own stream-ordered memory pool on the caller's stream
+ * (hipMallocFromPoolAsync / hipFreeAsync), so concurrent calls on different
+ * streams never share it and the device's default pool is left untouched.
  */
 #ifndef ARMSP_H
 #define ARMSP_H
@@ -183,6 +184,37 @@ int armsp_fill_uniform_f32(int64_t n, uint64_t seed, uint64_t stream_id,
                            float lo, float hi, float* out, void* stream);
 int armsp_fill_normal_f32(int64_t n, uint64_t seed, uint64_t stream_id,
                           float* out, void* stream);
+
+/* ------------------------------------------- learned schedule selector */
+/* No reference counterpart: the reference picks its SpMM kernel by a fixed N
+ * threshold (gcn/pytorch-custom/spmm_kernel.cu:437-457).  These kernels
+ * compute the selector's sparsity features and train / run its MLP. */
+/* Row-length histogram: hist[0] empty rows, hist[b] rows with
+ * 2^(b-1) <= len < 2^b, hist[nbins-1] everything longer; *maxlen = longest
+ * row.  hist and maxlen are device int arrays (overwritten). */
+int armsp_row_stats(int M, const int* rowptr, int nbins, int* hist, int* maxlen, void* stream);
+/* C = op(A)·op(B) (+ beta·C) (+ bias[n]) (then ReLU if relu != 0), fp32,
+ * strided operands as in armsp_gemm_bf16. */
+int armsp_sgemm_f32(int M, int N, int K, const float* A, int64_t sam, int64_t sak, const float* B, int64_t sbk,
+                    int64_t sbn, float* C, int64_t ldc, float beta, const float* bias, int relu, void* stream);
+/* Mean softmax cross-entropy of logits[rows×classes] into *loss (device)
+ * against hard labels[rows] or, when targets is non-NULL, soft target rows
+ * targets[rows×classes]; dlogits (nullable) = (softmax - target) / rows. */
+int armsp_softmax_xent(int rows, int classes, const float* logits, const int* labels, const float* targets,
+                       float* dlogits, float* loss, void* stream);
+/* grad[i] = 0 where act[i] <= 0. */
+int armsp_relu_backward(int64_t n, const float* act, float* grad, void* stream);
+/* out[n] = sum over m of X[m][n] (rows summed in order). */
+int armsp_colsum_f32(int M, int N, const float* X, float* out, void* stream);
+/* One Adam step (L2 weight decay added to the gradient); step counts from 1. */
+int armsp_adam_step(int64_t n, float* param, const float* grad, float* m, float* v, float lr, float beta1,
+                    float beta2, float eps, float weight_decay, int step, void* stream);
+/* MLP inference: x[nvec×F] standardised by (mu, sd), h1 = relu(x·W1 + b1),
+ * h2 = relu(h1·W2 + b2), logits = h2·W3 + b3 (W row-major [in×out]);
+ * cls[v] = argmax; logits (nullable) written.  F, H, C <= 256. */
+int armsp_mlp_infer(int nvec, int F, int H, int C, const float* x, const float* mu, const float* sd, const float* W1,
+                    const float* b1, const float* W2, const float* b2, const float* W3, const float* b3,
+                    float* logits, int* cls, void* stream);
 
 #ifdef __cplusplus
 }
