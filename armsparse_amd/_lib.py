@@ -61,6 +61,14 @@ class Config(ctypes.Structure):
                     chunk=self.chunk, unroll=self.unroll)
 
 
+class Epilogue(ctypes.Structure):
+    """armsp_epilogue: fused column scale, row scale, bias and ReLU of a SpMM."""
+
+    _fields_ = [("col_scale", c_void_p), ("row_scale", c_void_p), ("bias", c_void_p), ("relu", ctypes.c_int32)]
+
+
+DTYPE_F32, DTYPE_BF16, DTYPE_FP8_E4M3 = 0, 1, 2
+
 _P = c_void_p  # device or host pointer
 _I = c_int
 _CFG = POINTER(Config)
@@ -76,6 +84,8 @@ _SIGS = {
     "armsp_spmm_f32": (c_int, [_I, _I, _I, _I, _P, _P, _P, _P, _I, _P, _I, _I, _CFG, _P]),
     "armsp_spmm_bf16": (c_int, [_I, _I, _I, _I, _P, _P, _P, _P, _I, _P, _I, _I, _CFG, _P]),
     "armsp_spmm_fp8": (c_int, [_I, _I, _I, _I, _P, _P, _P, _P, _P, _I, _P, _I, _I, _CFG, _P]),
+    "armsp_spmm_ex": (c_int, [_I, _I, _I, _I, _P, _P, _P, _I, _P, _P, _I, _P, _I, _I, POINTER(Epilogue), _CFG,
+                              _P]),
     "armsp_spmv_f32": (c_int, [_I, _I, _I, _P, _P, _P, _P, _P, _I, _CFG, _P]),
     "armsp_spmv_vcsr_f32": (c_int, [_I, _I, _P, _P, _P, _P, _P, _I, _P]),
     "armsp_sddmm_csr_f32": (c_int, [_I, _I, _I, _I, _P, _P, _P, _P, _I, _P, _I, _P, _CFG, _P]),

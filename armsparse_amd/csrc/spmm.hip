@@ -88,18 +88,53 @@ __device__ __forceinline__ void store_c(float* __restrict__ p, const float (&o)[
     }
 }
 
-template <typename TB>
-__device__ __forceinline__ float coef(const float* __restrict__ val, const float* __restrict__ bscale, int idx, int c) {
-    float a = val ? val[idx] : 1.0f;
-    if constexpr (sizeof(TB) == 1) a *= bscale[c];
+// Optional operand scales and output epilogue (armsp_epilogue + the fp8
+// per-row scale of B).  coefficient of nonzero (r, c): val · bscale[c] ·
+// cscale[c]; output row r: rscale[r] · (A·B)[r] + bias, then ReLU.
+struct Epi {
+    const float* bscale;  // fp8 B row scales
+    const float* cscale;  // column scale of A
+    const float* rscale;  // output row scale
+    const float* bias;    // per-column bias
+    int relu;
+};
+
+__device__ __forceinline__ float coef(float a, const Epi& ep, int c) {
+    if (ep.bscale) a *= ep.bscale[c];
+    if (ep.cscale) a *= ep.cscale[c];
     return a;
+}
+
+// out = rscale·acc + bias (+ C_old) (ReLU when `final`)
+template <int V>
+__device__ __forceinline__ void finish(float (&o)[V], const float (&acc)[V], const Epi& ep, int r, int col,
+                                       const float* cold, bool final) {
+#pragma unroll
+    for (int v = 0; v < V; ++v) o[v] = acc[v];
+    if (ep.rscale) {
+        const float rs = ep.rscale[r];
+#pragma unroll
+        for (int v = 0; v < V; ++v) o[v] *= rs;
+    }
+    if (ep.bias) {
+#pragma unroll
+        for (int v = 0; v < V; ++v) o[v] += ep.bias[col + v];
+    }
+    if (cold) {
+#pragma unroll
+        for (int v = 0; v < V; ++v) o[v] += cold[v];
+    }
+    if (final && ep.relu) {
+#pragma unroll
+        for (int v = 0; v < V; ++v) o[v] = fmaxf(o[v], 0.0f);
+    }
 }
 
 // ------------------------------------------------------------- ROW split
 template <typename TB, int V, int G>
 __global__ __launch_bounds__(256) void k_spmm_row(int M, int N, const int* __restrict__ rowptr,
                                                   const int* __restrict__ colind, const float* __restrict__ val,
-                                                  const TB* __restrict__ B, int ldb, const float* __restrict__ bscale,
+                                                  const TB* __restrict__ B, int ldb, Epi ep,
                                                   float* __restrict__ C, int ldc, int flags) {
     constexpr int U = G < 4 ? G : 4;
     const int lane = threadIdx.x & (G - 1);
@@ -111,10 +146,13 @@ __global__ __launch_bounds__(256) void k_spmm_row(int M, int N, const int* __res
     const int beg = rowptr[row];
     int end = rowptr[row + 1];
     if (flags & ARMSP_TRUNC4) end = beg + ((end - beg) & ~3);
+    // Without an epilogue the accumulation starts from C (the reference's
+    // fmaf sequence, bit-exact); with one it starts from 0 and C is added last.
+    const bool plain = !ep.rscale && !ep.bias && !ep.relu;
     float acc[V];
 #pragma unroll
     for (int v = 0; v < V; ++v) acc[v] = 0.0f;
-    if ((flags & ARMSP_ACCUMULATE) && active) load_c<V>(C + (size_t)row * ldc + col, acc);
+    if (plain && (flags & ARMSP_ACCUMULATE) && active) load_c<V>(C + (size_t)row * ldc + col, acc);
 
     for (int base = beg; base < end; base += G) {
         const int n = min(G, end - base);
@@ -122,7 +160,7 @@ __global__ __launch_bounds__(256) void k_spmm_row(int M, int N, const int* __res
         float mv = 0.0f;
         if (lane < n) {
             mc = colind[base + lane];
-            mv = coef<TB>(val, bscale, base + lane, mc);
+            mv = coef(val ? val[base + lane] : 1.0f, ep, mc);
         }
         for (int j = 0; j < n; j += U) {
             float b[U][V];
@@ -149,7 +187,17 @@ __global__ __launch_bounds__(256) void k_spmm_row(int M, int N, const int* __res
                 }
         }
     }
-    if (active) store_c<V>(C + (size_t)row * ldc + col, acc);
+    if (!active) return;
+    float* cp = C + (size_t)row * ldc + col;
+    if (plain) {
+        store_c<V>(cp, acc);
+    } else {
+        float o[V], cold[V];
+        const bool accm = (flags & ARMSP_ACCUMULATE) != 0;
+        if (accm) load_c<V>(cp, cold);
+        finish<V>(o, acc, ep, row, col, accm ? cold : nullptr, true);
+        store_c<V>(cp, o);
+    }
 }
 
 // ----------------------------------------------------- NNZ / MERGE split
@@ -160,7 +208,7 @@ __global__ __launch_bounds__(256) void k_spmm_row(int M, int N, const int* __res
 template <typename TB, int V, int U, bool MERGE>
 __global__ __launch_bounds__(256) void k_spmm_split(int M, int N, int nnz, int chunk, const int* __restrict__ rowptr,
                                                     const int* __restrict__ colind, const float* __restrict__ val,
-                                                    const TB* __restrict__ B, int ldb, const float* __restrict__ bscale,
+                                                    const TB* __restrict__ B, int ldb, Epi ep,
                                                     float* __restrict__ C, int ldc, int flags,
                                                     int* __restrict__ carry_row, float* __restrict__ carry_val) {
     const int lane = threadIdx.x & 63;
@@ -186,33 +234,43 @@ __global__ __launch_bounds__(256) void k_spmm_split(int M, int N, int nnz, int c
     const int col = (int)(blockIdx.y * 64u + lane) * V;
     const bool active = col < N;
 
-    // register windows: nonzeros [jw, jw+64), row ends of rows [rw, rw+64)
-    int jw = j0, wc = 0;
-    float wv = 0.0f;
-    if (jw + lane < j1) {
-        wc = colind[jw + lane];
-        wv = coef<TB>(val, bscale, jw + lane, wc);
-    }
+    // Register windows, double-buffered: nonzeros [jw, jw+64) and the next 64,
+    // row ends of rows [rw, rw+64) and the next 64.  The next window is
+    // requested as soon as the current one is taken, 64 items ahead of use.
+    // The CSR streams are read once: non-temporal loads keep them from
+    // evicting hot dense rows from the L2.
+    auto load_nz = [&](int base, int& c, float& a) {
+        c = 0;
+        a = 0.0f;
+        if (base + lane < j1) {
+            c = __builtin_nontemporal_load(colind + base + lane);
+            a = coef(val ? __builtin_nontemporal_load(val + base + lane) : 1.0f, ep, c);
+        }
+    };
+    auto load_re = [&](int base) {
+        return (base + lane < M) ? __builtin_nontemporal_load(rowptr + base + 1 + lane) : INT_MAX;
+    };
+    int jw = j0, wc, nc;
+    float wv, nv;
+    load_nz(jw, wc, wv);
+    load_nz(jw + 64, nc, nv);
     int r = i0, rw = i0;
-    int wre = (rw + lane < M) ? rowptr[rw + 1 + lane] : INT_MAX;
+    int wre = load_re(rw), nre = load_re(rw + 64);
     int re = read_lane(wre, 0);
 
     float acc[V];
 #pragma unroll
     for (int v = 0; v < V; ++v) acc[v] = 0.0f;
 
+    // a row whose first nonzeros belong to an earlier wave gets its carries
+    // (and its ReLU) from the fixup kernel
+    const int carry_in_row = (i0 < M && rowptr[i0] < j0) ? i0 : -1;
     auto emit = [&]() {
         if (active) {
             float* cp = C + (size_t)r * ldc + col;
-            float o[V];
-            if (acc_mode) {
-                load_c<V>(cp, o);
-#pragma unroll
-                for (int v = 0; v < V; ++v) o[v] += acc[v];
-            } else {
-#pragma unroll
-                for (int v = 0; v < V; ++v) o[v] = acc[v];
-            }
+            float o[V], cold[V];
+            if (acc_mode) load_c<V>(cp, cold);
+            finish<V>(o, acc, ep, r, col, acc_mode ? cold : nullptr, r != carry_in_row);
             store_c<V>(cp, o);
         }
 #pragma unroll
@@ -220,7 +278,8 @@ __global__ __launch_bounds__(256) void k_spmm_split(int M, int N, int nnz, int c
         ++r;
         if (r - rw == 64) {
             rw = r;
-            wre = (rw + lane < M) ? rowptr[rw + 1 + lane] : INT_MAX;
+            wre = nre;
+            nre = load_re(rw + 64);
         }
         re = read_lane(wre, r - rw);
     };
@@ -228,12 +287,9 @@ __global__ __launch_bounds__(256) void k_spmm_split(int M, int N, int nnz, int c
     for (int jb = j0; jb < j1; jb += U) {
         if (jb - jw == 64) {
             jw = jb;
-            wc = 0;
-            wv = 0.0f;
-            if (jw + lane < j1) {
-                wc = colind[jw + lane];
-                wv = coef<TB>(val, bscale, jw + lane, wc);
-            }
+            wc = nc;
+            wv = nv;
+            load_nz(jw + 64, nc, nv);
         }
         float b[U][V];
         float a[U];
@@ -276,7 +332,7 @@ __global__ __launch_bounds__(256) void k_spmm_split(int M, int N, int nnz, int c
 template <int V>
 __global__ __launch_bounds__(256) void k_carry_fixup(int nwaves, int N, const int* __restrict__ carry_row,
                                                      const float* __restrict__ carry_val, float* __restrict__ C,
-                                                     int ldc) {
+                                                     int ldc, Epi ep) {
     const int lane = threadIdx.x & 63;
     const int w = uniform((int)(blockIdx.x * 4u + (threadIdx.x >> 6)));
     if (w >= nwaves) return;
@@ -315,8 +371,13 @@ __global__ __launch_bounds__(256) void k_carry_fixup(int nwaves, int N, const in
         float* cp = C + (size_t)r * ldc + col;
         float o[V];
         load_c<V>(cp, o);
+        const float rs = ep.rscale ? ep.rscale[r] : 1.0f;
 #pragma unroll
-        for (int v = 0; v < V; ++v) o[v] += s[v];
+        for (int v = 0; v < V; ++v) o[v] += rs * s[v];
+        if (ep.relu) {
+#pragma unroll
+            for (int v = 0; v < V; ++v) o[v] = fmaxf(o[v], 0.0f);
+        }
 #pragma unroll
         for (int v = 0; v < V; ++v) cp[v] = o[v];
     }
@@ -343,7 +404,7 @@ int pick_vec(int N, int ldb, int ldc, const TB* B, const float* C, int want, int
 
 template <typename TB, int V>
 int launch_row_v(int G, dim3 grid, hipStream_t s, int M, int N, const int* rp, const int* ci, const float* va,
-                 const TB* B, int ldb, const float* bs, float* C, int ldc, int flags) {
+                 const TB* B, int ldb, const Epi& bs, float* C, int ldc, int flags) {
     switch (G) {
 #define ARMSP_ROW_CASE(GG)                                                                                   \
     case GG:                                                                                                 \
@@ -363,7 +424,7 @@ int launch_row_v(int G, dim3 grid, hipStream_t s, int M, int N, const int* rp, c
 
 template <typename TB, int V, bool MERGE>
 int launch_split_vu(int U, dim3 grid, hipStream_t s, int M, int N, int nnz, int chunk, const int* rp, const int* ci,
-                    const float* va, const TB* B, int ldb, const float* bs, float* C, int ldc, int flags, int* crow,
+                    const float* va, const TB* B, int ldb, const Epi& bs, float* C, int ldc, int flags, int* crow,
                     float* cval) {
     switch (U) {
 #define ARMSP_SPLIT_CASE(UU)                                                                                \
@@ -382,12 +443,15 @@ int launch_split_vu(int U, dim3 grid, hipStream_t s, int M, int N, int nnz, int 
 
 template <typename TB>
 int spmm_impl(int M, int K, int N, int nnz, const int* rowptr, const int* colind, const float* val, const TB* B,
-              const float* bscale, int ldb, float* C, int ldc, int flags, const armsp_config* cfg, void* stream) {
+              const float* b_row_scale, int ldb, float* C, int ldc, int flags, const armsp_epilogue* epi,
+              const armsp_config* cfg, void* stream) {
     if (M < 0 || K < 0 || N < 0 || nnz < 0) return ARMSP_ERROR_INVALID_VALUE;
     if (M == 0 || N == 0) return ARMSP_SUCCESS;
     if (!rowptr || !C || ldc < N) return ARMSP_ERROR_INVALID_VALUE;
     if (nnz > 0 && (!colind || !B || ldb < N || K == 0)) return ARMSP_ERROR_INVALID_VALUE;
-    if (sizeof(TB) == 1 && nnz > 0 && !bscale) return ARMSP_ERROR_INVALID_VALUE;
+    if (sizeof(TB) == 1 && nnz > 0 && !b_row_scale) return ARMSP_ERROR_INVALID_VALUE;
+    Epi bscale{sizeof(TB) == 1 ? b_row_scale : nullptr, epi ? epi->col_scale : nullptr,
+               epi ? epi->row_scale : nullptr, epi ? epi->bias : nullptr, epi ? epi->relu : 0};
     if (flags & ~(ARMSP_ACCUMULATE | ARMSP_TRUNC4)) return ARMSP_ERROR_INVALID_VALUE;
     if ((int64_t)M + nnz > INT_MAX) return ARMSP_ERROR_NOT_SUPPORTED;
     hipStream_t s = as_stream(stream);
@@ -490,9 +554,9 @@ int spmm_impl(int M, int K, int N, int nnz, const int* rowptr, const int* colind
         const int fv = (N % 4 == 0 && ldc % 4 == 0 && aligned(C, 16)) ? 4 : 1;
         dim3 fgrid((unsigned)((nwaves + 3) / 4), 1);
         if (fv == 4)
-            hipLaunchKernelGGL((k_carry_fixup<4>), fgrid, dim3(256), 0, s, (int)nwaves, N, crow, cval, C, ldc);
+            hipLaunchKernelGGL((k_carry_fixup<4>), fgrid, dim3(256), 0, s, (int)nwaves, N, crow, cval, C, ldc, bscale);
         else
-            hipLaunchKernelGGL((k_carry_fixup<1>), fgrid, dim3(256), 0, s, (int)nwaves, N, crow, cval, C, ldc);
+            hipLaunchKernelGGL((k_carry_fixup<1>), fgrid, dim3(256), 0, s, (int)nwaves, N, crow, cval, C, ldc, bscale);
         hipError_t e = hipGetLastError();
         ws_free(ws, s);
         if (e != hipSuccess) return hip_fail(e);
@@ -508,19 +572,39 @@ extern "C" {
 
 int armsp_spmm_f32(int M, int K, int N, int nnz, const int* rowptr, const int* colind, const float* val,
                    const float* B, int ldb, float* C, int ldc, int flags, const armsp_config* cfg, void* stream) {
-    return armsp::spmm_impl<float>(M, K, N, nnz, rowptr, colind, val, B, nullptr, ldb, C, ldc, flags, cfg, stream);
+    return armsp::spmm_impl<float>(M, K, N, nnz, rowptr, colind, val, B, nullptr, ldb, C, ldc, flags, nullptr, cfg,
+                                   stream);
 }
 
 int armsp_spmm_bf16(int M, int K, int N, int nnz, const int* rowptr, const int* colind, const float* val,
                     const uint16_t* B, int ldb, float* C, int ldc, int flags, const armsp_config* cfg, void* stream) {
-    return armsp::spmm_impl<uint16_t>(M, K, N, nnz, rowptr, colind, val, B, nullptr, ldb, C, ldc, flags, cfg, stream);
+    return armsp::spmm_impl<uint16_t>(M, K, N, nnz, rowptr, colind, val, B, nullptr, ldb, C, ldc, flags, nullptr, cfg,
+                                      stream);
 }
 
 int armsp_spmm_fp8(int M, int K, int N, int nnz, const int* rowptr, const int* colind, const float* val,
                    const uint8_t* Bq, const float* b_row_scale, int ldb, float* C, int ldc, int flags,
                    const armsp_config* cfg, void* stream) {
-    return armsp::spmm_impl<uint8_t>(M, K, N, nnz, rowptr, colind, val, Bq, b_row_scale, ldb, C, ldc, flags, cfg,
-                                     stream);
+    return armsp::spmm_impl<uint8_t>(M, K, N, nnz, rowptr, colind, val, Bq, b_row_scale, ldb, C, ldc, flags, nullptr,
+                                     cfg, stream);
+}
+
+int armsp_spmm_ex(int M, int K, int N, int nnz, const int* rowptr, const int* colind, const float* val, int b_dtype,
+                  const void* B, const float* b_row_scale, int ldb, float* C, int ldc, int flags,
+                  const armsp_epilogue* epilogue, const armsp_config* cfg, void* stream) {
+    switch (b_dtype) {
+        case ARMSP_DTYPE_F32:
+            return armsp::spmm_impl<float>(M, K, N, nnz, rowptr, colind, val, static_cast<const float*>(B), nullptr,
+                                           ldb, C, ldc, flags, epilogue, cfg, stream);
+        case ARMSP_DTYPE_BF16:
+            return armsp::spmm_impl<uint16_t>(M, K, N, nnz, rowptr, colind, val, static_cast<const uint16_t*>(B),
+                                              nullptr, ldb, C, ldc, flags, epilogue, cfg, stream);
+        case ARMSP_DTYPE_FP8_E4M3:
+            return armsp::spmm_impl<uint8_t>(M, K, N, nnz, rowptr, colind, val, static_cast<const uint8_t*>(B),
+                                             b_row_scale, ldb, C, ldc, flags, epilogue, cfg, stream);
+        default:
+            return ARMSP_ERROR_INVALID_VALUE;
+    }
 }
 
 }  // extern "C"

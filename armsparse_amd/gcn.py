@@ -54,10 +54,12 @@ def make_task(precision: str = "fp32", seed: int = data.CFG_SEED + 3, n: int = 1
     g = data.gcn_graph(n, pairs, seed)
     r = np.random.default_rng(seed)
     x = np.zeros((n, features), np.float32)
-    mask = r.random((n, features)) < feat_density
-    x[mask] = r.random(int(mask.sum())).astype(np.float32)
+    for lo in range(0, n, 1 << 18):  # in row blocks to bound host temporaries
+        hi = min(n, lo + (1 << 18))
+        blk = r.random((hi - lo, features), dtype=np.float32)
+        x[lo:hi] = np.where(r.random((hi - lo, features), dtype=np.float32) < feat_density, blk, 0.0)
     rs = x.sum(1, keepdims=True)
-    x = np.divide(x, rs, out=np.zeros_like(x), where=rs > 0)  # NormalizeFeatures
+    np.divide(x, rs, out=x, where=rs > 0)  # NormalizeFeatures
     y = r.integers(0, classes, n)
     train = np.zeros(n, bool)
     train[r.choice(n, 20 * classes, replace=False)] = True
@@ -71,6 +73,9 @@ def make_task(precision: str = "fp32", seed: int = data.CFG_SEED + 3, n: int = 1
     opt = torch.optim.Adam([dict(params=model.reg_params, weight_decay=5e-4),
                             dict(params=model.non_reg_params, weight_decay=0)], lr=0.01)
     return GCNTask(model, opt, xt, graph, t(y), t(train), precision, int(g.colind.size))
+
+
+SCALED = dict(n=2_400_000, pairs=62_000_000, features=100, hidden=256, feat_density=0.1)
 
 
 def train_step(task: GCNTask, sync: bool = True):

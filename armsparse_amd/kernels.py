@@ -80,13 +80,17 @@ def _check_csr(fn: str, rowptr: torch.Tensor, colind: torch.Tensor, values: Opti
 # ----------------------------------------------------------------- SpMM
 def spmm(rowptr: torch.Tensor, colind: torch.Tensor, values: Optional[torch.Tensor], dense: torch.Tensor, *,
          dense_scale: Optional[torch.Tensor] = None, out: Optional[torch.Tensor] = None, accumulate: bool = False,
-         trunc4: bool = False, ncols: Optional[int] = None, config: ConfigLike = None) -> torch.Tensor:
+         trunc4: bool = False, ncols: Optional[int] = None, config: ConfigLike = None,
+         col_scale: Optional[torch.Tensor] = None, row_scale: Optional[torch.Tensor] = None,
+         bias: Optional[torch.Tensor] = None, relu: bool = False) -> torch.Tensor:
     """out[M×N] (=, or += with accumulate) A·dense.
 
     dense: float32, bfloat16, or fp8-e4m3 (torch.float8_e4m3fn or raw uint8
     bytes) with ``dense_scale`` holding one float32 scale per row.  values=None
     means all-ones edge values.  ``ncols`` restricts the update to the first
     ncols columns (the reference's ``irregular``/``rowBaseSpmm`` tails).
+    Fused epilogue (armsp_spmm_ex): out[r] = row_scale[r]·Σ val·col_scale[c]·dense[c]
+    + bias, then ReLU.
     """
     fn = "spmm"
     _check_csr(fn, rowptr, colind, values)
@@ -111,7 +115,26 @@ def spmm(rowptr: torch.Tensor, colind: torch.Tensor, values: Optional[torch.Tens
     _require(0 <= nc <= n, f"{fn}: ncols out of range")
     cfg = _config(config, "spmm", rowptr, m, nnz, nc)
     s = _stream(dev)
-    if d.dtype == torch.float32:
+    if col_scale is not None or row_scale is not None or bias is not None or relu:
+        ep_t = []
+        for name, t, n_expect in (("col_scale", col_scale, k), ("row_scale", row_scale, m), ("bias", bias, nc)):
+            if t is not None:
+                _require(t.dtype == torch.float32 and t.numel() == n_expect, f"{fn}: {name} must be float32[{n_expect}]")
+                t = _on_device(t.contiguous(), dev)
+            ep_t.append(t)
+        ep = _lib.Epilogue(*(None if t is None else t.data_ptr() for t in ep_t), 1 if relu else 0)
+        if d.dtype == torch.float32:
+            dt, sc = _lib.DTYPE_F32, None
+        elif d.dtype == torch.bfloat16:
+            dt, sc = _lib.DTYPE_BF16, None
+        elif d.dtype in (torch.uint8, torch.float8_e4m3fn):
+            _require(dense_scale is not None, f"{fn}: fp8 dense needs dense_scale")
+            dt, sc = _lib.DTYPE_FP8_E4M3, _on_device(dense_scale.contiguous(), dev)
+        else:
+            raise RuntimeError(f"{fn}: unsupported dense dtype {d.dtype}")
+        call("armsp_spmm_ex", m, k, nc, nnz, _ptr(rp), _ptr(ci), _ptr(va), dt, _ptr(d), _ptr(sc), d.stride(0),
+             _ptr(c), c.stride(0), flags, ctypes.byref(ep), cfg, s)
+    elif d.dtype == torch.float32:
         call("armsp_spmm_f32", m, k, nc, nnz, _ptr(rp), _ptr(ci), _ptr(va), _ptr(d), d.stride(0), _ptr(c),
              c.stride(0), flags, cfg, s)
     elif d.dtype == torch.bfloat16:

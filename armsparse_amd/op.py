@@ -66,6 +66,30 @@ class SPMMFunction(torch.autograd.Function):
         return None, None, None, None, grad_feat, grad_ew, None
 
 
+class GCNAggregate(torch.autograd.Function):
+    """y = diag(in_norm)·A·diag(out_norm)·h + bias in one fused SpMM (armsp_spmm_ex);
+    backward dh = diag(out_norm)·Aᵀ·diag(in_norm)·dy in one SpMM on (colptr, rowind, w_csc).
+    h may be bfloat16 (fp32 accumulation; y and the gradients are fp32)."""
+
+    @staticmethod
+    def forward(ctx, h, rowptr, colind, colptr, rowind, w_csr, w_csc, in_norm, out_norm, bias):
+        y = kernels.spmm(rowptr, colind, w_csr, h, col_scale=out_norm, row_scale=in_norm, bias=bias)
+        ctx.graph = (colptr, rowind, w_csr, w_csc, in_norm, out_norm, h.dtype)
+        return y
+
+    @staticmethod
+    def backward(ctx, gy):
+        colptr, rowind, w_csr, w_csc, in_norm, out_norm, hdt = ctx.graph
+        if w_csr is not None and w_csc is None:
+            raise RuntimeError("GCNAggregate backward needs the edge values in CSC order as well")
+        g = gy.contiguous()
+        if hdt == torch.bfloat16:
+            g = kernels.to_bf16(g.float())
+        dh = kernels.spmm(colptr, rowind, w_csc, g, col_scale=in_norm, row_scale=out_norm)
+        dbias = gy.sum(0) if ctx.needs_input_grad[9] else None
+        return dh.to(hdt), None, None, None, None, None, None, None, None, dbias
+
+
 class _BF16Linear(torch.autograd.Function):
     """y = x·W with x bfloat16 [M×F] and W float32 master weights [F×H]; MFMA GEMM, fp32 y."""
 
@@ -93,10 +117,13 @@ class GCNConv(torch.nn.Module):
     scale by 1/sqrt(in-degree) (counts from rowptr), add bias."""
 
     def __init__(self, in_channels, out_channels, improved=False, cached=False, bias=True, normalize=True,
-                 precision: str = "fp32", **kwargs):
+                 precision: str = "fp32", fused: Optional[bool] = None, **kwargs):
         super().__init__()
         if precision not in ("fp32", "bf16"):
             raise ValueError("precision must be 'fp32' or 'bf16'")
+        # fused: normalisation, aggregation and bias in one SpMM launch (default for bf16;
+        # the fp32 default keeps the reference's separate tensor ops)
+        self.fused = (precision == "bf16") if fused is None else bool(fused)
         self.in_channels = in_channels
         self.out_channels = out_channels
         self.improved = improved
@@ -139,6 +166,11 @@ class GCNConv(torch.nn.Module):
                 out_deg_norm = torch.ones(colptr.shape[0] - 1, 1, dtype=x.dtype, device=x.device)
             self.cached_result = in_deg_norm, out_deg_norm
         in_deg_norm, out_deg_norm = self.cached_result
+        if self.fused:
+            h = x.to(torch.bfloat16) if self.precision == "bf16" else x
+            return GCNAggregate.apply(h.contiguous(), rowptr, colind, colptr, rowind, edge_weight_csr,
+                                      edge_weight_csc, in_deg_norm.reshape(-1).contiguous() if self.normalize else None,
+                                      out_deg_norm.reshape(-1).contiguous() if self.normalize else None, self.bias)
         if self.normalize:
             x = x * out_deg_norm
         if self.precision == "bf16":

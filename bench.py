@@ -38,6 +38,8 @@ def parse():
     p.add_argument("--chunk", type=int, default=0)
     p.add_argument("--unroll", type=int, default=0)
     p.add_argument("--cpu-seconds", type=float, default=12.0)
+    p.add_argument("--copy-probe", action="store_true",
+                   help="also run a plain 512 MiB device copy (calibrates rocprof FETCH_SIZE/WRITE_SIZE)")
     return p.parse_args()
 
 
@@ -93,7 +95,14 @@ def main():
     def spmm_bytes(sb, scales=False, accumulate=False):
         return 4 * (M + 1) + 8 * nnz + sb * K * N + (4 * K if scales else 0) + 4 * M * N * (2 if accumulate else 1)
 
-    result = {}
+    if args.copy_probe:  # a copy kernel of known size: 512 MiB read + 512 MiB written per launch
+        src = torch.empty(128 << 20, device=dev)
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
+
     # ------------------------------------------------------------ SpMM fp32
     fn = lambda: kernels.spmm(rp, ci, va, B, out=C, config=cfg)  # noqa: E731
     el, fam = timed(fn, args.steps, args.warmup, ["spmm", "spmm_fixup"])

@@ -107,6 +107,33 @@ int armsp_spmm_fp8(int M, int K, int N, int nnz,
                    const uint8_t* Bq, const float* b_row_scale, int ldb,
                    float* C, int ldc, int flags, const armsp_config* cfg, void* stream);
 
+/* Dense operand element types for armsp_spmm_ex. */
+#define ARMSP_DTYPE_F32      0
+#define ARMSP_DTYPE_BF16     1
+#define ARMSP_DTYPE_FP8_E4M3 2
+
+/* Fused operand scales and output epilogue of a SpMM:
+ *   C[r,:] (=|+=) row_scale[r] · Σ_e val_e · col_scale[c_e] · B[c_e,:] + bias[:],
+ * then ReLU when relu != 0 (applied to the final value of the row).  Any
+ * pointer may be NULL.  This is the GCN layer D^-½·A·D^-½·(XW) + b of
+ * GCNConv (gcn/pytorch-custom/op.py:143-149, there four separate tensor ops)
+ * in one launch; its backward D^-½·Aᵀ·D^-½·dY is the same call on Aᵀ. */
+typedef struct armsp_epilogue {
+    const float* col_scale;
+    const float* row_scale;
+    const float* bias;
+    int32_t relu;
+} armsp_epilogue;
+
+/* SpMM with B of type b_dtype (b_row_scale: fp8 per-row scales, else
+ * ignored) and an optional epilogue.  With epilogue == NULL it computes
+ * exactly what armsp_spmm_f32 / _bf16 / _fp8 compute. */
+int armsp_spmm_ex(int M, int K, int N, int nnz,
+                  const int* rowptr, const int* colind, const float* val,
+                  int b_dtype, const void* B, const float* b_row_scale, int ldb,
+                  float* C, int ldc, int flags, const armsp_epilogue* epilogue,
+                  const armsp_config* cfg, void* stream);
+
 /* ------------------------------------------------------------------- SpMV */
 /* y[M] (=|+=) A·x on plain CSR.  Replaces the row split of
  * numaAware/numaAwareSpMV.cpp:102-125 (one launch, no per-node copies). */

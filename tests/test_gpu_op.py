@@ -123,3 +123,28 @@ def test_gcn_training_step(precision):
     losses = [gcn.train_step(task) for _ in range(5)]
     print(f"{precision} losses: {losses}")
     assert all(np.isfinite(losses)) and losses[-1] < losses[0]
+
+
+def test_fused_gcn_layer_matches_unfused(graph512):
+    """The fused aggregation (one SpMM with scales and bias) equals the reference's
+    four-op GCNConv in value and in every gradient."""
+    g, *_ = graph512
+    gt = dev(g.rowptr, g.colind, g.colptr, g.rowind, g.value_csr, g.value_csc)
+    torch.manual_seed(1)
+    plain = GCNConv(40, 24).cuda()
+    fused = GCNConv(40, 24, fused=True).cuda()
+    fused.load_state_dict(plain.state_dict())
+    with torch.no_grad():
+        plain.bias.uniform_(-0.2, 0.2)
+        fused.bias.copy_(plain.bias)
+    x = torch.rand(512, 40, device="cuda")
+    G = torch.randn(512, 24, device="cuda")
+    outs, grads = [], []
+    for conv in (plain, fused):
+        y = conv(x, *gt[:4], *gt[4:])
+        y.backward(G)
+        outs.append(y.detach().cpu().numpy())
+        grads.append((conv.weight.grad.cpu().numpy(), conv.bias.grad.cpu().numpy()))
+    assert np.max(oracle.row_rel_err(outs[1], outs[0])) <= 1e-5
+    for a, b in zip(grads[0], grads[1]):
+        assert np.max(np.abs(a - b)) <= 1e-5 * (np.max(np.abs(a)) + 1e-12)

@@ -211,3 +211,29 @@ def test_full_size_rmat_1m_n128():
     rowwise = kernels.spmm(d_rp, d_ci, d_va, B, config="row").cpu().numpy()
     f64 = oracle.spmm_f64(a.rowptr, a.colind, a.values, B.cpu().numpy())
     check_rows(merged, rowwise, f64, "1M R-MAT merge")
+
+
+@pytest.mark.parametrize("sched", ["row", "merge", "nnz"])
+@pytest.mark.parametrize("accumulate", [False, True])
+def test_epilogue_fusion(rmat14, sched, accumulate):
+    """row_scale·(A·diag(col_scale)·B) + bias, ReLU — ReLU on rows cut across waves is applied
+    only after their carries are added (hub rows of the power-law matrix span many waves)."""
+    case(f"epilogue {sched} acc={accumulate}")
+    a, (d_rp, d_ci, d_va) = rmat14
+    M, K = a.shape
+    r = np.random.default_rng(12)
+    B = r.uniform(-1, 1, (K, 64)).astype(np.float32)
+    cs = r.uniform(0.5, 1.5, K).astype(np.float32)
+    rs = r.uniform(0.5, 1.5, M).astype(np.float32)
+    bias = r.uniform(-0.5, 0.5, 64).astype(np.float32)
+    C0 = r.uniform(-1, 1, (M, 64)).astype(np.float32)
+    d_B, d_cs, d_rs, d_bias, d_C = dev(B, cs, rs, bias, C0)
+    cfg = dict(schedule=sched, chunk=256) if sched != "row" else "row"
+    out = kernels.spmm(d_rp, d_ci, d_va, d_B, out=d_C if accumulate else None, accumulate=accumulate,
+                       col_scale=d_cs, row_scale=d_rs, bias=d_bias, relu=True, config=cfg).cpu().numpy()
+    prod = oracle.spmm_f64(a.rowptr, a.colind, a.values * cs[a.colind], B)
+    ref = rs[:, None].astype(np.float64) * prod + bias + (C0 if accumulate else 0)
+    ref = np.maximum(ref, 0)
+    scale = rs[:, None] * oracle.spmm_f64(a.rowptr, a.colind, np.abs(a.values * cs[a.colind]), np.abs(B)) + 2
+    assert np.all(np.abs(out - ref) <= TOL * scale)
+    assert (out >= 0).all()
