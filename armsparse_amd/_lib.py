@@ -115,6 +115,7 @@ _SIGS = {
     "armsp_gen_uniform_csr": (c_int, [_I, _I, c_int64, c_uint64, _P, _P, _P]),
     "armsp_gen_rmat_csr": (c_int, [_I, c_int64, c_double, c_double, c_double, c_uint64, _P, _P, _P]),
     "armsp_gen_powerlaw_graph": (c_int, [_I, c_int64, c_double, c_uint64, _P, _P]),
+    "armsp_gen_gcn_graph": (c_int, [_I, c_int64, c_double, c_uint64, _P, _P]),
     "armsp_gen_corpus_csr": (c_int64, [_I, _I, c_double, c_double, c_uint64, c_int64, _P, _P, _P]),
     "armsp_csr_to_vcsr": (c_int64, [_I, _I, _P, _P, _P, c_float, _P, _P, _P]),
     "armsp_csr_check": (c_int, [_I, _I, c_int64, _P, _P]),

@@ -147,9 +147,47 @@ std::vector<uint64_t> first_distinct(int64_t want, Draw draw, Bucket bucket_of, 
     }
 }
 
+// Sorts keys (row<<32 | col) ascending: a parallel counting pass into buckets
+// of rows (row >> rshift), then every bucket sorted on its own.
+void sort_keys(std::vector<uint64_t>& keys, int rshift, int NB) {
+    const int64_t n = (int64_t)keys.size();
+    const int nt = n < 65536 ? 1 : num_threads();
+    if (nt == 1) {
+        std::sort(keys.begin(), keys.end());
+        return;
+    }
+    auto bucket_of = [&](uint64_t k) { return (int)((k >> 32) >> rshift); };
+    std::vector<int64_t> cnt((size_t)nt * NB, 0);
+    run_threads(nt, [&](int tid, int T) {
+        int64_t* c = &cnt[(size_t)tid * NB];
+        for (int64_t i = n * tid / T; i < n * (tid + 1) / T; ++i) c[bucket_of(keys[i])]++;
+    });
+    std::vector<int64_t> boff(NB + 1, 0), off((size_t)nt * NB);
+    int64_t acc = 0;
+    for (int b = 0; b < NB; ++b) {
+        boff[b] = acc;
+        for (int t = 0; t < nt; ++t) {
+            off[(size_t)t * NB + b] = acc;
+            acc += cnt[(size_t)t * NB + b];
+        }
+    }
+    boff[NB] = acc;
+    std::vector<uint64_t> out(n);
+    run_threads(nt, [&](int tid, int T) {
+        int64_t* o = &off[(size_t)tid * NB];
+        for (int64_t i = n * tid / T; i < n * (tid + 1) / T; ++i) out[o[bucket_of(keys[i])]++] = keys[i];
+    });
+    parallel_dynamic(NB, 4, [&](int64_t b) { std::sort(out.begin() + boff[b], out.begin() + boff[b + 1]); });
+    keys.swap(out);
+}
+
 // Sorted unique keys (row<<32 | col) → canonical CSR.
 void keys_to_csr(std::vector<uint64_t>& keys, int M, uint64_t vkey, int* rowptr, int* colind, float* val) {
-    if (!std::is_sorted(keys.begin(), keys.end())) std::sort(keys.begin(), keys.end());
+    if (!std::is_sorted(keys.begin(), keys.end())) {
+        int rshift = 0;
+        while ((M >> rshift) > 4096) ++rshift;
+        sort_keys(keys, rshift, (M >> rshift) + 1);
+    }
     std::fill(rowptr, rowptr + M + 1, 0);
     for (uint64_t k : keys) rowptr[(k >> 32) + 1]++;
     for (int r = 0; r < M; ++r) rowptr[r + 1] += rowptr[r];
@@ -237,9 +275,8 @@ int armsp_gen_rmat_csr(int scale, int64_t nnz, double a, double b, double c, uin
     return 0;
 }
 
-int armsp_gen_powerlaw_graph(int n, int64_t pairs, double gamma, uint64_t seed, int* rowptr, int* colind) {
-    if (n < 2 || pairs < 0 || gamma <= 1.0 || !rowptr || (pairs > 0 && !colind)) return 1;
-    if ((uint64_t)pairs > (uint64_t)n * (uint64_t)(n - 1) / 4) return 1;
+// Distinct undirected pairs (lo<<32 | hi, lo < hi) of a Chung–Lu-style power-law graph.
+static std::vector<uint64_t> powerlaw_pairs(int n, int64_t pairs, double gamma, uint64_t seed) {
     std::vector<double> cdf(n);
     double acc = 0.0, expo = -1.0 / (gamma - 1.0);
     for (int i = 0; i < n; ++i) {
@@ -255,7 +292,7 @@ int armsp_gen_powerlaw_graph(int n, int64_t pairs, double gamma, uint64_t seed, 
     const uint64_t gkey = stream_key(seed, ARMSP_STREAM_GRAPH);
     int rshift = 0;
     while ((n >> rshift) > 4096) ++rshift;
-    std::vector<uint64_t> und = first_distinct(
+    return first_distinct(
         pairs,
         [&](uint64_t i) -> uint64_t {
             int u = pick(draw(gkey, 2 * i)), v = pick(draw(gkey, 2 * i + 1));
@@ -264,13 +301,34 @@ int armsp_gen_powerlaw_graph(int n, int64_t pairs, double gamma, uint64_t seed, 
             return (lo << 32) | hi;
         },
         [&](uint64_t k) { return (int)((k >> 32) >> rshift); }, (n >> rshift) + 1);
-    std::vector<uint64_t> keys;
-    keys.reserve(2 * und.size());
-    for (uint64_t k : und) {
-        keys.push_back(k);
-        keys.push_back((k << 32) | (k >> 32));
-    }
+}
+
+// Both directions of every pair, plus the diagonal when `self_loops`, as canonical CSR.
+static void symmetric_csr(const std::vector<uint64_t>& und, int n, bool self_loops, int* rowptr, int* colind) {
+    const int64_t P = (int64_t)und.size();
+    std::vector<uint64_t> keys((size_t)(2 * P + (self_loops ? n : 0)));
+    parallel_for(P, [&](int64_t i) {
+        const uint64_t k = und[i];
+        keys[2 * i] = k;
+        keys[2 * i + 1] = (k << 32) | (k >> 32);
+    });
+    if (self_loops)
+        parallel_for(n, [&](int64_t i) { keys[2 * P + i] = ((uint64_t)i << 32) | (uint64_t)i; });
     keys_to_csr(keys, n, 0, rowptr, colind, nullptr);
+}
+
+int armsp_gen_powerlaw_graph(int n, int64_t pairs, double gamma, uint64_t seed, int* rowptr, int* colind) {
+    if (n < 2 || pairs < 0 || gamma <= 1.0 || !rowptr || (pairs > 0 && !colind)) return 1;
+    if ((uint64_t)pairs > (uint64_t)n * (uint64_t)(n - 1) / 4) return 1;
+    symmetric_csr(powerlaw_pairs(n, pairs, gamma, seed), n, false, rowptr, colind);
+    return 0;
+}
+
+int armsp_gen_gcn_graph(int n, int64_t pairs, double gamma, uint64_t seed, int* rowptr, int* colind) {
+    if (n < 2 || pairs < 0 || gamma <= 1.0 || !rowptr || !colind) return 1;
+    if ((uint64_t)pairs > (uint64_t)n * (uint64_t)(n - 1) / 4) return 1;
+    if (2 * pairs + n > (int64_t)INT32_MAX) return 1;
+    symmetric_csr(powerlaw_pairs(n, pairs, gamma, seed), n, true, rowptr, colind);
     return 0;
 }
 

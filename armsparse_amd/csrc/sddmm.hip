@@ -140,6 +140,199 @@ __global__ __launch_bounds__(256) void k_sddmm(int M, int Kd, int nnz, int chunk
     }
 }
 
+// ------------------------------------------------ staged, 32-bit offsets
+// Same lane mapping and transpose-reduce as k_sddmm, with the index work
+// moved out of the dot-product loop: the wave first stages, in LDS, the byte
+// offsets of the U row and the V row of every nonzero of its chunk (CSR: the
+// chunk is a merge-path range of rows + nonzeros, so it holds at most chunk+1
+// row ends and each nonzero's row is a binary search in LDS).  The loop then
+// issues buffer loads at (row offset + lane offset) with no 64-bit address
+// arithmetic and no predication: lanes past the chunk read row 0 and their
+// results are never stored.
+template <typename T, int W>
+__device__ __forceinline__ void gather_vec(__amdgpu_buffer_rsrc_t rs, uint32_t off, float (&o)[W]) {
+    constexpr int BYTES = W * (int)sizeof(T);
+    uint32_t w[BYTES >= 4 ? BYTES / 4 : 1];
+    if constexpr (BYTES == 16) {
+        auto t = __builtin_amdgcn_raw_buffer_load_b128(rs, off, 0, 0);
+        w[0] = t[0]; w[1] = t[1]; w[2] = t[2]; w[3] = t[3];
+    } else if constexpr (BYTES == 8) {
+        auto t = __builtin_amdgcn_raw_buffer_load_b64(rs, off, 0, 0);
+        w[0] = t[0]; w[1] = t[1];
+    } else if constexpr (BYTES == 4) {
+        w[0] = __builtin_amdgcn_raw_buffer_load_b32(rs, off, 0, 0);
+    } else {
+        w[0] = __builtin_amdgcn_raw_buffer_load_b16(rs, off, 0, 0);
+    }
+    if constexpr (sizeof(T) == 4) {
+#pragma unroll
+        for (int i = 0; i < W; ++i) o[i] = __uint_as_float(w[i]);
+    } else {
+#pragma unroll
+        for (int i = 0; i < W; ++i) o[i] = __uint_as_float((i & 1) ? (w[i >> 1] & 0xffff0000u) : (w[i >> 1] << 16));
+    }
+}
+
+// merge-path search by the whole wave (64 candidates per round)
+__device__ __forceinline__ int merge_search_wave(int d, const int* __restrict__ rowptr, int M, int nnz, int lane) {
+    int lo = d - nnz > 0 ? d - nnz : 0;
+    int hi = d < M ? d : M;
+    while (lo < hi) {
+        const int span = hi - lo;
+        const int step = (span + 63) >> 6;
+        const int i = lo + lane * step;
+        const bool p = (lane * step < span) && rowptr[i + 1] <= d - i - 1;
+        const int t = __popcll(__ballot(p));
+        if (step == 1) return lo + t;
+        if (t == 0) {
+            hi = lo;
+        } else {
+            const int nlo = lo + (t - 1) * step + 1;
+            hi = lo + t * step < hi ? lo + t * step : hi;
+            lo = nlo;
+        }
+    }
+    return lo;
+}
+
+template <typename T, int G, int W, bool COO, bool FULL>
+__global__ __launch_bounds__(256) void k_sddmm_stream(int M, int Kd, int nnz, int chunk,
+                                                      const int* __restrict__ rowidx, const int* __restrict__ colind,
+                                                      const float* __restrict__ scale, const T* __restrict__ U,
+                                                      int ldu, const T* __restrict__ V, int ldv,
+                                                      float* __restrict__ out) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int lane = threadIdx.x & 63;
+    const int wid = threadIdx.x >> 6;
+    const int g = lane / G, l = lane & (G - 1);
+    const int wave = uniform((int)(blockIdx.x * 4u + wid));
+    const int64_t total = COO ? (int64_t)nnz : (int64_t)M + nnz;
+    const int64_t d0l = (int64_t)wave * chunk;
+    if (d0l >= total) return;
+    const int d0 = (int)d0l;
+    const int d1 = (int)min(d0l + chunk, total);
+    int i0 = 0, j0 = d0, j1 = d1;
+    if constexpr (!COO) {
+        i0 = uniform(merge_search_wave(d0, rowidx, M, nnz, lane));
+        const int i1 = uniform(merge_search_wave(d1, rowidx, M, nnz, lane));
+        j0 = d0 - i0;
+        j1 = d1 - i1;
+    }
+    const int nz = j1 - j0;
+    const size_t wstride = ((size_t)chunk * 8 + (size_t)(chunk + 1) * 4 + 15) & ~(size_t)15;
+    uint2* s_off = reinterpret_cast<uint2*>(smem + wid * wstride);
+    int* s_re = reinterpret_cast<int*>(smem + wid * wstride + (size_t)chunk * 8);
+    const uint32_t urow = (uint32_t)ldu * (uint32_t)sizeof(T), vrow = (uint32_t)ldv * (uint32_t)sizeof(T);
+    int nre = 0;
+    if constexpr (!COO) {
+        nre = d1 - d0 - nz + 1;  // row ends of rows i0..i1
+        for (int k = lane; k < nre; k += 64) s_re[k] = (i0 + k < M) ? rowidx[i0 + k + 1] : INT_MAX;
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    }
+    const int nzr = (nz + 63) & ~63;
+    for (int k = lane; k < nzr; k += 64) {
+        uint2 o = make_uint2(0u, 0u);
+        if (k < nz) {
+            const int e = j0 + k;
+            int r;
+            if constexpr (COO) {
+                r = __builtin_nontemporal_load(rowidx + e);
+            } else {  // rows i0 + (number of row ends ≤ e)
+                int lo = 0, hi = nre - 1;
+                while (lo < hi) {
+                    const int mid = (lo + hi) >> 1;
+                    if (s_re[mid] <= e)
+                        lo = mid + 1;
+                    else
+                        hi = mid;
+                }
+                r = i0 + lo;
+            }
+            o = make_uint2((uint32_t)r * urow, (uint32_t)__builtin_nontemporal_load(colind + e) * vrow);
+        }
+        s_off[k] = o;
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+
+    const __amdgpu_buffer_rsrc_t ru = __builtin_amdgcn_make_buffer_rsrc(const_cast<T*>(U), (short)0, (int)0xffffffff,
+                                                                         0x00020000);
+    const __amdgpu_buffer_rsrc_t rv = __builtin_amdgcn_make_buffer_rsrc(const_cast<T*>(V), (short)0, (int)0xffffffff,
+                                                                         0x00020000);
+    const int steps = (Kd + G * W - 1) / (G * W);  // FULL: Kd = steps·G·W; else Kd % W == 0
+    // dot products whose operands are fetched together: 16 floats of each operand per lane
+    constexpr int QB = (16 / W) < G ? (16 / W) : G;
+
+    for (int base = 0; base < nz; base += 64) {
+        float p[G];
+#pragma unroll
+        for (int q = 0; q < G; ++q) p[q] = 0.0f;
+        for (int t = 0; t < steps; ++t) {
+            const int k = (t * G + l) * W;
+            const bool in = FULL || k < Kd;  // lanes past Kd in the last step read column 0, use 0
+            const uint32_t lo = in ? (uint32_t)(k * (int)sizeof(T)) : 0u;
+#pragma unroll
+            for (int qb = 0; qb < G; qb += QB) {
+                float a[QB][W], b[QB][W];
+#pragma unroll
+                for (int q = 0; q < QB; ++q) {
+                    const uint2 o = s_off[base + g * G + qb + q];
+                    gather_vec<T, W>(ru, o.x + lo, a[q]);
+                    gather_vec<T, W>(rv, o.y + lo, b[q]);
+                    if (!FULL && !in) {
+#pragma unroll
+                        for (int i = 0; i < W; ++i) a[q][i] = 0.0f;
+                    }
+                }
+#pragma unroll
+                for (int q = 0; q < QB; ++q)
+#pragma unroll
+                    for (int i = 0; i < W; ++i) p[qb + q] = fmaf(a[q][i], b[q][i], p[qb + q]);
+                // keep the next batch's loads below this point: bounded registers, more waves
+                __builtin_amdgcn_sched_barrier(0);
+            }
+        }
+#pragma unroll
+        for (int m = G / 2; m >= 1; m >>= 1) {
+            const bool upper = (l & m) != 0;
+#pragma unroll
+            for (int i = 0; i < m; ++i) {
+                const float send = upper ? p[i] : p[i + m];
+                const float keep = upper ? p[i + m] : p[i];
+                p[i] = keep + __shfl_xor(send, m);
+            }
+        }
+        const int e = base + lane;
+        if (e < nz) {
+            const int ge = j0 + e;
+            out[ge] = scale ? scale[ge] * p[0] : p[0];
+        }
+    }
+}
+
+inline size_t sddmm_lds_bytes(int chunk) {
+    return 4 * (((size_t)chunk * 8 + (size_t)(chunk + 1) * 4 + 15) & ~(size_t)15);
+}
+
+template <typename T, int G, int W>
+void launch_sddmm_stream(bool coo, dim3 grid, hipStream_t s, int M, int Kd, int nnz, int chunk, const int* ri,
+                         const int* ci, const float* sc, const T* U, int ldu, const T* V, int ldv, float* out) {
+    const size_t lds = sddmm_lds_bytes(chunk);
+    const bool full = Kd % (G * W) == 0;
+#define ARMSP_SDDMM_ST(CO, FU)                                                                                 \
+    hipLaunchKernelGGL((k_sddmm_stream<T, G, W, CO, FU>), grid, dim3(256), lds, s, M, Kd, nnz, chunk, ri, ci, sc, U, \
+                       ldu, V, ldv, out)
+    if (coo) {
+        if (full) ARMSP_SDDMM_ST(true, true); else ARMSP_SDDMM_ST(true, false);
+    } else {
+        if (full) ARMSP_SDDMM_ST(false, true); else ARMSP_SDDMM_ST(false, false);
+    }
+#undef ARMSP_SDDMM_ST
+}
+
 template <typename T, int G, int W>
 void launch_sddmm(bool coo, dim3 grid, hipStream_t s, int M, int Kd, int nnz, int chunk, const int* ri, const int* ci,
                   const float* sc, const T* U, int ldu, const T* V, int ldv, float* out) {
@@ -161,30 +354,49 @@ int sddmm_impl(bool coo, int M, int K, int Kd, int nnz, const int* rowidx, const
     if (!rowidx || !colind || !out || Kd == 0 || !U || !V || ldu < Kd || ldv < Kd) return ARMSP_ERROR_INVALID_VALUE;
     hipStream_t s = as_stream(stream);
     armsp_config c = cfg ? *cfg : armsp_config{0, 0, 0, 0, 0};
-    if (c.schedule != ARMSP_SCHED_AUTO && c.schedule != ARMSP_SCHED_NNZ) return ARMSP_ERROR_NOT_SUPPORTED;
+    if (c.schedule != ARMSP_SCHED_AUTO && c.schedule != ARMSP_SCHED_NNZ && c.schedule != ARMSP_SCHED_MERGE)
+        return ARMSP_ERROR_NOT_SUPPORTED;
     // vector width: 16-byte loads when every row start is aligned
     constexpr int WMAX = sizeof(T) == 4 ? 4 : 8;
     int W = c.vec > 0 ? c.vec : WMAX;
     if (W != 1 && W != WMAX) W = 1;
     if (W > 1 && (ldu % W || ldv % W || !aligned(U, 16) || !aligned(V, 16))) W = 1;
-    int G = c.group;
-    if (G <= 0) {
-        G = 4;
-        while (G < 16 && G * W < Kd) G <<= 1;
-    }
-    if (G != 4 && G != 8 && G != 16 && G != 32) return ARMSP_ERROR_INVALID_VALUE;
     const int chunk = c.chunk > 0 ? c.chunk : 256;
     if (chunk % 64) return ARMSP_ERROR_INVALID_VALUE;
-    const int64_t nwaves = ((int64_t)nnz + chunk - 1) / chunk;
+    // staged kernel: 32-bit row offsets into U and V, at most 2048 items per wave in
+    // LDS, Kd a multiple of the vector width (schedule NNZ asks for the unstaged one)
+    const bool staged = c.schedule != ARMSP_SCHED_NNZ && chunk <= 2048 && Kd % W == 0 &&
+                        (uint64_t)M * (uint64_t)ldu * sizeof(T) <= 0xffffffffull &&
+                        (uint64_t)K * (uint64_t)ldv * sizeof(T) <= 0xffffffffull;
+    int G = c.group;
+    if (G <= 0) {
+        // lanes per dot product: the staged kernel keeps G partials per lane, so it
+        // stops at 8 (4 for 8-wide bf16 loads) and loops over Kd instead
+        const int gmax = staged ? (W == 8 ? 4 : 8) : 16;
+        G = 4;
+        while (G < gmax && G * W < Kd) G <<= 1;
+    }
+    if (G != 4 && G != 8 && G != 16 && G != 32) return ARMSP_ERROR_INVALID_VALUE;
+    const int64_t items = (staged && !coo) ? (int64_t)M + nnz : (int64_t)nnz;
+    const int64_t nwaves = (items + chunk - 1) / chunk;
     if (nwaves > INT_MAX / 4) return ARMSP_ERROR_NOT_SUPPORTED;
     const dim3 grid((unsigned)((nwaves + 3) / 4));
     TimedLaunch t("sddmm", s);
 #define ARMSP_SDDMM_G(WW)                                                                                    \
-    switch (G) {                                                                                             \
-        case 4: launch_sddmm<T, 4, WW>(coo, grid, s, M, Kd, nnz, chunk, rowidx, colind, scale, U, ldu, V, ldv, out); break; \
-        case 8: launch_sddmm<T, 8, WW>(coo, grid, s, M, Kd, nnz, chunk, rowidx, colind, scale, U, ldu, V, ldv, out); break; \
-        case 16: launch_sddmm<T, 16, WW>(coo, grid, s, M, Kd, nnz, chunk, rowidx, colind, scale, U, ldu, V, ldv, out); break; \
-        default: launch_sddmm<T, 32, WW>(coo, grid, s, M, Kd, nnz, chunk, rowidx, colind, scale, U, ldu, V, ldv, out); break; \
+    if (staged) {                                                                                            \
+        switch (G) {                                                                                         \
+            case 4: launch_sddmm_stream<T, 4, WW>(coo, grid, s, M, Kd, nnz, chunk, rowidx, colind, scale, U, ldu, V, ldv, out); break; \
+            case 8: launch_sddmm_stream<T, 8, WW>(coo, grid, s, M, Kd, nnz, chunk, rowidx, colind, scale, U, ldu, V, ldv, out); break; \
+            case 16: launch_sddmm_stream<T, 16, WW>(coo, grid, s, M, Kd, nnz, chunk, rowidx, colind, scale, U, ldu, V, ldv, out); break; \
+            default: launch_sddmm_stream<T, 32, WW>(coo, grid, s, M, Kd, nnz, chunk, rowidx, colind, scale, U, ldu, V, ldv, out); break; \
+        }                                                                                                    \
+    } else {                                                                                                 \
+        switch (G) {                                                                                         \
+            case 4: launch_sddmm<T, 4, WW>(coo, grid, s, M, Kd, nnz, chunk, rowidx, colind, scale, U, ldu, V, ldv, out); break; \
+            case 8: launch_sddmm<T, 8, WW>(coo, grid, s, M, Kd, nnz, chunk, rowidx, colind, scale, U, ldu, V, ldv, out); break; \
+            case 16: launch_sddmm<T, 16, WW>(coo, grid, s, M, Kd, nnz, chunk, rowidx, colind, scale, U, ldu, V, ldv, out); break; \
+            default: launch_sddmm<T, 32, WW>(coo, grid, s, M, Kd, nnz, chunk, rowidx, colind, scale, U, ldu, V, ldv, out); break; \
+        }                                                                                                    \
     }
     if (W == 1) {
         ARMSP_SDDMM_G(1)

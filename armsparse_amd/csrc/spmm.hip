@@ -108,7 +108,7 @@ __device__ __forceinline__ float coef(float a, const Epi& ep, int c) {
 // out = rscale·acc + bias (+ C_old) (ReLU when `final`)
 template <int V>
 __device__ __forceinline__ void finish(float (&o)[V], const float (&acc)[V], const Epi& ep, int r, int col,
-                                       const float* cold, bool final) {
+                                       const float (&cold)[V], bool has_cold, bool final) {
 #pragma unroll
     for (int v = 0; v < V; ++v) o[v] = acc[v];
     if (ep.rscale) {
@@ -120,7 +120,7 @@ __device__ __forceinline__ void finish(float (&o)[V], const float (&acc)[V], con
 #pragma unroll
         for (int v = 0; v < V; ++v) o[v] += ep.bias[col + v];
     }
-    if (cold) {
+    if (has_cold) {
 #pragma unroll
         for (int v = 0; v < V; ++v) o[v] += cold[v];
     }
@@ -195,7 +195,7 @@ __global__ __launch_bounds__(256) void k_spmm_row(int M, int N, const int* __res
         float o[V], cold[V];
         const bool accm = (flags & ARMSP_ACCUMULATE) != 0;
         if (accm) load_c<V>(cp, cold);
-        finish<V>(o, acc, ep, row, col, accm ? cold : nullptr, true);
+        finish<V>(o, acc, ep, row, col, cold, accm, true);
         store_c<V>(cp, o);
     }
 }
@@ -270,7 +270,7 @@ __global__ __launch_bounds__(256) void k_spmm_split(int M, int N, int nnz, int c
             float* cp = C + (size_t)r * ldc + col;
             float o[V], cold[V];
             if (acc_mode) load_c<V>(cp, cold);
-            finish<V>(o, acc, ep, r, col, acc_mode ? cold : nullptr, r != carry_in_row);
+            finish<V>(o, acc, ep, r, col, cold, acc_mode, r != carry_in_row);
             store_c<V>(cp, o);
         }
 #pragma unroll
@@ -325,6 +325,210 @@ __global__ __launch_bounds__(256) void k_spmm_split(int M, int N, int nnz, int c
 #pragma unroll
         for (int v = 0; v < V; ++v) cp[v] = acc[v];
     }
+}
+
+// ------------------------------------------ MERGE split, 32-bit offsets
+// The merge-path split of k_spmm_split with a leaner per-nonzero stream, used
+// whenever every dense-row byte offset fits in 32 bits:
+//  - the wave first stages its chunk in LDS: per nonzero the dense-row BYTE
+//    offset and the coefficient, per row its end; the nonzero loop then reads
+//    them with wave-uniform LDS loads (a broadcast, no bank conflicts);
+//  - a gather is one buffer load, address = row offset + lane column offset;
+//  - lanes past N load column 0 of the tile (always in bounds) and never
+//    store, so no load is predicated;
+//  - U gathers form a batch and one scalar compare per batch decides whether a
+//    row ends inside it; only such batches walk nonzero by nonzero;
+//  - two batches are in flight: batch k+1 is issued before batch k is used.
+template <typename TB, int V>
+__device__ __forceinline__ void gather_b(__amdgpu_buffer_rsrc_t rs, uint32_t voff, float (&o)[V]) {
+    constexpr int BYTES = V * (int)sizeof(TB);
+    constexpr int W = BYTES >= 4 ? BYTES / 4 : 1;
+    uint32_t w[W];
+    if constexpr (BYTES == 16) {
+        auto t = __builtin_amdgcn_raw_buffer_load_b128(rs, voff, 0, 0);
+        w[0] = t[0]; w[1] = t[1]; w[2] = t[2]; w[3] = t[3];
+    } else if constexpr (BYTES == 8) {
+        auto t = __builtin_amdgcn_raw_buffer_load_b64(rs, voff, 0, 0);
+        w[0] = t[0]; w[1] = t[1];
+    } else if constexpr (BYTES == 4) {
+        w[0] = __builtin_amdgcn_raw_buffer_load_b32(rs, voff, 0, 0);
+    } else if constexpr (BYTES == 2) {
+        w[0] = __builtin_amdgcn_raw_buffer_load_b16(rs, voff, 0, 0);
+    } else {
+        w[0] = __builtin_amdgcn_raw_buffer_load_b8(rs, voff, 0, 0);
+    }
+    if constexpr (sizeof(TB) == 4) {
+#pragma unroll
+        for (int i = 0; i < V; ++i) o[i] = __uint_as_float(w[i]);
+    } else if constexpr (sizeof(TB) == 2) {
+#pragma unroll
+        for (int i = 0; i < V; ++i) o[i] = __uint_as_float((i & 1) ? (w[i >> 1] & 0xffff0000u) : (w[i >> 1] << 16));
+    } else {
+#pragma unroll
+        for (int i = 0; i < V; ++i) o[i] = e4m3_to_f32((w[i >> 2] >> (8 * (i & 3))) & 0xffu);
+    }
+}
+
+// Merge-path search by a whole wave: each round the 64 lanes probe 64 evenly
+// spaced candidates and a ballot narrows the range 64-fold, so a search over
+// a million rows takes 4 rounds of loads instead of 20 dependent ones.
+// Returns the number of rows consumed at diagonal d (as merge_path_search).
+__device__ __forceinline__ int merge_path_search_wave(int d, const int* __restrict__ rowptr, int M, int nnz,
+                                                      int lane) {
+    int lo = d - nnz > 0 ? d - nnz : 0;  // answer in [lo, hi]
+    int hi = d < M ? d : M;
+    while (lo < hi) {
+        const int span = hi - lo;
+        const int step = (span + 63) >> 6;  // lanes probe lo + l·step, l·step < span
+        const int i = lo + lane * step;
+        // P(i): row i ends at or before the path reaches it; true on a prefix
+        const bool p = (lane * step < span) && rowptr[i + 1] <= d - i - 1;
+        const int t = __popcll(__ballot(p));
+        if (step == 1) return lo + t;
+        if (t == 0) {
+            hi = lo;  // P(lo) false
+        } else {
+            const int nlo = lo + (t - 1) * step + 1;
+            const int nhi = lo + t * step < hi ? lo + t * step : hi;
+            lo = nlo;
+            hi = nhi;
+        }
+    }
+    return lo;
+}
+
+template <typename TB, int V, int U>
+__global__ __launch_bounds__(256) void k_spmm_stream(int M, int N, int nnz, int chunk,
+                                                     const int* __restrict__ rowptr, const int* __restrict__ colind,
+                                                     const float* __restrict__ val, const TB* __restrict__ B,
+                                                     int ldb, Epi ep, float* __restrict__ C, int ldc, int flags,
+                                                     int* __restrict__ carry_row, float* __restrict__ carry_val) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int lane = threadIdx.x & 63;
+    const int wid = threadIdx.x >> 6;
+    const int wave = uniform((int)(blockIdx.x * 4u + wid));
+    const int64_t total = (int64_t)M + nnz;
+    const int64_t d0l = (int64_t)wave * chunk;
+    if (d0l >= total) return;
+    const int d0 = (int)d0l;
+    const int d1 = (int)min(d0l + chunk, total);
+    const int i0 = uniform(merge_path_search_wave(d0, rowptr, M, nnz, lane));
+    const int i1 = uniform(merge_path_search_wave(d1, rowptr, M, nnz, lane));
+    const int j0 = d0 - i0, j1 = d1 - i1;
+    const int nz = j1 - j0;      // ≤ chunk
+    const int nr = i1 - i0 + 1;  // rows i0..i1, ≤ chunk + 1
+
+    // stage the chunk: (row byte offset, coefficient) per nonzero, end per row
+    const uint32_t rowbytes = (uint32_t)ldb * (uint32_t)sizeof(TB);
+    const size_t wstride = ((size_t)chunk * 8 + (size_t)(chunk + 1) * 4 + 15) & ~(size_t)15;
+    uint2* s_nz = reinterpret_cast<uint2*>(smem + wid * wstride);
+    int* s_re = reinterpret_cast<int*>(smem + wid * wstride + (size_t)chunk * 8);
+    for (int k = lane; k < nz; k += 64) {
+        const int c = __builtin_nontemporal_load(colind + j0 + k);
+        const float a = coef(val ? __builtin_nontemporal_load(val + j0 + k) : 1.0f, ep, c);
+        s_nz[k] = make_uint2((uint32_t)c * rowbytes, __float_as_uint(a));
+    }
+    for (int k = lane; k < nr; k += 64) s_re[k] = (i0 + k < M) ? rowptr[i0 + k + 1] - j0 : INT_MAX;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+
+    const bool acc_mode = (flags & ARMSP_ACCUMULATE) != 0;
+    const int tile0 = (int)blockIdx.y * 64 * V;
+    const int col = tile0 + lane * V;
+    const bool active = col < N;
+    const __amdgpu_buffer_rsrc_t rs =
+        __builtin_amdgcn_make_buffer_rsrc(const_cast<TB*>(B + tile0), (short)0, (int)0xffffffff, 0x00020000);
+    const uint32_t voff = active ? (uint32_t)(lane * V * (int)sizeof(TB)) : 0u;
+
+    float acc[V];
+#pragma unroll
+    for (int v = 0; v < V; ++v) acc[v] = 0.0f;
+    // rows are emitted in order; re = end of row i0 + rr, relative to j0
+    int rr = 0;
+    int re = uniform(s_re[0]);
+    const int carry_in_row = (i0 < M && rowptr[i0] < j0) ? i0 : -1;
+    auto emit = [&]() {
+        const int r = i0 + rr;
+        if (active) {
+            float* cp = C + (size_t)r * ldc + col;
+            float o[V], cold[V];
+            if (acc_mode) load_c<V>(cp, cold);
+            finish<V>(o, acc, ep, r, col, cold, acc_mode, r != carry_in_row);
+            store_c<V>(cp, o);
+        }
+#pragma unroll
+        for (int v = 0; v < V; ++v) acc[v] = 0.0f;
+        ++rr;
+        re = uniform(s_re[rr]);
+    };
+    auto issue = [&](int k, float (&b)[U][V], float (&a)[U]) {
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const uint2 e = s_nz[k + u];
+            a[u] = __uint_as_float(e.y);
+            gather_b<TB, V>(rs, voff + e.x, b[u]);
+        }
+    };
+    auto consume = [&](int k, const float (&b)[U][V], const float (&a)[U]) {
+        if (re >= k + U) {
+#pragma unroll
+            for (int u = 0; u < U; ++u)
+#pragma unroll
+                for (int v = 0; v < V; ++v) acc[v] = fmaf(a[u], b[u][v], acc[v]);
+        } else {
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                while (re <= k + u) emit();
+#pragma unroll
+                for (int v = 0; v < V; ++v) acc[v] = fmaf(a[u], b[u][v], acc[v]);
+            }
+        }
+    };
+
+    const int kend = (nz / U) * U;  // end of the full batches
+    float bA[U][V], bB[U][V];
+    float aA[U], aB[U];
+    int k = 0;
+    if (k < kend) issue(k, bA, aA);
+    while (k < kend) {
+        if (k + U < kend) issue(k + U, bB, aB);
+        consume(k, bA, aA);
+        k += U;
+        if (k >= kend) break;
+        if (k + U < kend) issue(k + U, bA, aA);
+        consume(k, bB, aB);
+        k += U;
+    }
+#pragma unroll
+    for (int u = 0; u < U - 1; ++u) {  // fewer than U nonzeros left
+        if (kend + u < nz) {
+            const uint2 e = s_nz[kend + u];
+            float b[V];
+            gather_b<TB, V>(rs, voff + e.x, b);
+            while (re <= kend + u) emit();
+#pragma unroll
+            for (int v = 0; v < V; ++v) acc[v] = fmaf(__uint_as_float(e.y), b[v], acc[v]);
+        }
+    }
+    while (rr < nr - 1) emit();
+
+    // partial sum of row i1 (nonzeros [max(j0, rowptr[i1]), j1)) goes to the carry
+    bool has = false;
+    if (i1 < M) {
+        const int rs0 = rowptr[i1];
+        has = j1 > (rs0 > j0 ? rs0 : j0);
+    }
+    if (blockIdx.y == 0 && lane == 0) carry_row[wave] = has ? i1 : -1;
+    if (has && active) {
+        float* cp = carry_val + (size_t)wave * N + col;
+#pragma unroll
+        for (int v = 0; v < V; ++v) cp[v] = acc[v];
+    }
+}
+
+inline size_t stream_lds_bytes(int chunk) {
+    return 4 * (((size_t)chunk * 8 + (size_t)(chunk + 1) * 4 + 15) & ~(size_t)15);
 }
 
 // Adds each run of carries (consecutive waves cut inside the same row) into
@@ -441,6 +645,37 @@ int launch_split_vu(int U, dim3 grid, hipStream_t s, int M, int N, int nnz, int 
     return ARMSP_SUCCESS;
 }
 
+template <typename TB, int V>
+int launch_stream_vu(int U, dim3 grid, hipStream_t s, int M, int N, int nnz, int chunk, const int* rp, const int* ci,
+                     const float* va, const TB* B, int ldb, const Epi& bs, float* C, int ldc, int flags, int* crow,
+                     float* cval) {
+    const size_t lds = stream_lds_bytes(chunk);
+    switch (U) {
+#define ARMSP_STREAM_CASE(UU)                                                                                  \
+    case UU:                                                                                                   \
+        hipLaunchKernelGGL((k_spmm_stream<TB, V, UU>), grid, dim3(256), lds, s, M, N, nnz, chunk, rp, ci, va, B, ldb, \
+                           bs, C, ldc, flags, crow, cval);                                                     \
+        break;
+        ARMSP_STREAM_CASE(4)
+        ARMSP_STREAM_CASE(8)
+        ARMSP_STREAM_CASE(16)
+#undef ARMSP_STREAM_CASE
+        default: return ARMSP_ERROR_INVALID_VALUE;
+    }
+    return ARMSP_SUCCESS;
+}
+
+// MERGE gathers through 32-bit buffer offsets unless a dense-row offset could exceed them
+template <typename TB, int V, bool MERGE>
+int launch_split(bool wide, int U, dim3 grid, hipStream_t s, int M, int N, int nnz, int chunk, const int* rp,
+                 const int* ci, const float* va, const TB* B, int ldb, const Epi& bs, float* C, int ldc, int flags,
+                 int* crow, float* cval) {
+    if (wide || !MERGE || chunk > 2048)  // the LDS staging holds at most 2048 items per wave
+        return launch_split_vu<TB, V, MERGE>(U, grid, s, M, N, nnz, chunk, rp, ci, va, B, ldb, bs, C, ldc, flags, crow,
+                                             cval);
+    return launch_stream_vu<TB, V>(U, grid, s, M, N, nnz, chunk, rp, ci, va, B, ldb, bs, C, ldc, flags, crow, cval);
+}
+
 template <typename TB>
 int spmm_impl(int M, int K, int N, int nnz, const int* rowptr, const int* colind, const float* val, const TB* B,
               const float* b_row_scale, int ldb, float* C, int ldc, int flags, const armsp_epilogue* epi,
@@ -512,15 +747,17 @@ int spmm_impl(int M, int K, int N, int nnz, const int* rowptr, const int* colind
     int* crow = static_cast<int*>(ws);
     float* cval = reinterpret_cast<float*>(static_cast<char*>(ws) + row_bytes);
 
+    // dense-row byte offsets in 32 bits: (K-1)·ldb + N elements from B
+    const bool wide = c.group == 1 || ((uint64_t)K * (uint64_t)ldb * sizeof(TB)) > 0xffffffffull;
     dim3 grid((unsigned)((nwaves + 3) / 4), tiles);
     {
         TimedLaunch t("spmm", s);
 #define ARMSP_SPLIT_V(VV)                                                                                          \
     case VV:                                                                                                       \
-        st = merge ? launch_split_vu<TB, VV, true>(U, grid, s, M, N, nnz, chunk, rowptr, colind, val, B, ldb, bscale, \
-                                                   C, ldc, flags, crow, cval)                                      \
-                   : launch_split_vu<TB, VV, false>(U, grid, s, M, N, nnz, chunk, rowptr, colind, val, B, ldb,      \
-                                                    bscale, C, ldc, flags, crow, cval);                            \
+        st = merge ? launch_split<TB, VV, true>(wide, U, grid, s, M, N, nnz, chunk, rowptr, colind, val, B, ldb,    \
+                                                bscale, C, ldc, flags, crow, cval)                                 \
+                   : launch_split<TB, VV, false>(wide, U, grid, s, M, N, nnz, chunk, rowptr, colind, val, B, ldb,   \
+                                                 bscale, C, ldc, flags, crow, cval);                               \
         break;
         switch (V) {
             ARMSP_SPLIT_V(1)
@@ -528,10 +765,10 @@ int spmm_impl(int M, int K, int N, int nnz, const int* rowptr, const int* colind
             ARMSP_SPLIT_V(4)
             case 8:
                 if constexpr (sizeof(TB) < 4) {
-                    st = merge ? launch_split_vu<TB, 8, true>(U, grid, s, M, N, nnz, chunk, rowptr, colind, val, B, ldb,
-                                                              bscale, C, ldc, flags, crow, cval)
-                               : launch_split_vu<TB, 8, false>(U, grid, s, M, N, nnz, chunk, rowptr, colind, val, B,
-                                                               ldb, bscale, C, ldc, flags, crow, cval);
+                    st = merge ? launch_split<TB, 8, true>(wide, U, grid, s, M, N, nnz, chunk, rowptr, colind, val, B,
+                                                           ldb, bscale, C, ldc, flags, crow, cval)
+                               : launch_split<TB, 8, false>(wide, U, grid, s, M, N, nnz, chunk, rowptr, colind, val, B,
+                                                            ldb, bscale, C, ldc, flags, crow, cval);
                     break;
                 }
                 [[fallthrough]];

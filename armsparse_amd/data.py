@@ -148,16 +148,9 @@ class GCNGraph:
 
 def gcn_graph(n: int = 19_717, pairs: int = 44_324, seed: int = CFG_SEED + 3) -> GCNGraph:
     """Power-law symmetric graph plus self loops, as proc() (gcn_custom.py:38-58) builds from Planetoid."""
-    g = powerlaw_graph(n, pairs, seed)
-    # insert the diagonal into each canonical row (the generator has no self loops)
-    lens = np.diff(g.rowptr).astype(np.int64)
-    rows = np.repeat(np.arange(n, dtype=np.int64), lens)
-    greater = g.colind > rows
-    less_cnt = lens - np.bincount(rows[greater], minlength=n)
-    rp = (g.rowptr.astype(np.int64) + np.arange(n + 1)).astype(np.int32)
-    c = np.empty(g.nnz + n, np.int32)
-    c[np.arange(g.nnz, dtype=np.int64) + rows + greater] = g.colind
-    c[g.rowptr[:-1].astype(np.int64) + np.arange(n) + less_cnt] = np.arange(n, dtype=np.int32)
+    rp = np.empty(n + 1, np.int32)
+    c = np.empty(2 * pairs + n, np.int32)
+    _lib.check("armsp_gen_gcn_graph", lib.armsp_gen_gcn_graph(n, pairs, 3.0, seed, _p(rp), _p(c)))
     ones = np.ones(c.size, np.float32)
     # the adjacency is symmetric: its CSC equals its CSR
     return GCNGraph(n, rp, c, ones, rp.copy(), c.copy(), ones.copy())

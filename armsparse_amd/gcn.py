@@ -78,6 +78,16 @@ def make_task(precision: str = "fp32", seed: int = data.CFG_SEED + 3, n: int = 1
 SCALED = dict(n=2_400_000, pairs=62_000_000, features=100, hidden=256, feat_density=0.1)
 
 
+def retarget(task: GCNTask, precision: str) -> GCNTask:
+    """A fresh model and optimizer of `precision` on the same graph, features and labels."""
+    x = task.x.float() if precision == "fp32" else task.x.to(torch.bfloat16)
+    dev = x.device
+    model = Net(x.shape[1], task.model.conv1.out_channels, task.model.conv2.out_channels, precision).to(dev)
+    opt = torch.optim.Adam([dict(params=model.reg_params, weight_decay=5e-4),
+                            dict(params=model.non_reg_params, weight_decay=0)], lr=0.01)
+    return GCNTask(model, opt, x, task.graph, task.y, task.train_mask, precision, task.nnz)
+
+
 def train_step(task: GCNTask, sync: bool = True):
     """forward + nll loss + backward + Adam step (gcn_custom.py:122-126)."""
     task.model.train()

@@ -33,7 +33,7 @@ def parse():
     p.add_argument("--gpus", type=int, default=1)
     p.add_argument("--steps", type=int, default=50)
     p.add_argument("--warmup", type=int, default=5)
-    p.add_argument("--only", default="", help="comma list: spmm,sddmm,spmv,mixed,gcn,cpu (default all)")
+    p.add_argument("--only", default="", help="comma list: spmm,sddmm,spmv,mixed,gcn,gcn_scaled,cpu (default all)")
     p.add_argument("--schedule", default="", help="override the SpMM schedule (row|nnz|merge)")
     p.add_argument("--chunk", type=int, default=0)
     p.add_argument("--unroll", type=int, default=0)
@@ -54,7 +54,7 @@ def main():
 
     from armsparse_amd import _lib, data, kernels
 
-    only = set(filter(None, args.only.split(","))) or {"spmm", "sddmm", "spmv", "mixed", "gcn", "cpu"}
+    only = set(filter(None, args.only.split(","))) or {"spmm", "sddmm", "spmv", "mixed", "gcn", "gcn_scaled", "cpu"}
     dev = torch.device("cuda", 0)
     torch.cuda.set_device(dev)
     stream = torch.cuda.current_stream(dev)
@@ -161,7 +161,26 @@ def main():
             task = gcn.make_task(precision=prec)
             el_g, _ = timed(lambda: gcn.train_step(task, sync=False), 30, 5, [])
             extras[f"gcn_{prec}_ms_per_epoch"] = round(el_g / 30 * 1e3, 3)
+            del task
         extras["gcn_bf16_speedup"] = round(extras["gcn_fp32_ms_per_epoch"] / extras["gcn_bf16_ms_per_epoch"], 3)
+    if "gcn_scaled" in only:
+        from armsparse_amd import gcn
+
+        t0 = time.time()
+        base = gcn.make_task(precision="fp32", **gcn.SCALED)
+        log(f"[bench] scaled GCN graph ({base.nnz} nnz) built in {time.time() - t0:.1f}s")
+        for prec in ("fp32", "bf16"):
+            torch.manual_seed(0)
+            task = gcn.retarget(base, prec)
+            el_g, fam_g = timed(lambda: gcn.train_step(task, sync=False), 10, 3, ["spmm", "gemm"])
+            extras[f"gcn_scaled_{prec}_ms_per_epoch"] = round(el_g / 10 * 1e3, 2)
+            extras[f"gcn_scaled_{prec}_spmm_ms"] = round(fam_g["spmm"][0] / 10, 2)
+            log(f"[bench] scaled GCN {prec}: {el_g / 10 * 1e3:.1f} ms/epoch, SpMM {fam_g['spmm'][0] / 10:.1f} ms")
+            del task
+        del base
+        torch.cuda.empty_cache()
+        extras["gcn_scaled_bf16_speedup"] = round(
+            extras["gcn_scaled_fp32_ms_per_epoch"] / extras["gcn_scaled_bf16_ms_per_epoch"], 3)
 
     cpu = None
     if "cpu" in only:

@@ -59,6 +59,11 @@ int armsp_gen_rmat_csr(int scale, int64_t nnz, double a, double b, double c, uin
 int armsp_gen_powerlaw_graph(int n, int64_t pairs, double gamma, uint64_t seed,
                              int* rowptr, int* colind);
 
+/* The same graph with a self loop on every node, as the GCN data path builds
+ * A + I (gcn/pytorch-custom/gcn_custom.py:38-58): 2·pairs + n entries. */
+int armsp_gen_gcn_graph(int n, int64_t pairs, double gamma, uint64_t seed,
+                        int* rowptr, int* colind);
+
 /* Row-wise generators for the schedule-selector corpus (square M×M).  They
  * fill caller arrays of capacity `cap` and return the nonzero count, or -1 if
  * `cap` is too small.  Duplicates inside a row are merged, so the count is

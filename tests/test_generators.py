@@ -70,6 +70,13 @@ def test_gcn_graph_matches_proc_convention():
     assert g.rowptr[-1] == 2 * 1200 + n  # directed edges plus self loops
     rows = np.repeat(np.arange(n), np.diff(g.rowptr))
     assert np.all(np.isin(np.arange(n) * (n + 1), rows.astype(np.int64) * n + g.colind))
+    # A + I of the self-loop-free generator, built independently with scipy
+    import scipy.sparse as sp
+    a = data.powerlaw_graph(n, 1200, seed=2)
+    ref = (sp.csr_matrix((np.ones(a.nnz), a.colind, a.rowptr), shape=(n, n)) + sp.identity(n, format="csr")).tocsr()
+    ref.sort_indices()
+    assert np.array_equal(g.rowptr, ref.indptr) and np.array_equal(g.colind, ref.indices)
+    assert np.array_equal(g.colptr, g.rowptr) and np.array_equal(g.rowind, g.colind)
 
 
 @pytest.mark.parametrize("kind,avg,param", [("uniform", 5.5, 0), ("rmat", 8, 0.6), ("banded", 12, 16),

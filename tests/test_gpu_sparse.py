@@ -77,16 +77,22 @@ def test_spmv_full_rmat_1m():
 
 
 # ----------------------------------------------------------------- SDDMM
+# schedule "merge" (and AUTO) runs the LDS-staged kernel over merge-path chunks,
+# "nnz" the unstaged nnz-split kernel
 @pytest.mark.parametrize("kd", [1, 3, 8, 32, 100, 128])
 @pytest.mark.parametrize("group", [0, 4, 8, 16, 32])
-def test_sddmm_csr(rmat14, kd, group):
-    case(f"sddmm kd={kd} G={group}")
+@pytest.mark.parametrize("sched", ["merge", "nnz"])
+def test_sddmm_csr(rmat14, kd, group, sched):
+    case(f"sddmm kd={kd} G={group} {sched}")
     a, (d_rp, d_ci, d_va) = rmat14
     r = np.random.default_rng(kd)
     U = r.standard_normal((a.shape[0], kd)).astype(np.float32)
     V = r.standard_normal((a.shape[1], kd)).astype(np.float32)
     d_U, d_V = dev(U, V)
-    cfg = None if group == 0 else dict(schedule="nnz", group=group)
+    if group == 0:
+        cfg = None if sched == "merge" else dict(schedule="nnz")
+    else:
+        cfg = dict(schedule=sched, group=group, chunk=512 if sched == "merge" else 0)
     ref = oracle.sddmm_f64(a.rowptr, a.colind, U, V, scale=a.values)
     bound = oracle.sddmm_f64(a.rowptr, a.colind, np.abs(U), np.abs(V), scale=np.abs(a.values))
     out = kernels.sddmm(d_rp, d_ci, d_U, d_V, scale=d_va, config=cfg).cpu().numpy()
@@ -109,8 +115,9 @@ def test_sddmm_coo_and_bf16(rmat14):
     assert np.all(np.abs(out - ref) <= TOL * bound + 1e-30)
     Ub, Vb = d_U.to(torch.bfloat16), d_V.to(torch.bfloat16)
     refb = oracle.sddmm_f64(a.rowptr, a.colind, Ub.float().cpu().numpy(), Vb.float().cpu().numpy())
-    out = kernels.sddmm(d_rp, d_ci, Ub, Vb).cpu().numpy()
-    assert np.all(np.abs(out - refb) <= TOL * bound * 1.01 + 1e-30)
+    for cfg in (None, dict(schedule="nnz"), dict(schedule="merge", group=8, vec=1, chunk=2048)):
+        out = kernels.sddmm(d_rp, d_ci, Ub, Vb, config=cfg).cpu().numpy()
+        assert np.all(np.abs(out - refb) <= TOL * bound * 1.01 + 1e-30)
 
 
 def test_sddmm_full_rmat_1m_kd32():

@@ -97,8 +97,12 @@ def test_reference_tail_semantics():
         assert np.array_equal(bits(d_C), bits(oracle.irregular(rp, ci, va, B, C0.copy())))
 
 
-SPLIT_CFGS = [dict(schedule=s, chunk=c, unroll=u) for s in ("merge", "nnz") for c in (64, 512, 2048)
-              for u in (4, 8, 16)]
+# MERGE with chunk ≤ 2048 runs the LDS-staged 32-bit-offset kernel; group=1 or a
+# larger chunk runs the 64-bit-address kernel that NNZ always uses
+SPLIT_CFGS = ([dict(schedule=s, chunk=c, unroll=u) for s in ("merge", "nnz") for c in (64, 512, 2048)
+               for u in (4, 8, 16)]
+              + [dict(schedule="merge", chunk=c, unroll=u, group=1) for c in (64, 512) for u in (4, 8, 16)]
+              + [dict(schedule="merge", chunk=4096, unroll=8)])
 
 
 @pytest.fixture(scope="module")
@@ -107,7 +111,8 @@ def rmat14():
     return a, dev(a.rowptr, a.colind, a.values)
 
 
-@pytest.mark.parametrize("cfg", SPLIT_CFGS, ids=lambda c: f"{c['schedule']}-c{c['chunk']}-u{c['unroll']}")
+@pytest.mark.parametrize("cfg", SPLIT_CFGS,
+                         ids=lambda c: f"{c['schedule']}-c{c['chunk']}-u{c['unroll']}" + ("-wide" if c.get("group") else ""))
 def test_split_schedules_power_law(rmat14, cfg):
     case(f"split {cfg}")
     a, (d_rp, d_ci, d_va) = rmat14
@@ -127,12 +132,13 @@ def test_split_schedules_power_law(rmat14, cfg):
 
 
 @pytest.mark.parametrize("N", [1, 16, 32, 64, 96, 256, 512, 520])
-@pytest.mark.parametrize("sched", ["merge", "nnz"])
+@pytest.mark.parametrize("sched", ["merge", "nnz", "merge-wide"])
 def test_split_widths(rmat14, N, sched):
     case(f"split width N={N} {sched}")
     a, (d_rp, d_ci, d_va) = rmat14
     B = np.random.default_rng(N).uniform(-1, 1, (a.shape[1], N)).astype(np.float32)
-    out = kernels.spmm(d_rp, d_ci, d_va, dev(B)[0], config=sched).cpu().numpy()
+    cfg = dict(schedule="merge", group=1) if sched == "merge-wide" else sched
+    out = kernels.spmm(d_rp, d_ci, d_va, dev(B)[0], config=cfg).cpu().numpy()
     ref = oracle.spmm_f64(a.rowptr, a.colind, a.values, B)
     if N >= 16:
         assert np.max(oracle.row_rel_err(out, ref)) <= TOL
