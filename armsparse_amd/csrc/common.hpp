@@ -96,6 +96,33 @@ __device__ __forceinline__ int merge_path_search(int d, const int* __restrict__ 
     return lo;
 }
 
+// The same search done by a whole wave: each round the 64 lanes probe 64
+// evenly spaced candidates and a ballot narrows the range 64-fold, so a search
+// over a million rows takes 4 rounds of loads instead of 20 dependent ones.
+// Every lane of the wave must call it with the same d; all get the result.
+__device__ __forceinline__ int merge_path_search_wave(int d, const int* __restrict__ rowptr, int M, int nnz) {
+    const int lane = __lane_id();
+    int lo = d - nnz > 0 ? d - nnz : 0;  // answer in [lo, hi]
+    int hi = d < M ? d : M;
+    while (lo < hi) {
+        const int span = hi - lo;
+        const int step = (span + 63) >> 6;  // lanes probe lo + l·step, l·step < span
+        const int i = lo + lane * step;
+        // P(i): row i ends at or before the path reaches it; true on a prefix of i
+        const bool p = (lane * step < span) && rowptr[i + 1] <= d - i - 1;
+        const int t = __popcll(__ballot(p));
+        if (step == 1) return lo + t;
+        if (t == 0) {
+            hi = lo;  // P(lo) false
+        } else {
+            const int nlo = lo + (t - 1) * step + 1;
+            hi = lo + t * step < hi ? lo + t * step : hi;
+            lo = nlo;
+        }
+    }
+    return lo;
+}
+
 // Number of rows whose end is ≤ j (rows fully before nonzero j).
 __device__ __forceinline__ int rows_ending_by(int j, const int* __restrict__ rowptr, int M) {
     int lo = 0, hi = M;
@@ -105,6 +132,28 @@ __device__ __forceinline__ int rows_ending_by(int j, const int* __restrict__ row
             lo = mid + 1;
         else
             hi = mid;
+    }
+    return lo;
+}
+
+// rows_ending_by done by a whole wave, 64 probes per round.
+__device__ __forceinline__ int rows_ending_by_wave(int j, const int* __restrict__ rowptr, int M) {
+    const int lane = __lane_id();
+    int lo = 0, hi = M;  // answer in [lo, hi]
+    while (lo < hi) {
+        const int span = hi - lo;
+        const int step = (span + 63) >> 6;
+        const int i = lo + lane * step;
+        const bool p = (lane * step < span) && rowptr[i + 1] <= j;
+        const int t = __popcll(__ballot(p));
+        if (step == 1) return lo + t;
+        if (t == 0) {
+            hi = lo;
+        } else {
+            const int nlo = lo + (t - 1) * step + 1;
+            hi = lo + t * step < hi ? lo + t * step : hi;
+            lo = nlo;
+        }
     }
     return lo;
 }

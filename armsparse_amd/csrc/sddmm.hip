@@ -61,8 +61,8 @@ __global__ __launch_bounds__(256) void k_sddmm(int M, int Kd, int nnz, int chunk
     const int j1 = (int)(j0l + chunk < nnz ? j0l + chunk : nnz);
     int rlo = 0, rhi = 0;
     if constexpr (!COO) {
-        rlo = uniform(rows_ending_by(j0, rowidx, M));
-        rhi = uniform(rows_ending_by(j1 - 1, rowidx, M));
+        rlo = uniform(rows_ending_by_wave(j0, rowidx, M));
+        rhi = uniform(rows_ending_by_wave(j1 - 1, rowidx, M));
     }
     const int steps = (Kd + G * W - 1) / (G * W);
     const bool full = (Kd % (G * W)) == 0;
@@ -173,28 +173,6 @@ __device__ __forceinline__ void gather_vec(__amdgpu_buffer_rsrc_t rs, uint32_t o
     }
 }
 
-// merge-path search by the whole wave (64 candidates per round)
-__device__ __forceinline__ int merge_search_wave(int d, const int* __restrict__ rowptr, int M, int nnz, int lane) {
-    int lo = d - nnz > 0 ? d - nnz : 0;
-    int hi = d < M ? d : M;
-    while (lo < hi) {
-        const int span = hi - lo;
-        const int step = (span + 63) >> 6;
-        const int i = lo + lane * step;
-        const bool p = (lane * step < span) && rowptr[i + 1] <= d - i - 1;
-        const int t = __popcll(__ballot(p));
-        if (step == 1) return lo + t;
-        if (t == 0) {
-            hi = lo;
-        } else {
-            const int nlo = lo + (t - 1) * step + 1;
-            hi = lo + t * step < hi ? lo + t * step : hi;
-            lo = nlo;
-        }
-    }
-    return lo;
-}
-
 template <typename T, int G, int W, bool COO, bool FULL>
 __global__ __launch_bounds__(256) void k_sddmm_stream(int M, int Kd, int nnz, int chunk,
                                                       const int* __restrict__ rowidx, const int* __restrict__ colind,
@@ -213,8 +191,8 @@ __global__ __launch_bounds__(256) void k_sddmm_stream(int M, int Kd, int nnz, in
     const int d1 = (int)min(d0l + chunk, total);
     int i0 = 0, j0 = d0, j1 = d1;
     if constexpr (!COO) {
-        i0 = uniform(merge_search_wave(d0, rowidx, M, nnz, lane));
-        const int i1 = uniform(merge_search_wave(d1, rowidx, M, nnz, lane));
+        i0 = uniform(merge_path_search_wave(d0, rowidx, M, nnz));
+        const int i1 = uniform(merge_path_search_wave(d1, rowidx, M, nnz));
         j0 = d0 - i0;
         j1 = d1 - i1;
     }

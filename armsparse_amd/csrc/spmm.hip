@@ -220,15 +220,15 @@ __global__ __launch_bounds__(256) void k_spmm_split(int M, int N, int nnz, int c
     const int d1 = (int)min(d0l + chunk, total);
     int i0, j0, i1, j1;
     if constexpr (MERGE) {
-        i0 = uniform(merge_path_search(d0, rowptr, M, nnz));
-        i1 = uniform(merge_path_search(d1, rowptr, M, nnz));
+        i0 = uniform(merge_path_search_wave(d0, rowptr, M, nnz));
+        i1 = uniform(merge_path_search_wave(d1, rowptr, M, nnz));
         j0 = d0 - i0;
         j1 = d1 - i1;
     } else {
         j0 = d0;
         j1 = d1;
-        i0 = d0 == 0 ? 0 : uniform(rows_ending_by(j0, rowptr, M));
-        i1 = uniform(rows_ending_by(j1, rowptr, M));
+        i0 = d0 == 0 ? 0 : uniform(rows_ending_by_wave(j0, rowptr, M));
+        i1 = uniform(rows_ending_by_wave(j1, rowptr, M));
     }
     const bool acc_mode = (flags & ARMSP_ACCUMULATE) != 0;
     const int col = (int)(blockIdx.y * 64u + lane) * V;
@@ -369,34 +369,6 @@ __device__ __forceinline__ void gather_b(__amdgpu_buffer_rsrc_t rs, uint32_t vof
     }
 }
 
-// Merge-path search by a whole wave: each round the 64 lanes probe 64 evenly
-// spaced candidates and a ballot narrows the range 64-fold, so a search over
-// a million rows takes 4 rounds of loads instead of 20 dependent ones.
-// Returns the number of rows consumed at diagonal d (as merge_path_search).
-__device__ __forceinline__ int merge_path_search_wave(int d, const int* __restrict__ rowptr, int M, int nnz,
-                                                      int lane) {
-    int lo = d - nnz > 0 ? d - nnz : 0;  // answer in [lo, hi]
-    int hi = d < M ? d : M;
-    while (lo < hi) {
-        const int span = hi - lo;
-        const int step = (span + 63) >> 6;  // lanes probe lo + l·step, l·step < span
-        const int i = lo + lane * step;
-        // P(i): row i ends at or before the path reaches it; true on a prefix
-        const bool p = (lane * step < span) && rowptr[i + 1] <= d - i - 1;
-        const int t = __popcll(__ballot(p));
-        if (step == 1) return lo + t;
-        if (t == 0) {
-            hi = lo;  // P(lo) false
-        } else {
-            const int nlo = lo + (t - 1) * step + 1;
-            const int nhi = lo + t * step < hi ? lo + t * step : hi;
-            lo = nlo;
-            hi = nhi;
-        }
-    }
-    return lo;
-}
-
 template <typename TB, int V, int U>
 __global__ __launch_bounds__(256) void k_spmm_stream(int M, int N, int nnz, int chunk,
                                                      const int* __restrict__ rowptr, const int* __restrict__ colind,
@@ -412,8 +384,8 @@ __global__ __launch_bounds__(256) void k_spmm_stream(int M, int N, int nnz, int 
     if (d0l >= total) return;
     const int d0 = (int)d0l;
     const int d1 = (int)min(d0l + chunk, total);
-    const int i0 = uniform(merge_path_search_wave(d0, rowptr, M, nnz, lane));
-    const int i1 = uniform(merge_path_search_wave(d1, rowptr, M, nnz, lane));
+    const int i0 = uniform(merge_path_search_wave(d0, rowptr, M, nnz));
+    const int i1 = uniform(merge_path_search_wave(d1, rowptr, M, nnz));
     const int j0 = d0 - i0, j1 = d1 - i1;
     const int nz = j1 - j0;      // ≤ chunk
     const int nr = i1 - i0 + 1;  // rows i0..i1, ≤ chunk + 1
@@ -460,7 +432,7 @@ __global__ __launch_bounds__(256) void k_spmm_stream(int M, int N, int nnz, int 
 #pragma unroll
         for (int v = 0; v < V; ++v) acc[v] = 0.0f;
         ++rr;
-        re = uniform(s_re[rr]);
+        re = rr < nr ? uniform(s_re[rr]) : INT_MAX;  // never past the staged rows
     };
     auto issue = [&](int k, float (&b)[U][V], float (&a)[U]) {
 #pragma unroll
@@ -524,6 +496,180 @@ __global__ __launch_bounds__(256) void k_spmm_stream(int M, int N, int nnz, int 
         float* cp = carry_val + (size_t)wave * N + col;
 #pragma unroll
         for (int v = 0; v < V; ++v) cp[v] = acc[v];
+    }
+}
+
+// ------------------------------------- MERGE split over 128-byte column slices
+// The stream kernel with the dense columns cut into slices of 128 bytes (one
+// cache line of a B row: 32 fp32, 64 bf16 or 128 fp8 columns).  Half a wave
+// covers a slice (4 bytes per lane), so each load instruction gathers two
+// nonzeros, and the two halves' partial sums of a row meet at the row end
+// through one cross-half shuffle.  Blocks are numbered so that, with the
+// round-robin assignment of workgroups to the 8 XCDs, every XCD serves one
+// slice (when the slice count divides 8): each XCD's L2 then holds one line
+// per B row instead of all of them, i.e. several times more distinct rows of
+// the hot set.  The numbering only affects speed, never results.
+template <typename TB, int U>
+__global__ __launch_bounds__(256) void k_spmm_slice(int M, int N, int nnz, int chunk, int S, int G,
+                                                    const int* __restrict__ rowptr, const int* __restrict__ colind,
+                                                    const float* __restrict__ val, const TB* __restrict__ B, int ldb,
+                                                    Epi ep, float* __restrict__ C, int ldc, int flags,
+                                                    int* __restrict__ carry_row, float* __restrict__ carry_val) {
+    constexpr int V = 4 / (int)sizeof(TB);  // columns per lane
+    constexpr int SW = 32 * V;              // columns per slice
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int lane = threadIdx.x & 63;
+    const int wid = threadIdx.x >> 6;
+    const int h = lane >> 5, l = lane & 31;
+    int g, sl;
+    if (8 % S == 0) {  // XCD x (= block % 8) serves slice x % S
+        const int x = (int)(blockIdx.x & 7u);
+        sl = x % S;
+        g = (int)(blockIdx.x >> 3) * (8 / S) + x / S;
+    } else {
+        sl = (int)(blockIdx.x % (unsigned)S);
+        g = (int)(blockIdx.x / (unsigned)S);
+    }
+    if (g >= G) return;
+    const int wave = g * 4 + wid;
+    const int64_t total = (int64_t)M + nnz;
+    const int64_t d0l = (int64_t)wave * chunk;
+    if (d0l >= total) return;
+    const int d0 = (int)d0l;
+    const int d1 = (int)min(d0l + chunk, total);
+    const int i0 = uniform(merge_path_search_wave(d0, rowptr, M, nnz));
+    const int i1 = uniform(merge_path_search_wave(d1, rowptr, M, nnz));
+    const int j0 = d0 - i0, j1 = d1 - i1;
+    const int nz = j1 - j0;
+    const int nr = i1 - i0 + 1;
+
+    const uint32_t rowbytes = (uint32_t)ldb * (uint32_t)sizeof(TB);
+    const size_t wstride = ((size_t)chunk * 8 + (size_t)(chunk + 1) * 4 + 15) & ~(size_t)15;
+    uint2* s_nz = reinterpret_cast<uint2*>(smem + wid * wstride);
+    int* s_re = reinterpret_cast<int*>(smem + wid * wstride + (size_t)chunk * 8);
+    for (int k = lane; k < nz; k += 64) {
+        const int c = __builtin_nontemporal_load(colind + j0 + k);
+        const float a = coef(val ? __builtin_nontemporal_load(val + j0 + k) : 1.0f, ep, c);
+        s_nz[k] = make_uint2((uint32_t)c * rowbytes, __float_as_uint(a));
+    }
+    for (int k = lane; k < nr; k += 64) s_re[k] = (i0 + k < M) ? rowptr[i0 + k + 1] - j0 : INT_MAX;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+
+    const bool acc_mode = (flags & ARMSP_ACCUMULATE) != 0;
+    const int col = sl * SW + l * V;
+    const bool active = col < N;
+    const __amdgpu_buffer_rsrc_t rs =
+        __builtin_amdgcn_make_buffer_rsrc(const_cast<TB*>(B + sl * SW), (short)0, (int)0xffffffff, 0x00020000);
+    const uint32_t voff = active ? (uint32_t)(l * 4) : 0u;
+
+    float acc[V];
+#pragma unroll
+    for (int v = 0; v < V; ++v) acc[v] = 0.0f;
+    int rr = 0;
+    int re = uniform(s_re[0]);
+    const int carry_in_row = (i0 < M && rowptr[i0] < j0) ? i0 : -1;
+    auto emit = [&]() {  // called by the whole wave
+        const int r = i0 + rr;
+        float sum[V];
+#pragma unroll
+        for (int v = 0; v < V; ++v) sum[v] = acc[v] + __shfl_xor(acc[v], 32);
+        if (h == 0 && active) {
+            float* cp = C + (size_t)r * ldc + col;
+            float o[V], cold[V];
+            if (acc_mode) load_c<V>(cp, cold);
+            finish<V>(o, sum, ep, r, col, cold, acc_mode, r != carry_in_row);
+            store_c<V>(cp, o);
+        }
+#pragma unroll
+        for (int v = 0; v < V; ++v) acc[v] = 0.0f;
+        ++rr;
+        re = rr < nr ? uniform(s_re[rr]) : INT_MAX;  // never past the staged rows
+    };
+    auto issue = [&](int kb, float (&b)[U][V], float (&a)[U]) {
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const uint2 e = s_nz[kb + 2 * u + h];
+            a[u] = __uint_as_float(e.y);
+            gather_b<TB, V>(rs, voff + e.x, b[u]);
+        }
+    };
+    // nonzeros k2 (lower half) and k2+1 (upper half, when k2+1 < nz).  Rows are
+    // emitted only up to the pair's last existing nonzero `last`: a row ending at
+    // nz is either emitted after the loop or is the carry row i1.
+    auto step = [&](int k2, int last, const float (&b)[V], float a, bool has) {
+        const int kk = k2 + h;
+        bool done = !has;
+        while (re <= last) {  // a row ends at or before the pair's last nonzero
+            if (!done && kk < re) {
+#pragma unroll
+                for (int v = 0; v < V; ++v) acc[v] = fmaf(a, b[v], acc[v]);
+                done = true;
+            }
+            emit();
+        }
+        if (!done) {
+#pragma unroll
+            for (int v = 0; v < V; ++v) acc[v] = fmaf(a, b[v], acc[v]);
+        }
+    };
+    auto consume = [&](int kb, const float (&b)[U][V], const float (&a)[U]) {
+        if (re >= kb + 2 * U) {
+#pragma unroll
+            for (int u = 0; u < U; ++u)
+#pragma unroll
+                for (int v = 0; v < V; ++v) acc[v] = fmaf(a[u], b[u][v], acc[v]);
+        } else {
+#pragma unroll
+            for (int u = 0; u < U; ++u) step(kb + 2 * u, kb + 2 * u + 1, b[u], a[u], true);
+        }
+    };
+
+    const int kend = (nz / (2 * U)) * (2 * U);
+    float bA[U][V], bB[U][V];
+    float aA[U], aB[U];
+    int k = 0;
+    if (k < kend) issue(k, bA, aA);
+    while (k < kend) {
+        if (k + 2 * U < kend) issue(k + 2 * U, bB, aB);
+        consume(k, bA, aA);
+        k += 2 * U;
+        if (k >= kend) break;
+        if (k + 2 * U < kend) issue(k + 2 * U, bA, aA);
+        consume(k, bB, aB);
+        k += 2 * U;
+    }
+    for (int k2 = kend; k2 < nz; k2 += 2) {  // fewer than 2U nonzeros left
+        const bool has = k2 + h < nz;
+        float b[V];
+        float a = 0.0f;
+#pragma unroll
+        for (int v = 0; v < V; ++v) b[v] = 0.0f;
+        if (has) {
+            const uint2 e = s_nz[k2 + h];
+            a = __uint_as_float(e.y);
+            gather_b<TB, V>(rs, voff + e.x, b);
+        }
+        step(k2, k2 + 1 < nz ? k2 + 1 : k2, b, a, has);
+    }
+    while (rr < nr - 1) emit();
+
+    bool has = false;
+    if (i1 < M) {
+        const int rs0 = rowptr[i1];
+        has = j1 > (rs0 > j0 ? rs0 : j0);
+    }
+    if (sl == 0 && lane == 0) carry_row[wave] = has ? i1 : -1;
+    if (has) {
+        float sum[V];
+#pragma unroll
+        for (int v = 0; v < V; ++v) sum[v] = acc[v] + __shfl_xor(acc[v], 32);
+        if (h == 0 && active) {
+            float* cp = carry_val + (size_t)wave * N + col;
+#pragma unroll
+            for (int v = 0; v < V; ++v) cp[v] = sum[v];
+        }
     }
 }
 
@@ -665,6 +811,29 @@ int launch_stream_vu(int U, dim3 grid, hipStream_t s, int M, int N, int nnz, int
     return ARMSP_SUCCESS;
 }
 
+template <typename TB>
+int launch_slice(int U, int S, int G, hipStream_t s, int M, int N, int nnz, int chunk, const int* rp, const int* ci,
+                 const float* va, const TB* B, int ldb, const Epi& bs, float* C, int ldc, int flags, int* crow,
+                 float* cval) {
+    const size_t lds = stream_lds_bytes(chunk);
+    const int64_t nblk = (8 % S == 0) ? ((int64_t)(G + 8 / S - 1) / (8 / S)) * 8 : (int64_t)G * S;
+    if (nblk > INT_MAX) return ARMSP_ERROR_NOT_SUPPORTED;
+    const dim3 grid((unsigned)nblk);
+    switch (U) {
+#define ARMSP_SLICE_CASE(UU)                                                                                  \
+    case UU:                                                                                                  \
+        hipLaunchKernelGGL((k_spmm_slice<TB, UU>), grid, dim3(256), lds, s, M, N, nnz, chunk, S, G, rp, ci, va, B, \
+                           ldb, bs, C, ldc, flags, crow, cval);                                               \
+        break;
+        ARMSP_SLICE_CASE(4)
+        ARMSP_SLICE_CASE(8)
+        ARMSP_SLICE_CASE(16)
+#undef ARMSP_SLICE_CASE
+        default: return ARMSP_ERROR_INVALID_VALUE;
+    }
+    return ARMSP_SUCCESS;
+}
+
 // MERGE gathers through 32-bit buffer offsets unless a dense-row offset could exceed them
 template <typename TB, int V, bool MERGE>
 int launch_split(bool wide, int U, dim3 grid, hipStream_t s, int M, int N, int nnz, int chunk, const int* rp,
@@ -749,8 +918,27 @@ int spmm_impl(int M, int K, int N, int nnz, const int* rowptr, const int* colind
 
     // dense-row byte offsets in 32 bits: (K-1)·ldb + N elements from B
     const bool wide = c.group == 1 || ((uint64_t)K * (uint64_t)ldb * sizeof(TB)) > 0xffffffffull;
+    // group 32: half a wave per nonzero over 128-byte column slices of B (k_spmm_slice)
+    constexpr int SV = 4 / (int)sizeof(TB);
+    const bool slice = merge && !wide && c.group == 32 && chunk <= 2048 && N % SV == 0 && ldb % SV == 0 &&
+                       ldc % SV == 0 && aligned(B, 4) && aligned(C, 4 * SV);
+    if (c.group == 32 && !slice) {
+        ws_free(ws, s);
+        return ARMSP_ERROR_NOT_SUPPORTED;
+    }
     dim3 grid((unsigned)((nwaves + 3) / 4), tiles);
-    {
+    if (slice) {
+        TimedLaunch t("spmm", s);
+        const int S = (N + 32 * SV - 1) / (32 * SV);
+        st = launch_slice<TB>(U, S, (int)((nwaves + 3) / 4), s, M, N, nnz, chunk, rowptr, colind, val, B, ldb, bscale,
+                              C, ldc, flags, crow, cval);
+        hipError_t e = st ? hipSuccess : hipGetLastError();
+        if (st || e != hipSuccess) {
+            ws_free(ws, s);
+            return st ? st : hip_fail(e);
+        }
+        t.stop();
+    } else {
         TimedLaunch t("spmm", s);
 #define ARMSP_SPLIT_V(VV)                                                                                          \
     case VV:                                                                                                       \

@@ -56,8 +56,8 @@ __global__ __launch_bounds__(256) void k_spmv_merge(int M, int nnz, const int* _
     if (d0l >= total) return;
     const int d0 = (int)d0l;
     const int d1 = (int)(d0l + TILE < total ? d0l + TILE : total);
-    const int i0 = uniform(merge_path_search(d0, rowptr, M, nnz));
-    const int i1 = uniform(merge_path_search(d1, rowptr, M, nnz));
+    const int i0 = uniform(merge_path_search_wave(d0, rowptr, M, nnz));
+    const int i1 = uniform(merge_path_search_wave(d1, rowptr, M, nnz));
     const int j0 = d0 - i0, j1 = d1 - i1;
     const int nrows = i1 - i0, nz = j1 - j0;
     int* re = s_re[wid];

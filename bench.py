@@ -33,7 +33,7 @@ def parse():
     p.add_argument("--gpus", type=int, default=1)
     p.add_argument("--steps", type=int, default=50)
     p.add_argument("--warmup", type=int, default=5)
-    p.add_argument("--only", default="", help="comma list: spmm,sddmm,spmv,mixed,gcn,gcn_scaled,cpu (default all)")
+    p.add_argument("--only", default="", help="comma list: spmm,sddmm,spmv,mixed,widths,shards,small,gcn,gcn_scaled,cpu (default all)")
     p.add_argument("--schedule", default="", help="override the SpMM schedule (row|nnz|merge)")
     p.add_argument("--chunk", type=int, default=0)
     p.add_argument("--unroll", type=int, default=0)
@@ -54,7 +54,8 @@ def main():
 
     from armsparse_amd import _lib, data, kernels
 
-    only = set(filter(None, args.only.split(","))) or {"spmm", "sddmm", "spmv", "mixed", "gcn", "gcn_scaled", "cpu"}
+    only = set(filter(None, args.only.split(","))) or {"spmm", "sddmm", "spmv", "mixed", "widths", "shards", "small",
+                                                        "gcn", "gcn_scaled", "cpu"}
     dev = torch.device("cuda", 0)
     torch.cuda.set_device(dev)
     stream = torch.cuda.current_stream(dev)
@@ -134,6 +135,52 @@ def main():
         extras["spmm_bf16_us"] = round(el_b / steps_x * 1e6, 1)
         extras["spmm_fp8_us"] = round(el_q / steps_x * 1e6, 1)
         del Bb, q, sc
+    if "widths" in only:  # the same matrix at N = 64 and 256 (fp32 and bf16)
+        for n2 in (64, 256):
+            B2 = torch.empty((K, n2), device=dev)
+            kernels.fill_uniform_(B2, data.CFG_SEED + 2, data.STREAM_DENSE)
+            C2 = torch.empty((M, n2), device=dev)
+            for nm, BB, sb in (("fp32", B2, 4), ("bf16", kernels.to_bf16(B2), 2)):
+                el_w, _ = timed(lambda: kernels.spmm(rp, ci, va, BB, out=C2), 10, 2, [])
+                byt = 4 * (M + 1) + 8 * nnz + sb * K * n2 + 4 * M * n2
+                extras[f"spmm_n{n2}_{nm}_us"] = round(el_w / 10 * 1e6, 1)
+                extras[f"spmm_n{n2}_{nm}_GBps"] = round(byt * 10 / el_w / 1e9, 1)
+            del B2, C2
+    if "shards" in only:  # one GPU's share of an 8-way row split: heaviest and median 131,072-row blocks
+        rows = M // 8
+        per = [int(A.rowptr[(i + 1) * rows] - A.rowptr[i * rows]) for i in range(8)]
+        order = sorted(range(8), key=lambda i: per[i])
+        for tag, i in (("heavy", order[-1]), ("median", order[3])):
+            lo, hi = int(A.rowptr[i * rows]), int(A.rowptr[(i + 1) * rows])
+            srp = (rp[i * rows:(i + 1) * rows + 1] - lo).contiguous()
+            sci, sva = ci[lo:hi], va[lo:hi]
+            Cs = C[i * rows:(i + 1) * rows]
+            el_s, _ = timed(lambda: kernels.spmm(srp, sci, sva, B, out=Cs), 10, 2, [])
+            extras[f"shard_{tag}_nnz"] = hi - lo
+            extras[f"shard_{tag}_us"] = round(el_s / 10 * 1e6, 1)
+        extras["shard_rows"] = rows
+    if "small" in only:  # the ./SpMM 4096 case and the 65,536² 0.1% case (latency-bound, reported anyway)
+        As, Bs, xs = data.spmm_small()
+        srp, sci, sva, sB, sx = (torch.from_numpy(np.ascontiguousarray(t)).to(dev) for t in
+                                 (As.rowptr, As.colind, As.values, Bs, xs))
+        sC = torch.empty((4096, 64), device=dev)
+        sy = torch.empty(4096, device=dev)
+        el_a, _ = timed(lambda: kernels.spmm(srp, sci, sva, sB, out=sC), 200, 10, [])
+        el_b, _ = timed(lambda: kernels.spmv(srp, sci, sva, sx, out=sy), 200, 10, [])
+        extras["cfg0_spmm_us"] = round(el_a / 200 * 1e6, 2)
+        extras["cfg0_spmv_us"] = round(el_b / 200 * 1e6, 2)
+        A6 = data.uniform_65k()
+        r6, c6, v6 = (torch.from_numpy(t).to(dev) for t in (A6.rowptr, A6.colind, A6.values))
+        x6 = torch.empty(65536, device=dev)
+        kernels.fill_uniform_(x6, data.CFG_SEED + 1, data.STREAM_DENSE + 1)
+        y6 = torch.empty(65536, device=dev)
+        el_c, _ = timed(lambda: kernels.spmv(r6, c6, v6, x6, out=y6), 100, 5, [])
+        extras["spmv_65k_us"] = round(el_c / 100 * 1e6, 2)
+        for kd in (32, 128):
+            U6 = torch.empty((65536, kd), device=dev)
+            kernels.fill_normal_(U6, data.CFG_SEED + 1, data.STREAM_DENSE + 2)
+            el_d, _ = timed(lambda: kernels.sddmm(r6, c6, U6, U6, scale=v6), 50, 5, [])
+            extras[f"sddmm_65k_kd{kd}_us"] = round(el_d / 50 * 1e6, 2)
     if "spmv" in only:
         x = torch.empty(K, device=dev)
         kernels.fill_uniform_(x, data.CFG_SEED + 1, data.STREAM_DENSE + 1)

@@ -50,9 +50,11 @@ typedef enum armsp_schedule {
 /* Schedule and tile shape.  Zero fields mean "choose automatically". */
 typedef struct armsp_config {
     int32_t schedule; /* armsp_schedule */
-    int32_t group;    /* lanes per row (ROW) or per nonzero (SDDMM): 1,2,4,...,64;
-                         SpMM NNZ/MERGE: 1 forces the 64-bit-address gather kernel
-                         (otherwise used only when a dense-row offset needs > 32 bits) */
+    int32_t group;    /* lanes per row (ROW) or per nonzero (SDDMM): 1,2,4,...,64.
+                         SpMM MERGE: 0/64 a whole wave per nonzero; 32 half a wave per
+                         nonzero over 128-byte column slices of B; 1 forces the
+                         64-bit-address kernel (otherwise used only when a dense-row
+                         offset needs more than 32 bits) */
     int32_t vec;      /* dense elements per lane: 1, 2, 4 (8 for bf16/fp8 B) */
     int32_t chunk;    /* items per wave for NNZ / MERGE (multiple of 64) */
     int32_t unroll;   /* nonzeros whose dense rows are fetched together: 1,2,4,8,16 */

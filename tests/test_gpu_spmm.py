@@ -102,7 +102,10 @@ def test_reference_tail_semantics():
 SPLIT_CFGS = ([dict(schedule=s, chunk=c, unroll=u) for s in ("merge", "nnz") for c in (64, 512, 2048)
                for u in (4, 8, 16)]
               + [dict(schedule="merge", chunk=c, unroll=u, group=1) for c in (64, 512) for u in (4, 8, 16)]
+              + [dict(schedule="merge", chunk=c, unroll=u, group=32) for c in (64, 512, 2048) for u in (4, 8, 16)]
               + [dict(schedule="merge", chunk=4096, unroll=8)])
+
+SLICE = dict(schedule="merge", group=32)  # half a wave per nonzero over 128-byte column slices
 
 
 @pytest.fixture(scope="module")
@@ -112,7 +115,8 @@ def rmat14():
 
 
 @pytest.mark.parametrize("cfg", SPLIT_CFGS,
-                         ids=lambda c: f"{c['schedule']}-c{c['chunk']}-u{c['unroll']}" + ("-wide" if c.get("group") else ""))
+                         ids=lambda c: f"{c['schedule']}-c{c['chunk']}-u{c['unroll']}"
+                         + {1: "-wide", 32: "-slice"}.get(c.get("group"), ""))
 def test_split_schedules_power_law(rmat14, cfg):
     case(f"split {cfg}")
     a, (d_rp, d_ci, d_va) = rmat14
@@ -132,12 +136,12 @@ def test_split_schedules_power_law(rmat14, cfg):
 
 
 @pytest.mark.parametrize("N", [1, 16, 32, 64, 96, 256, 512, 520])
-@pytest.mark.parametrize("sched", ["merge", "nnz", "merge-wide"])
+@pytest.mark.parametrize("sched", ["merge", "nnz", "merge-wide", "merge-slice"])
 def test_split_widths(rmat14, N, sched):
     case(f"split width N={N} {sched}")
     a, (d_rp, d_ci, d_va) = rmat14
     B = np.random.default_rng(N).uniform(-1, 1, (a.shape[1], N)).astype(np.float32)
-    cfg = dict(schedule="merge", group=1) if sched == "merge-wide" else sched
+    cfg = {"merge-wide": dict(schedule="merge", group=1), "merge-slice": SLICE}.get(sched, sched)
     out = kernels.spmm(d_rp, d_ci, d_va, dev(B)[0], config=cfg).cpu().numpy()
     ref = oracle.spmm_f64(a.rowptr, a.colind, a.values, B)
     if N >= 16:
@@ -165,12 +169,13 @@ def test_value_less_and_empty():
     assert bool((C == 3.0).all())
 
 
-@pytest.mark.parametrize("sched", ["row", "merge"])
+@pytest.mark.parametrize("sched", ["row", "merge", "merge-slice"])
 def test_bf16_dense(rmat14, sched):
     a, (d_rp, d_ci, d_va) = rmat14
     B = torch.from_numpy(np.random.default_rng(3).uniform(-1, 1, (a.shape[1], 128)).astype(np.float32))
     Bb = B.to(torch.bfloat16)
-    out = kernels.spmm(d_rp, d_ci, d_va, Bb.cuda(), config=sched).cpu().numpy()
+    cfg = SLICE if sched == "merge-slice" else sched
+    out = kernels.spmm(d_rp, d_ci, d_va, Bb.cuda(), config=cfg).cpu().numpy()
     deq = Bb.float().numpy()
     assert np.max(oracle.row_rel_err(out, oracle.spmm_f64(a.rowptr, a.colind, a.values, deq))) <= TOL
     err32 = np.max(oracle.row_rel_err(out, oracle.spmm_f64(a.rowptr, a.colind, a.values, B.numpy())))
@@ -191,12 +196,13 @@ def test_fp8_quantiser_matches_torch():
     assert torch.equal(q, expect_q)
 
 
-@pytest.mark.parametrize("sched", ["row", "merge"])
+@pytest.mark.parametrize("sched", ["row", "merge", "merge-slice"])
 def test_fp8_dense(rmat14, sched):
     a, (d_rp, d_ci, d_va) = rmat14
     B = torch.from_numpy(np.random.default_rng(8).uniform(-1, 1, (a.shape[1], 128)).astype(np.float32)).cuda()
     q, sc = kernels.quantize_fp8_rows(B)
-    out = kernels.spmm(d_rp, d_ci, d_va, q, dense_scale=sc, config=sched).cpu().numpy()
+    cfg = SLICE if sched == "merge-slice" else sched
+    out = kernels.spmm(d_rp, d_ci, d_va, q, dense_scale=sc, config=cfg).cpu().numpy()
     deq = (q.cpu().view(torch.float8_e4m3fn).float() * sc.cpu()[:, None]).numpy()
     assert np.max(oracle.row_rel_err(out, oracle.spmm_f64(a.rowptr, a.colind, a.values, deq))) <= TOL
     err32 = np.max(oracle.row_rel_err(out, oracle.spmm_f64(a.rowptr, a.colind, a.values, B.cpu().numpy())))
@@ -217,9 +223,11 @@ def test_full_size_rmat_1m_n128():
     rowwise = kernels.spmm(d_rp, d_ci, d_va, B, config="row").cpu().numpy()
     f64 = oracle.spmm_f64(a.rowptr, a.colind, a.values, B.cpu().numpy())
     check_rows(merged, rowwise, f64, "1M R-MAT merge")
+    sliced = kernels.spmm(d_rp, d_ci, d_va, B, config=SLICE).cpu().numpy()
+    check_rows(sliced, rowwise, f64, "1M R-MAT merge, column slices")
 
 
-@pytest.mark.parametrize("sched", ["row", "merge", "nnz"])
+@pytest.mark.parametrize("sched", ["row", "merge", "nnz", "slice"])
 @pytest.mark.parametrize("accumulate", [False, True])
 def test_epilogue_fusion(rmat14, sched, accumulate):
     """row_scale·(A·diag(col_scale)·B) + bias, ReLU — ReLU on rows cut across waves is applied
@@ -234,7 +242,7 @@ def test_epilogue_fusion(rmat14, sched, accumulate):
     bias = r.uniform(-0.5, 0.5, 64).astype(np.float32)
     C0 = r.uniform(-1, 1, (M, 64)).astype(np.float32)
     d_B, d_cs, d_rs, d_bias, d_C = dev(B, cs, rs, bias, C0)
-    cfg = dict(schedule=sched, chunk=256) if sched != "row" else "row"
+    cfg = {"row": "row", "slice": dict(SLICE, chunk=256)}.get(sched, dict(schedule=sched, chunk=256))
     out = kernels.spmm(d_rp, d_ci, d_va, d_B, out=d_C if accumulate else None, accumulate=accumulate,
                        col_scale=d_cs, row_scale=d_rs, bias=d_bias, relu=True, config=cfg).cpu().numpy()
     prod = oracle.spmm_f64(a.rowptr, a.colind, a.values * cs[a.colind], B)

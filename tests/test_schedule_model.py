@@ -1,0 +1,111 @@
+"""CPU model of the merge-path control flow of the LDS-staged SpMM kernels
+(k_spmm_stream: one nonzero per step; k_spmm_slice: a pair of nonzeros per
+step, lower and upper half-wave), step for step as the kernels walk a chunk:
+full batches with one row-end test per batch, then the tail.  Each chunk of
+the merge path must emit exactly the rows it completes, read only the row ends
+it staged, and leave the partial sum of its last row to the carry; the sum of
+emitted rows and carries must equal A·1 exactly (small-integer values)."""
+import numpy as np
+import pytest
+
+from armsparse_amd import data
+
+INF = 1 << 62
+
+
+def merge_search(rp, d, M, nnz):
+    lo, hi = max(0, d - nnz), min(d, M)
+    while lo < hi:
+        mid = (lo + hi) // 2
+        if rp[mid + 1] <= d - mid - 1:
+            lo = mid + 1
+        else:
+            hi = mid
+    return lo
+
+
+def run_model(rp, vals, chunk, U, pairs, last_bound="fixed"):
+    M, nnz = len(rp) - 1, int(rp[-1])
+    out = np.zeros(M, np.int64)
+    emitted = np.zeros(M, np.int64)
+    carries = {}
+    lanes = 2 if pairs else 1
+    step_len = 2 * U if pairs else U
+    for d0 in range(0, M + nnz, chunk):
+        d1 = min(d0 + chunk, M + nnz)
+        i0, i1 = merge_search(rp, d0, M, nnz), merge_search(rp, d1, M, nnz)
+        j0, j1 = d0 - i0, d1 - i1
+        nz, nr = j1 - j0, i1 - i0 + 1
+        s_re = [int(rp[i0 + k + 1]) - j0 if i0 + k < M else INF for k in range(nr)]
+        st = dict(rr=0, re=s_re[0], acc=[0] * lanes)
+
+        def emit():
+            assert st["rr"] < nr - 1, "emitted the chunk's carry row or past the staged rows"
+            r = i0 + st["rr"]
+            out[r] += sum(st["acc"])
+            emitted[r] += 1
+            st["acc"] = [0] * lanes
+            st["rr"] += 1
+            st["re"] = s_re[st["rr"]] if st["rr"] < nr else INF
+
+        def step(k2, last, has):
+            done = [not h for h in has]
+            while st["re"] <= last:
+                for h in range(lanes):
+                    if not done[h] and k2 + h < st["re"]:
+                        st["acc"][h] += int(vals[j0 + k2 + h])
+                        done[h] = True
+                emit()
+            for h in range(lanes):
+                if not done[h]:
+                    st["acc"][h] += int(vals[j0 + k2 + h])
+
+        kend = (nz // step_len) * step_len
+        for kb in range(0, kend, step_len):
+            if st["re"] >= kb + step_len:  # fast path: no row ends inside the batch
+                for k in range(kb, kb + step_len):
+                    st["acc"][(k - kb) % lanes] += int(vals[j0 + k])
+            else:
+                for k2 in range(kb, kb + step_len, lanes):
+                    step(k2, k2 + lanes - 1, [True] * lanes)
+        for k2 in range(kend, nz, lanes):
+            has = [k2 + h < nz for h in range(lanes)]
+            if last_bound == "fixed":
+                last = max(k2 + h for h in range(lanes) if has[h])
+            else:  # the bound the slice kernel first had: the pair's upper index even when it has no nonzero
+                last = k2 + lanes - 1
+            step(k2, last, has)
+        while st["rr"] < nr - 1:
+            emit()
+        if i1 < M and j1 > max(int(rp[i1]), j0):
+            carries.setdefault(i1, []).append(sum(st["acc"]))
+    for r, cs in carries.items():
+        out[r] += sum(cs)
+    return out, emitted
+
+
+@pytest.fixture(scope="module")
+def power_law():
+    a = data.rmat_csr(11, 1 << 14, seed=1234)
+    vals = np.random.default_rng(0).integers(-9, 10, a.nnz)
+    return a.rowptr.astype(np.int64), vals
+
+
+@pytest.mark.parametrize("pairs", [False, True], ids=["stream", "slice"])
+@pytest.mark.parametrize("chunk,U", [(64, 4), (64, 8), (128, 16), (512, 8), (2048, 16)])
+def test_merge_control_flow(power_law, pairs, chunk, U):
+    rp, vals = power_law
+    out, emitted = run_model(rp, vals, chunk, U, pairs)
+    expect = np.add.reduceat(np.append(vals, 0), rp[:-1])
+    expect[np.diff(rp) == 0] = 0
+    assert np.array_equal(out, expect)
+    assert np.all(emitted == 1)
+
+
+def test_model_catches_the_unbounded_tail(power_law):
+    """With the pair's upper index as the bound even when the upper half holds no
+    nonzero, a chunk whose carry row ends exactly at the chunk end emits it and
+    reads the row end past the staged ones."""
+    rp, vals = power_law
+    with pytest.raises(AssertionError, match="carry row or past"):
+        run_model(rp, vals, 64, 4, True, last_bound="old")
