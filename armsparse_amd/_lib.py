@@ -10,6 +10,12 @@ import ctypes
 import os
 from ctypes import POINTER, c_char_p, c_double, c_float, c_int, c_int64, c_uint64, c_void_p
 
+# torch first: it loads the HIP runtime it ships with, and libarmsparse.so's
+# libamdhip64.so.7 dependency then resolves to that same runtime.  Loaded the
+# other way round the process would hold two HIP runtimes, and launches on
+# torch's memory from the second one fail (invalid device).
+import torch  # noqa: F401,E402
+
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libarmsparse.so")
 
@@ -96,6 +102,7 @@ _SIGS = {
     "armsp_gemm_bf16": (c_int, [_I, _I, _I, _P, c_int64, c_int64, _P, c_int64, c_int64, _P, c_int64, _I, _P]),
     "armsp_convert_f32_to_bf16": (c_int, [c_int64, _P, _P, _P]),
     "armsp_quantize_fp8_rows": (c_int, [_I, _I, _P, _I, _P, _P, _P]),
+    "armsp_dequantize_fp8_rows": (c_int, [_I, _I, _P, _I, _P, _P, _I, _P]),
     "armsp_fill_uniform_f32": (c_int, [c_int64, c_uint64, c_uint64, c_float, c_float, _P, _P]),
     "armsp_fill_normal_f32": (c_int, [c_int64, c_uint64, c_uint64, _P, _P]),
     # learned schedule selector

@@ -26,7 +26,14 @@ int hip_fail(hipError_t e);
         if (e_ != hipSuccess) return ::armsp::hip_fail(e_);      \
     } while (0)
 
-inline hipStream_t as_stream(void* s) { return reinterpret_cast<hipStream_t>(s); }
+// The stream of an entry point.  Every entry point calls this before its
+// launches; it also clears an error that an earlier runtime call on this
+// thread left behind (hipGetLastError reports the last error of any call), so
+// the launch checks below report only this entry point's launches.
+inline hipStream_t as_stream(void* s) {
+    (void)hipGetLastError();
+    return reinterpret_cast<hipStream_t>(s);
+}
 
 // Stream-ordered workspace from the device's default memory pool.  The pool's
 // release threshold is raised at first use so freed blocks stay cached.
@@ -79,6 +86,17 @@ __device__ __forceinline__ float e4m3_to_f32(uint32_t b) {
         mag = __uint_as_float(((e + 120u) << 23) | (m << 20));
     }
     return __uint_as_float(__float_as_uint(mag) | sign);
+}
+
+// Four OCP e4m3 codes (bytes of w, lowest first) → fp32 with the gfx950
+// packed conversion (two codes per instruction).
+__device__ __forceinline__ void e4m3x4_to_f32(uint32_t w, float (&o)[4]) {
+    const auto lo = __builtin_amdgcn_cvt_pk_f32_fp8((int)w, false);
+    const auto hi = __builtin_amdgcn_cvt_pk_f32_fp8((int)w, true);
+    o[0] = lo[0];
+    o[1] = lo[1];
+    o[2] = hi[0];
+    o[3] = hi[1];
 }
 
 // Merge-path search: for diagonal d over (row ends rowptr[1..M], nnz 0..nnz-1)

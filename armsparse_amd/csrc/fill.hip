@@ -81,6 +81,24 @@ __global__ __launch_bounds__(256) void k_quant_fp8(int K, int N, const float* __
     for (int j = lane; j < N; j += 64) qrow[j] = (uint8_t)f32_to_e4m3(row[j] / sc);
 }
 
+// out[k][j] = decode(Q[k][j])·scale[k]; four codes per thread
+__global__ __launch_bounds__(256) void k_dequant_fp8(int K, int N, const uint8_t* __restrict__ Q, int ldq,
+                                                     const float* __restrict__ scale, float* __restrict__ out,
+                                                     int ldo) {
+    const int64_t quads = (int64_t)K * ((N + 3) / 4);
+    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < quads; t += (int64_t)gridDim.x * blockDim.x) {
+        const int qpr = (N + 3) / 4;
+        const int k = (int)(t / qpr), j = (int)(t - (int64_t)k * qpr) * 4;
+        const uint8_t* q = Q + (size_t)k * ldq + j;
+        uint32_t w = 0;
+        for (int i = 0; i < 4 && j + i < N; ++i) w |= (uint32_t)q[i] << (8 * i);
+        float f[4];
+        e4m3x4_to_f32(w, f);
+        const float sc = scale ? scale[k] : 1.0f;
+        for (int i = 0; i < 4 && j + i < N; ++i) out[(size_t)k * ldo + j + i] = f[i] * sc;
+    }
+}
+
 inline dim3 grid_for(int64_t n) {
     int64_t b = (n + 255) / 256;
     if (b > 65536) b = 65536;
@@ -130,6 +148,18 @@ int armsp_quantize_fp8_rows(int K, int N, const float* B, int ldb, uint8_t* Bq, 
     if (!B || !Bq || !scale) return ARMSP_ERROR_INVALID_VALUE;
     hipLaunchKernelGGL(k_quant_fp8, dim3((unsigned)((K + 3) / 4)), dim3(256), 0, as_stream(stream), K, N, B, ldb, Bq,
                        scale);
+    ARMSP_LAUNCHED();
+    return ARMSP_SUCCESS;
+}
+
+int armsp_dequantize_fp8_rows(int K, int N, const uint8_t* Bq, int ldq, const float* scale, float* out, int ldo,
+                              void* stream) {
+    using namespace armsp;
+    if (K < 0 || N < 0 || ldq < N || ldo < N) return ARMSP_ERROR_INVALID_VALUE;
+    if (K == 0 || N == 0) return ARMSP_SUCCESS;
+    if (!Bq || !out) return ARMSP_ERROR_INVALID_VALUE;
+    hipLaunchKernelGGL(k_dequant_fp8, grid_for((int64_t)K * ((N + 3) / 4)), dim3(256), 0, as_stream(stream), K, N, Bq,
+                       ldq, scale, out, ldo);
     ARMSP_LAUNCHED();
     return ARMSP_SUCCESS;
 }

@@ -3,11 +3,12 @@
 // Used for the dense feature transform of the GCN layer, x @ W
 // (gcn/pytorch-custom/op.py:114), and for its weight gradient xᵀ @ dY.  Both
 // operands take arbitrary element strides so transposed views need no copy.
-// Block tile 64×64×32, four waves each owning a 32×32 quarter; tiles are
-// staged in LDS k-contiguous so each lane fetches its 8 operand values with one
-// 16-byte LDS read.  When the output grid is too small to fill the chip the K
-// range is split; partial tiles go to a workspace and are summed in split
-// order by a second kernel (deterministic).
+// Three kernels: a tall one for M ≫ K, N (B resident in LDS, A streamed once),
+// a reduction one for K ≫ M, N (per-block K slabs into partials summed in
+// block order), and a general 64×64×32 tile kernel, split along K when its
+// tile grid is too small (partials again summed in split order).  Operand
+// tiles sit in LDS k-contiguous so each lane fetches its 8 MFMA operand
+// values with one 16-byte LDS read.
 #include "common.hpp"
 
 namespace armsp {
@@ -102,6 +103,185 @@ __global__ __launch_bounds__(256) void k_gemm_bf16(int M, int N, int K, int kchu
     }
 }
 
+// ------------------------------------------------------------ tall GEMM
+// C[M×N] (=|+=) A[M×K]·B[K×N] for M ≫ K, N (the GCN layer's x·W and the
+// gradient back to x): each block keeps all of B in LDS (n-major, k
+// contiguous) and loops over row tiles of A, so A is read once and B once per
+// block.  Four waves as WR row strips of 32 × (4/WR) column groups of 32·NTW;
+// a row tile is 32·WR rows, staged in LDS 32 k at a time with vector loads
+// along k when A's rows allow (va = 8, 4 or 1 elements).
+constexpr int TK = 32, TKP = TK + 8;  // A stage: 32 k, row pitch 40 bf16
+
+template <int WR, int NTW>
+__global__ __launch_bounds__(256) void k_gemm_tall(int M, int N, int K, int KP, const uint16_t* __restrict__ A,
+                                                   int64_t sam, int64_t sak, const uint16_t* __restrict__ B,
+                                                   int64_t sbk, int64_t sbn, float* __restrict__ C, int64_t ldc,
+                                                   int accumulate, int va) {
+    constexpr int TR = 32 * WR;               // rows per tile
+    constexpr int NP = 32 * NTW * (4 / WR);  // columns held
+    extern __shared__ __attribute__((aligned(16))) uint16_t lds[];
+    uint16_t* sB = lds;                      // [NP][KP + 8]
+    uint16_t* sA = lds + NP * (KP + 8);      // [TR][TKP]
+    const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+    const int wr = wid % WR, wc = wid / WR;
+    const int bp = KP + 8;
+    for (int idx = tid; idx < NP * KP; idx += 256) {
+        const int n = idx / KP, k = idx - n * KP;
+        sB[n * bp + k] = (n < N && k < K) ? B[(int64_t)k * sbk + (int64_t)n * sbn] : (uint16_t)0;
+    }
+    __syncthreads();
+    const int ntiles = (M + TR - 1) / TR;
+    for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+        const int m0 = tile * TR;
+        f32x16 acc[NTW];
+#pragma unroll
+        for (int j = 0; j < NTW; ++j)
+#pragma unroll
+            for (int i = 0; i < 16; ++i) acc[j][i] = 0.0f;
+        for (int k0 = 0; k0 < KP; k0 += TK) {
+            if (sak == 1 && va > 1) {  // rows of A contiguous in k: va-wide loads
+                const int per_row = TK / va;
+                for (int idx = tid; idx < TR * per_row; idx += 256) {
+                    const int r = idx / per_row, kk = (idx - r * per_row) * va;
+                    const int gm = m0 + r, gk = k0 + kk;
+                    uint16_t* dst = &sA[r * TKP + kk];
+                    const uint16_t* src = A + (int64_t)gm * sam + gk;
+                    if (gm < M && gk + va <= K) {
+                        if (va == 8)
+                            *reinterpret_cast<uint4*>(dst) = *reinterpret_cast<const uint4*>(src);
+                        else
+                            *reinterpret_cast<uint2*>(dst) = *reinterpret_cast<const uint2*>(src);
+                    } else {
+                        for (int i = 0; i < va; ++i) dst[i] = (gm < M && gk + i < K) ? src[i] : (uint16_t)0;
+                    }
+                }
+            } else {  // any strides: element loads, k fastest when sak == 1, else m fastest
+                for (int idx = tid; idx < TR * TK; idx += 256) {
+                    int r, kk;
+                    if (sak == 1) {
+                        r = idx / TK;
+                        kk = idx - r * TK;
+                    } else {
+                        kk = idx / TR;
+                        r = idx - kk * TR;
+                    }
+                    const int gm = m0 + r, gk = k0 + kk;
+                    sA[r * TKP + kk] = (gm < M && gk < K) ? A[(int64_t)gm * sam + (int64_t)gk * sak] : (uint16_t)0;
+                }
+            }
+            __syncthreads();
+#pragma unroll
+            for (int ks = 0; ks < TK; ks += 16) {
+                const int koff = ks + (lane >> 5) * 8;
+                const bf16x8 a = *reinterpret_cast<const bf16x8*>(&sA[(wr * 32 + (lane & 31)) * TKP + koff]);
+#pragma unroll
+                for (int j = 0; j < NTW; ++j) {
+                    const int n = (wc * NTW + j) * 32 + (lane & 31);
+                    const bf16x8 b = *reinterpret_cast<const bf16x8*>(&sB[n * bp + k0 + koff]);
+                    acc[j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc[j], 0, 0, 0);
+                }
+            }
+            __syncthreads();
+        }
+#pragma unroll
+        for (int j = 0; j < NTW; ++j) {
+            const int gn = (wc * NTW + j) * 32 + (lane & 31);
+            if (gn < N) {
+#pragma unroll
+                for (int i = 0; i < 16; ++i) {
+                    const int gm = m0 + wr * 32 + 8 * (i >> 2) + 4 * (lane >> 5) + (i & 3);
+                    if (gm < M) {
+                        float* p = C + (int64_t)gm * ldc + gn;
+                        *p = accumulate ? *p + acc[j][i] : acc[j][i];
+                    }
+                }
+            }
+        }
+    }
+}
+
+// ------------------------------------------------------- reduction GEMM
+// C[M×N] = Σ_k A(m,k)·B(k,n) over a long K with M, N ≤ 256 (the weight
+// gradient xᵀ·dY): block b reduces k ∈ [b·ks, (b+1)·ks) into part[b] with the
+// whole M×N output in MFMA accumulators (TPW 32×32 tiles per wave); operand
+// slabs of 32 k are staged in LDS with the global reads running along the
+// contiguous dimension.  The partials are summed in block order afterwards.
+template <int TPW>
+__global__ __launch_bounds__(256) void k_gemm_reduce(int M, int N, int K, int ks, const uint16_t* __restrict__ A,
+                                                     int64_t sam, int64_t sak, const uint16_t* __restrict__ B,
+                                                     int64_t sbk, int64_t sbn, float* __restrict__ part) {
+    extern __shared__ __attribute__((aligned(16))) uint16_t lds[];
+    const int MT = (M + 31) / 32, NT = (N + 31) / 32;
+    const int MP = 32 * MT, NP = 32 * NT;
+    uint16_t* sA = lds;              // [MP][TKP]
+    uint16_t* sB = lds + MP * TKP;   // [NP][TKP]
+    const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+    const int kb = blockIdx.x * ks, ke = min(K, kb + ks);
+    f32x16 acc[TPW];
+#pragma unroll
+    for (int j = 0; j < TPW; ++j)
+#pragma unroll
+        for (int i = 0; i < 16; ++i) acc[j][i] = 0.0f;
+    for (int k0 = kb; k0 < ke; k0 += TK) {
+        for (int idx = tid; idx < MP * TK; idx += 256) {
+            int m, kk;
+            if (sam == 1) {
+                kk = idx / MP;
+                m = idx - kk * MP;
+            } else {
+                m = idx / TK;
+                kk = idx - m * TK;
+            }
+            const int gk = k0 + kk;
+            sA[m * TKP + kk] = (m < M && gk < ke) ? A[(int64_t)m * sam + (int64_t)gk * sak] : (uint16_t)0;
+        }
+        for (int idx = tid; idx < NP * TK; idx += 256) {
+            int n, kk;
+            if (sbn == 1) {
+                kk = idx / NP;
+                n = idx - kk * NP;
+            } else {
+                n = idx / TK;
+                kk = idx - n * TK;
+            }
+            const int gk = k0 + kk;
+            sB[n * TKP + kk] = (n < N && gk < ke) ? B[(int64_t)gk * sbk + (int64_t)n * sbn] : (uint16_t)0;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int kq = 0; kq < TK; kq += 16) {
+            const int koff = kq + (lane >> 5) * 8;
+#pragma unroll
+            for (int j = 0; j < TPW; ++j) {
+                const int t = wid * TPW + j;
+                if (t < MT * NT) {
+                    const int tm = t / NT, tn = t - tm * NT;
+                    const bf16x8 a = *reinterpret_cast<const bf16x8*>(&sA[(tm * 32 + (lane & 31)) * TKP + koff]);
+                    const bf16x8 b = *reinterpret_cast<const bf16x8*>(&sB[(tn * 32 + (lane & 31)) * TKP + koff]);
+                    acc[j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc[j], 0, 0, 0);
+                }
+            }
+        }
+        __syncthreads();
+    }
+    float* P = part + (int64_t)blockIdx.x * M * N;
+#pragma unroll
+    for (int j = 0; j < TPW; ++j) {
+        const int t = wid * TPW + j;
+        if (t < MT * NT) {
+            const int tm = t / NT, tn = t - tm * NT;
+            const int gn = tn * 32 + (lane & 31);
+            if (gn < N) {
+#pragma unroll
+                for (int i = 0; i < 16; ++i) {
+                    const int gm = tm * 32 + 8 * (i >> 2) + 4 * (lane >> 5) + (i & 3);
+                    if (gm < M) P[(int64_t)gm * N + gn] = acc[j][i];
+                }
+            }
+        }
+    }
+}
+
 __global__ void k_splitk_reduce(int M, int N, int splits, const float* __restrict__ part, float* __restrict__ C,
                                 int64_t ldc, int accumulate) {
     const int64_t idx = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
@@ -123,6 +303,85 @@ extern "C" int armsp_gemm_bf16(int M, int N, int Kd, const uint16_t* A, int64_t 
     if (M == 0 || N == 0) return ARMSP_SUCCESS;
     if (!A || !B || !C) return ARMSP_ERROR_INVALID_VALUE;
     hipStream_t s = as_stream(stream);
+    // tall: M ≫ K, N — B resident in LDS, A streamed once
+    if (M >= 4096 && N <= 256 && Kd <= 256 && Kd > 0) {
+        const int KP = (Kd + TK - 1) / TK * TK;
+        int WR = 2, NTW = 4;
+        if (N <= 32) {
+            WR = 4;
+            NTW = 1;
+        } else if (N <= 64) {
+            NTW = 1;
+        } else if (N <= 128) {
+            NTW = 2;
+        }
+        const int NP = 32 * NTW * (4 / WR);
+        const size_t lds = ((size_t)NP * (KP + 8) + (size_t)32 * WR * TKP) * sizeof(uint16_t);
+        if (lds <= 96 * 1024) {
+            int va = 1;
+            if (sak == 1 && sam % 8 == 0 && ((uintptr_t)A % 16) == 0)
+                va = 8;
+            else if (sak == 1 && sam % 4 == 0 && ((uintptr_t)A % 8) == 0)
+                va = 4;
+            const int ntiles = (M + 32 * WR - 1) / (32 * WR);
+            const int nblk = ntiles < 2 * device_cus() ? ntiles : 2 * device_cus();
+            TimedLaunch t("gemm", s);
+#define ARMSP_TALL(WW, NN)                                                                                     \
+    hipLaunchKernelGGL((k_gemm_tall<WW, NN>), dim3(nblk), dim3(256), lds, s, M, N, Kd, KP, A, sam, sak, B, sbk, sbn, C, \
+                       (int64_t)ldc, accumulate, va)
+            if (WR == 4)
+                ARMSP_TALL(4, 1);
+            else if (NTW == 1)
+                ARMSP_TALL(2, 1);
+            else if (NTW == 2)
+                ARMSP_TALL(2, 2);
+            else
+                ARMSP_TALL(2, 4);
+#undef ARMSP_TALL
+            ARMSP_LAUNCHED();
+            t.stop();
+            return ARMSP_SUCCESS;
+        }
+    }
+    // reduction: K ≫ M, N — each block's K slab into a partial, summed in block order
+    if (Kd >= 65536 && M <= 256 && N <= 256) {
+        const int MT = (M + 31) / 32, NT = (N + 31) / 32;
+        const int tpw = (MT * NT + 3) / 4;
+        if (tpw <= 8) {
+            int nblk = 2 * device_cus();
+            int ks = (Kd + nblk - 1) / nblk;
+            ks = (ks + TK - 1) / TK * TK;
+            nblk = (Kd + ks - 1) / ks;
+            const size_t lds = (size_t)32 * (MT + NT) * TKP * sizeof(uint16_t);
+            void* ws = nullptr;
+            int st = ws_alloc(&ws, (size_t)nblk * M * N * sizeof(float), s);
+            if (st) return st;
+            float* part = static_cast<float*>(ws);
+            TimedLaunch t("gemm", s);
+#define ARMSP_RED(TT)                                                                                              \
+    hipLaunchKernelGGL((k_gemm_reduce<TT>), dim3(nblk), dim3(256), lds, s, M, N, Kd, ks, A, sam, sak, B, sbk, sbn, part)
+            if (tpw <= 1)
+                ARMSP_RED(1);
+            else if (tpw <= 2)
+                ARMSP_RED(2);
+            else if (tpw <= 4)
+                ARMSP_RED(4);
+            else
+                ARMSP_RED(8);
+#undef ARMSP_RED
+            hipError_t e = hipGetLastError();
+            if (e == hipSuccess) {
+                const int64_t n = (int64_t)M * N;
+                hipLaunchKernelGGL(k_splitk_reduce, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, M, N, nblk,
+                                   part, C, ldc, accumulate);
+                e = hipGetLastError();
+            }
+            ws_free(ws, s);
+            if (e != hipSuccess) return hip_fail(e);
+            t.stop();
+            return ARMSP_SUCCESS;
+        }
+    }
     const int gx = (M + BM - 1) / BM, gy = (N + BN - 1) / BN;
     if (gy > 65535) return ARMSP_ERROR_NOT_SUPPORTED;
     // split K when the tile grid is small and K is long

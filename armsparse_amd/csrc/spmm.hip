@@ -363,9 +363,14 @@ __device__ __forceinline__ void gather_b(__amdgpu_buffer_rsrc_t rs, uint32_t vof
     } else if constexpr (sizeof(TB) == 2) {
 #pragma unroll
         for (int i = 0; i < V; ++i) o[i] = __uint_as_float((i & 1) ? (w[i >> 1] & 0xffff0000u) : (w[i >> 1] << 16));
-    } else {
+    } else {  // packed fp8 conversion, two codes per instruction
 #pragma unroll
-        for (int i = 0; i < V; ++i) o[i] = e4m3_to_f32((w[i >> 2] >> (8 * (i & 3))) & 0xffu);
+        for (int i = 0; i < (V + 3) / 4; ++i) {
+            float f[4];
+            e4m3x4_to_f32(w[i], f);
+#pragma unroll
+            for (int q = 0; q < 4 && 4 * i + q < V; ++q) o[4 * i + q] = f[q];
+        }
     }
 }
 
@@ -673,6 +678,180 @@ __global__ __launch_bounds__(256) void k_spmm_slice(int M, int N, int nnz, int c
     }
 }
 
+// ---------------------------------------------- MERGE split for narrow N
+// For N ≤ 8 a wave per nonzero would leave most lanes idle, and a row per
+// lane group stalls on hub rows.  Here the wave first stages the products
+// a·B[c, 0..N) of its tile's nonzeros in LDS (every lane issues the gathers of
+// IPT nonzeros at once), then each lane walks IPT consecutive items of the
+// tile's merge path (as the SpMV merge kernel does), keeping the N outputs of
+// its current row in registers: rows that start and end inside the lane are
+// written directly; the first row a lane finishes also gets the partial sums
+// of the lanes before it (a segmented scan over the wave); the partial of the
+// tile's last row goes to the carry.
+template <typename TB, int NV, int IPT>
+__global__ __launch_bounds__(256) void k_spmm_narrow(int M, int N, int nnz, const int* __restrict__ rowptr,
+                                                     const int* __restrict__ colind, const float* __restrict__ val,
+                                                     const TB* __restrict__ B, int ldb, Epi ep, float* __restrict__ C,
+                                                     int ldc, int flags, int* __restrict__ carry_row,
+                                                     float* __restrict__ carry_val) {
+    constexpr int TILE = 64 * IPT;
+    __shared__ float s_prod[4][TILE * NV];
+    __shared__ int s_re[4][TILE + 1];
+    const int lane = threadIdx.x & 63;
+    const int wid = threadIdx.x >> 6;
+    const int wave = uniform((int)(blockIdx.x * 4u + wid));
+    const int64_t total = (int64_t)M + nnz;
+    const int64_t d0l = (int64_t)wave * TILE;
+    if (d0l >= total) return;
+    const int d0 = (int)d0l;
+    const int d1 = (int)min(d0l + TILE, total);
+    const int i0 = uniform(merge_path_search_wave(d0, rowptr, M, nnz));
+    const int i1 = uniform(merge_path_search_wave(d1, rowptr, M, nnz));
+    const int j0 = d0 - i0, j1 = d1 - i1;
+    const int nz = j1 - j0, nrows = i1 - i0;  // rows i0..i1-1 end inside the tile
+    const uint32_t rowbytes = (uint32_t)ldb * (uint32_t)sizeof(TB);
+    const __amdgpu_buffer_rsrc_t rs =
+        __builtin_amdgcn_make_buffer_rsrc(const_cast<TB*>(B), (short)0, (int)0xffffffff, 0x00020000);
+    float* prod = s_prod[wid];
+    int* re = s_re[wid];
+    {
+        // nonzero k = lane + 64·q; all IPT gathers of a lane are in flight together
+        int cs[IPT];
+        float as[IPT];
+#pragma unroll
+        for (int q = 0; q < IPT; ++q) {
+            const int k = lane + 64 * q;
+            cs[q] = 0;
+            as[q] = 0.0f;
+            if (k < nz) {
+                cs[q] = __builtin_nontemporal_load(colind + j0 + k);
+                as[q] = coef(val ? __builtin_nontemporal_load(val + j0 + k) : 1.0f, ep, cs[q]);
+            }
+        }
+        float bv[IPT][NV];
+#pragma unroll
+        for (int q = 0; q < IPT; ++q)
+#pragma unroll
+            for (int n = 0; n < NV; ++n) {
+                bv[q][n] = 0.0f;
+                if (n < N) {
+                    float b[1];
+                    gather_b<TB, 1>(rs, (uint32_t)cs[q] * rowbytes + (uint32_t)(n * (int)sizeof(TB)), b);
+                    bv[q][n] = b[0];
+                }
+            }
+#pragma unroll
+        for (int q = 0; q < IPT; ++q) {
+            const int k = lane + 64 * q;
+            if (k < nz) {
+#pragma unroll
+                for (int n = 0; n < NV; ++n) prod[k * NV + n] = as[q] * bv[q][n];
+            }
+        }
+    }
+    for (int k = lane; k < nrows; k += 64) re[k] = rowptr[i0 + k + 1] - j0;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+
+    const bool acc_mode = (flags & ARMSP_ACCUMULATE) != 0;
+    const int carry_in_row = (i0 < M && rowptr[i0] < j0) ? i0 : -1;
+    // this lane's start on the tile's merge path
+    const int items = nrows + nz;
+    const int ld = lane * IPT < items ? lane * IPT : items;
+    int lo = ld - nz > 0 ? ld - nz : 0, hi = ld < nrows ? ld : nrows;
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (re[mid] <= ld - mid - 1)
+            lo = mid + 1;
+        else
+            hi = mid;
+    }
+    int ri = lo, rj = ld - lo;
+    const int ri_start = ri;
+    float acc[NV], first[NV];
+#pragma unroll
+    for (int n = 0; n < NV; ++n) acc[n] = first[n] = 0.0f;
+    auto write_row = [&](int r, const float (&v)[NV]) {
+        float* cp = C + (size_t)r * ldc;
+#pragma unroll
+        for (int n = 0; n < NV; ++n) {
+            if (n < N) {
+                float o = v[n];
+                if (ep.rscale) o *= ep.rscale[r];
+                if (ep.bias) o += ep.bias[n];
+                if (acc_mode) o += cp[n];
+                if (ep.relu && r != carry_in_row) o = fmaxf(o, 0.0f);
+                cp[n] = o;
+            }
+        }
+    };
+#pragma unroll
+    for (int q = 0; q < IPT; ++q) {
+        if (ri + rj < items) {
+            if (ri < nrows && re[ri] <= rj) {  // row end
+                if (ri == ri_start) {
+#pragma unroll
+                    for (int n = 0; n < NV; ++n) first[n] = acc[n];
+                } else {
+                    write_row(i0 + ri, acc);
+                }
+#pragma unroll
+                for (int n = 0; n < NV; ++n) acc[n] = 0.0f;
+                ++ri;
+            } else {
+#pragma unroll
+                for (int n = 0; n < NV; ++n) acc[n] += prod[rj * NV + n];
+                ++rj;
+            }
+        }
+    }
+    // segmented inclusive scan of the open-row partials: key = the row each lane ends in
+    const int key = ri;
+    float S[NV];
+#pragma unroll
+    for (int n = 0; n < NV; ++n) S[n] = acc[n];
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const int ok = __shfl_up(key, d);
+#pragma unroll
+        for (int n = 0; n < NV; ++n) {
+            const float os = __shfl_up(S[n], d);
+            if (lane >= d && ok == key) S[n] += os;
+        }
+    }
+    // the first row this lane finished began in earlier lanes when ri_start is
+    // the key the lane before ended in: add their partials
+    const int prev_key = __shfl_up(key, 1);
+    float prev[NV];
+#pragma unroll
+    for (int n = 0; n < NV; ++n) prev[n] = __shfl_up(S[n], 1);
+    if (ri > ri_start) {
+        if (lane > 0 && prev_key == ri_start) {
+#pragma unroll
+            for (int n = 0; n < NV; ++n) first[n] += prev[n];
+        }
+        write_row(i0 + ri_start, first);
+    }
+    // partial of row i1 (the scan total of the lanes that end in it) → carry
+    float tail[NV];
+#pragma unroll
+    for (int n = 0; n < NV; ++n) tail[n] = read_lane(S[n], 63);
+    if (lane == 0) {
+        bool has = false;
+        if (i1 < M) {
+            const int rs0 = rowptr[i1];
+            has = j1 > (rs0 > j0 ? rs0 : j0);
+        }
+        carry_row[wave] = has ? i1 : -1;
+        if (has) {
+#pragma unroll
+            for (int n = 0; n < NV; ++n)
+                if (n < N) carry_val[(size_t)wave * N + n] = tail[n];
+        }
+    }
+}
+
 inline size_t stream_lds_bytes(int chunk) {
     return 4 * (((size_t)chunk * 8 + (size_t)(chunk + 1) * 4 + 15) & ~(size_t)15);
 }
@@ -834,6 +1013,36 @@ int launch_slice(int U, int S, int G, hipStream_t s, int M, int N, int nnz, int 
     return ARMSP_SUCCESS;
 }
 
+template <typename TB, int NV>
+int launch_narrow_nv(int ipt, int64_t nwaves, hipStream_t s, int M, int N, int nnz, const int* rp, const int* ci,
+                     const float* va, const TB* B, int ldb, const Epi& bs, float* C, int ldc, int flags, int* crow,
+                     float* cval) {
+    const dim3 grid((unsigned)((nwaves + 3) / 4));
+    switch (ipt) {
+#define ARMSP_NARROW_CASE(II)                                                                                   \
+    case II:                                                                                                    \
+        hipLaunchKernelGGL((k_spmm_narrow<TB, NV, II>), grid, dim3(256), 0, s, M, N, nnz, rp, ci, va, B, ldb, bs, C, \
+                           ldc, flags, crow, cval);                                                             \
+        break;
+        ARMSP_NARROW_CASE(4)
+        ARMSP_NARROW_CASE(8)
+        ARMSP_NARROW_CASE(16)
+#undef ARMSP_NARROW_CASE
+        default: return ARMSP_ERROR_INVALID_VALUE;
+    }
+    return ARMSP_SUCCESS;
+}
+
+template <typename TB>
+int launch_narrow(int ipt, int64_t nwaves, hipStream_t s, int M, int N, int nnz, const int* rp, const int* ci,
+                  const float* va, const TB* B, int ldb, const Epi& bs, float* C, int ldc, int flags, int* crow,
+                  float* cval) {
+    if (N <= 1) return launch_narrow_nv<TB, 1>(ipt, nwaves, s, M, N, nnz, rp, ci, va, B, ldb, bs, C, ldc, flags, crow, cval);
+    if (N <= 2) return launch_narrow_nv<TB, 2>(ipt, nwaves, s, M, N, nnz, rp, ci, va, B, ldb, bs, C, ldc, flags, crow, cval);
+    if (N <= 4) return launch_narrow_nv<TB, 4>(ipt, nwaves, s, M, N, nnz, rp, ci, va, B, ldb, bs, C, ldc, flags, crow, cval);
+    return launch_narrow_nv<TB, 8>(ipt, nwaves, s, M, N, nnz, rp, ci, va, B, ldb, bs, C, ldc, flags, crow, cval);
+}
+
 // MERGE gathers through 32-bit buffer offsets unless a dense-row offset could exceed them
 template <typename TB, int V, bool MERGE>
 int launch_split(bool wide, int U, dim3 grid, hipStream_t s, int M, int N, int nnz, int chunk, const int* rp,
@@ -862,7 +1071,9 @@ int spmm_impl(int M, int K, int N, int nnz, const int* rowptr, const int* colind
 
     armsp_config c = cfg ? *cfg : armsp_config{0, 0, 0, 0, 0};
     int sched = c.schedule;
-    if (sched == ARMSP_SCHED_AUTO) sched = (N >= 32) ? ARMSP_SCHED_MERGE : ARMSP_SCHED_ROW;
+    // AUTO: merge path except for 9..31 columns (a row per lane group); N ≤ 8 runs the
+    // per-lane item walk (k_spmm_narrow), wider N a whole wave per nonzero
+    if (sched == ARMSP_SCHED_AUTO) sched = (N >= 32 || N <= 8) ? ARMSP_SCHED_MERGE : ARMSP_SCHED_ROW;
     if (flags & ARMSP_TRUNC4) sched = ARMSP_SCHED_ROW;  // truncation is per row
     if (nnz == 0) sched = ARMSP_SCHED_ROW;               // writes (or keeps) every row
 
@@ -903,6 +1114,15 @@ int spmm_impl(int M, int K, int N, int nnz, const int* rowptr, const int* colind
     if (U != 4 && U != 8 && U != 16) return ARMSP_ERROR_INVALID_VALUE;
     int chunk = c.chunk > 0 ? c.chunk : 512;
     if (chunk % 64) return ARMSP_ERROR_INVALID_VALUE;
+    // dense-row byte offsets in 32 bits: (K-1)·ldb + N elements from B
+    const bool wide = c.group == 1 || ((uint64_t)K * (uint64_t)ldb * sizeof(TB)) > 0xffffffffull;
+    // N ≤ 8 (group 0): lanes walk runs of 4, 8 or 16 merge items (k_spmm_narrow); the
+    // wave tile is 64 × that run, so the chunk is rounded to 256, 512 or 1024
+    const bool narrow = merge && !wide && N <= 8 && c.group == 0;
+    if (narrow) {  // default: runs of 8 items for N ≤ 2, of 4 above (the staged products fill the LDS)
+        if (c.chunk <= 0) chunk = N <= 2 ? 512 : 256;
+        chunk = chunk <= 256 ? 256 : (chunk <= 512 ? 512 : 1024);
+    }
     const int64_t total = merge ? (int64_t)M + nnz : (int64_t)nnz;
     const int64_t nwaves = (total + chunk - 1) / chunk;
     if (nwaves > INT_MAX / 4) return ARMSP_ERROR_NOT_SUPPORTED;
@@ -916,8 +1136,6 @@ int spmm_impl(int M, int K, int N, int nnz, const int* rowptr, const int* colind
     int* crow = static_cast<int*>(ws);
     float* cval = reinterpret_cast<float*>(static_cast<char*>(ws) + row_bytes);
 
-    // dense-row byte offsets in 32 bits: (K-1)·ldb + N elements from B
-    const bool wide = c.group == 1 || ((uint64_t)K * (uint64_t)ldb * sizeof(TB)) > 0xffffffffull;
     // group 32: half a wave per nonzero over 128-byte column slices of B (k_spmm_slice)
     constexpr int SV = 4 / (int)sizeof(TB);
     const bool slice = merge && !wide && c.group == 32 && chunk <= 2048 && N % SV == 0 && ldb % SV == 0 &&
@@ -927,7 +1145,17 @@ int spmm_impl(int M, int K, int N, int nnz, const int* rowptr, const int* colind
         return ARMSP_ERROR_NOT_SUPPORTED;
     }
     dim3 grid((unsigned)((nwaves + 3) / 4), tiles);
-    if (slice) {
+    if (narrow) {
+        TimedLaunch t("spmm", s);
+        st = launch_narrow<TB>(chunk / 64, nwaves, s, M, N, nnz, rowptr, colind, val, B, ldb, bscale, C, ldc, flags,
+                               crow, cval);
+        hipError_t e = st ? hipSuccess : hipGetLastError();
+        if (st || e != hipSuccess) {
+            ws_free(ws, s);
+            return st ? st : hip_fail(e);
+        }
+        t.stop();
+    } else if (slice) {
         TimedLaunch t("spmm", s);
         const int S = (N + 32 * SV - 1) / (32 * SV);
         st = launch_slice<TB>(U, S, (int)((nwaves + 3) / 4), s, M, N, nnz, chunk, rowptr, colind, val, B, ldb, bscale,

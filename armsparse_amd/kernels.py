@@ -319,6 +319,20 @@ def quantize_fp8_rows(b: torch.Tensor):
     return q, sc
 
 
+def dequantize_fp8_rows(q: torch.Tensor, scale: Optional[torch.Tensor] = None) -> torch.Tensor:
+    """float32 [K×N] = e4m3(q)·scale[:, None] with the device's packed fp8 conversion."""
+    _require(q.is_cuda and q.dtype == torch.uint8 and q.dim() == 2 and q.stride(1) == 1,
+             "dequantize_fp8_rows: needs a uint8 device matrix with contiguous rows")
+    k, n = q.shape
+    out = torch.empty((k, n), dtype=torch.float32, device=q.device)
+    if scale is not None:
+        _require(scale.is_cuda and scale.dtype == torch.float32 and scale.numel() == k and scale.is_contiguous(),
+                 "dequantize_fp8_rows: bad scale")
+    if k and n:
+        call("armsp_dequantize_fp8_rows", k, n, _ptr(q), q.stride(0), _ptr(scale), _ptr(out), n, _stream(q.device))
+    return out
+
+
 def fill_uniform_(t: torch.Tensor, seed: int, stream_id: int, lo: float = -1.0, hi: float = 1.0) -> torch.Tensor:
     _require(t.is_cuda and t.dtype == torch.float32 and t.is_contiguous(), "fill_uniform_: float32 device tensor")
     call("armsp_fill_uniform_f32", t.numel(), seed, stream_id, lo, hi, _ptr(t), _stream(t.device))

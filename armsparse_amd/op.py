@@ -68,26 +68,39 @@ class SPMMFunction(torch.autograd.Function):
 
 class GCNAggregate(torch.autograd.Function):
     """y = diag(in_norm)·A·diag(out_norm)·h + bias in one fused SpMM (armsp_spmm_ex);
-    backward dh = diag(out_norm)·Aᵀ·diag(in_norm)·dy in one SpMM on (colptr, rowind, w_csc).
-    h may be bfloat16 (fp32 accumulation; y and the gradients are fp32)."""
+    backward dh = diag(out_norm)·Aᵀ·diag(in_norm)·dy in one SpMM on (colptr, rowind).
+    The column scales are folded into per-edge coefficients once per graph:
+    coef_fwd[e] = w_csr[e]·out_norm[colind[e]], coef_bwd[e] = w_csc[e]·in_norm[rowind[e]]
+    (the same products the kernel would form per nonzero).  h may be bfloat16
+    (fp32 accumulation; y and the gradients are fp32)."""
 
     @staticmethod
-    def forward(ctx, h, rowptr, colind, colptr, rowind, w_csr, w_csc, in_norm, out_norm, bias):
-        y = kernels.spmm(rowptr, colind, w_csr, h, col_scale=out_norm, row_scale=in_norm, bias=bias)
-        ctx.graph = (colptr, rowind, w_csr, w_csc, in_norm, out_norm, h.dtype)
+    def forward(ctx, h, rowptr, colind, colptr, rowind, coef_fwd, coef_bwd, in_norm, out_norm, bias):
+        y = kernels.spmm(rowptr, colind, coef_fwd, h, row_scale=in_norm, bias=bias)
+        ctx.graph = (colptr, rowind, coef_bwd, out_norm, h.dtype)
         return y
 
     @staticmethod
     def backward(ctx, gy):
-        colptr, rowind, w_csr, w_csc, in_norm, out_norm, hdt = ctx.graph
-        if w_csr is not None and w_csc is None:
-            raise RuntimeError("GCNAggregate backward needs the edge values in CSC order as well")
+        colptr, rowind, coef_bwd, out_norm, hdt = ctx.graph
         g = gy.contiguous()
         if hdt == torch.bfloat16:
             g = kernels.to_bf16(g.float())
-        dh = kernels.spmm(colptr, rowind, w_csc, g, col_scale=in_norm, row_scale=out_norm)
+        dh = kernels.spmm(colptr, rowind, coef_bwd, g, row_scale=out_norm)
         dbias = gy.sum(0) if ctx.needs_input_grad[9] else None
         return dh.to(hdt), None, None, None, None, None, None, None, None, dbias
+
+
+def gcn_edge_coefficients(colind, rowind, w_csr, w_csc, in_norm, out_norm):
+    """Per-edge coefficients of the fused GCN aggregation (see GCNAggregate)."""
+    if w_csr is not None and w_csc is None:
+        raise RuntimeError("GCNAggregate backward needs the edge values in CSC order as well")
+    cf = out_norm[colind.long()] if out_norm is not None else torch.ones(colind.numel(), device=colind.device)
+    cb = in_norm[rowind.long()] if in_norm is not None else torch.ones(rowind.numel(), device=rowind.device)
+    if w_csr is not None:
+        cf = cf * w_csr
+        cb = cb * w_csc
+    return cf.contiguous(), cb.contiguous()
 
 
 class _BF16Linear(torch.autograd.Function):
@@ -142,6 +155,7 @@ class GCNConv(torch.nn.Module):
         zeros(self.bias)
         self.cached_result = None
         self.cached_num_edges = None
+        self.cached_coef = None
 
     @staticmethod
     def in_deg_sqrt(indptr):
@@ -167,10 +181,18 @@ class GCNConv(torch.nn.Module):
             self.cached_result = in_deg_norm, out_deg_norm
         in_deg_norm, out_deg_norm = self.cached_result
         if self.fused:
-            h = x.to(torch.bfloat16) if self.precision == "bf16" else x
-            return GCNAggregate.apply(h.contiguous(), rowptr, colind, colptr, rowind, edge_weight_csr,
-                                      edge_weight_csc, in_deg_norm.reshape(-1).contiguous() if self.normalize else None,
-                                      out_deg_norm.reshape(-1).contiguous() if self.normalize else None, self.bias)
+            in_n = in_deg_norm.reshape(-1).contiguous() if self.normalize else None
+            out_n = out_deg_norm.reshape(-1).contiguous() if self.normalize else None
+            coefs = self.cached_coef if self.cached else None
+            if coefs is None:
+                coefs = gcn_edge_coefficients(colind, rowind, edge_weight_csr, edge_weight_csc, in_n, out_n)
+                if self.cached:
+                    self.cached_coef = coefs
+            # bf16 features halve the gather bytes; a handful of output channels gather
+            # next to nothing, so those stay fp32 (no conversions either way)
+            h = x.to(torch.bfloat16) if self.precision == "bf16" and self.out_channels > 8 else x
+            return GCNAggregate.apply(h.contiguous(), rowptr, colind, colptr, rowind, coefs[0], coefs[1], in_n,
+                                      out_n, self.bias)
         if self.normalize:
             x = x * out_deg_norm
         if self.precision == "bf16":

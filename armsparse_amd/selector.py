@@ -43,11 +43,15 @@ WEIGHTS_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "selecto
 # without the network".
 SPMM_CONFIGS = [None,
                 dict(schedule="row"), dict(schedule="row", group=16),
-                dict(schedule="nnz", chunk=256, unroll=8), dict(schedule="nnz", chunk=1024, unroll=8),
-                dict(schedule="merge", chunk=256, unroll=8), dict(schedule="merge", chunk=1024, unroll=8),
-                dict(schedule="merge", chunk=2048, unroll=16), dict(schedule="merge", chunk=512, unroll=4),
-                dict(schedule="merge", chunk=1024, unroll=16)]
-SDDMM_CONFIGS = [None] + [dict(schedule="nnz", group=g, chunk=c) for g in (4, 8, 16, 32) for c in (128, 512, 2048)]
+                dict(schedule="nnz", chunk=256, unroll=8),
+                dict(schedule="merge", chunk=128, unroll=16), dict(schedule="merge", chunk=256, unroll=4),
+                dict(schedule="merge", chunk=256, unroll=16), dict(schedule="merge", chunk=512, unroll=8),
+                dict(schedule="merge", chunk=1024, unroll=8),
+                dict(schedule="merge", group=32, chunk=256, unroll=4),
+                dict(schedule="merge", group=32, chunk=512, unroll=8),
+                dict(schedule="merge", group=64, chunk=256, unroll=8)]
+SDDMM_CONFIGS = ([None] + [dict(schedule="merge", group=g, chunk=c) for g in (4, 8, 16) for c in (128, 256, 512)]
+                 + [dict(schedule="nnz", group=g, chunk=256) for g in (8, 16)])
 CONFIGS = {"spmm": SPMM_CONFIGS, "sddmm": SDDMM_CONFIGS}
 
 
@@ -297,16 +301,26 @@ def corpus_specs(n: int, seed: int = 2024, nnz_cap: int = 1 << 22) -> list:
             param = float(max(1, round(avg / 32)))
             avg = 32 * param
         if m * avg > nnz_cap:
-            if kind == "blockdiag":
+            if kind == "blockdiag":  # ≥ 32 nonzeros per row: fewer blocks, then fewer rows
                 param = float(max(1, nnz_cap // (32 * m)))
                 avg = 32 * param
+                while m * avg > nnz_cap:
+                    m //= 2
             else:
                 avg = nnz_cap / m
         specs.append(dict(kind=kind, m=m, avg=avg, param=param, seed=seed * 100003 + i))
     return specs
 
 
-def build_dataset(specs: list, widths=WIDTHS, log=print) -> dict:
+def _pack(feats, times, mids) -> dict:
+    return {op: dict(x=np.array(feats[op]), t=np.array(times[op]), mid=np.array(mids[op])) for op in feats}
+
+
+def save_dataset(path: str, ds: dict) -> None:
+    np.savez_compressed(path, **{f"{op}_{k}": v for op, d in ds.items() for k, v in d.items()})
+
+
+def build_dataset(specs: list, widths=WIDTHS, log=print, checkpoint: str = "") -> dict:
     """Autotune every (matrix, width) for SpMM and SDDMM; matrices are generated
     on the host by a background thread while the device times the previous one."""
     q: "queue.Queue" = queue.Queue(maxsize=4)
@@ -341,7 +355,9 @@ def build_dataset(specs: list, widths=WIDTHS, log=print) -> dict:
         del prob, rp, ci, va
         if (idx + 1) % 25 == 0 or idx + 1 == len(specs):
             log(f"[selector] {idx + 1}/{len(specs)} matrices autotuned, {time.time() - t0:.0f}s", flush=True)
-    return {op: dict(x=np.array(feats[op]), t=np.array(times[op]), mid=np.array(mids[op])) for op in feats}
+        if checkpoint and (idx + 1) % 100 == 0:
+            save_dataset(checkpoint, _pack(feats, times, mids))
+    return _pack(feats, times, mids)
 
 
 # ------------------------------------------------------------ evaluation
@@ -432,9 +448,10 @@ def main(argv=None):
         z = np.load(a.dataset)
         ds = {op: dict(x=z[f"{op}_x"], t=z[f"{op}_t"], mid=z[f"{op}_mid"]) for op in ("spmm", "sddmm")}
     else:
-        ds = build_dataset(corpus_specs(a.matrices, a.seed), log=lambda *x, **k: print(*x, **k, file=sys.stderr))
-        np.savez_compressed(os.path.join(a.out, "dataset.npz"),
-                            **{f"{op}_{k}": v for op, d in ds.items() for k, v in d.items()})
+        path = os.path.join(a.out, "dataset.npz")
+        ds = build_dataset(corpus_specs(a.matrices, a.seed), log=lambda *x, **k: print(*x, **k, file=sys.stderr),
+                           checkpoint=path)
+        save_dataset(path, ds)
     report, nets = train_eval(ds, steps=a.steps, log=lambda *x: print(*x, file=sys.stderr, flush=True))
     report["configs"] = {op: c for op, c in CONFIGS.items()}
     with open(os.path.join(a.out, "report.json"), "w") as f:

@@ -33,7 +33,8 @@ def parse():
     p.add_argument("--gpus", type=int, default=1)
     p.add_argument("--steps", type=int, default=50)
     p.add_argument("--warmup", type=int, default=5)
-    p.add_argument("--only", default="", help="comma list: spmm,sddmm,spmv,mixed,widths,shards,small,gcn,gcn_scaled,cpu (default all)")
+    p.add_argument("--only", default="", help="comma list: spmm,sddmm,spmv,mixed,widths,shards,small,gcn,gcn_scaled,cpu (default all); "
+                        "regimes (column-placement variants of the headline matrix) runs only when named")
     p.add_argument("--schedule", default="", help="override the SpMM schedule (row|nnz|merge)")
     p.add_argument("--chunk", type=int, default=0)
     p.add_argument("--unroll", type=int, default=0)
@@ -135,6 +136,21 @@ def main():
         extras["spmm_bf16_us"] = round(el_b / steps_x * 1e6, 1)
         extras["spmm_fp8_us"] = round(el_q / steps_x * 1e6, 1)
         del Bb, q, sc
+    if "regimes" in only:  # not in the default set: what bounds the gathers (see the counter summaries)
+        rng = np.random.default_rng(0)
+        c64 = A.colind.astype(np.int64)
+        variants = {"rmat": A.colind, "rmat_mod64k": (c64 % 65536).astype(np.int32),
+                    "uniform": rng.integers(0, K, nnz).astype(np.int32),
+                    "rmat_oddmul": ((c64 * 2654435761) & (K - 1)).astype(np.int32)}
+        rows = np.repeat(np.arange(M, dtype=np.int64), np.diff(A.rowptr))
+        for name, cv in variants.items():
+            order = np.argsort(rows * K + cv, kind="stable")  # keep every row's columns sorted
+            vci = torch.from_numpy(np.ascontiguousarray(cv[order])).to(dev)
+            vva = torch.from_numpy(np.ascontiguousarray(A.values[order])).to(dev)
+            el_r, fam_r = timed(lambda: kernels.spmm(rp, vci, vva, B, out=C), max(2, args.steps // 5), 1, ["spmm"])
+            extras[f"regime_{name}_kernel_us"] = round(fam_r["spmm"][0] / max(fam_r["spmm"][1], 1) * 1e3, 1)
+            log(f"[bench] regime {name}: {extras[f'regime_{name}_kernel_us']} us")
+            del vci, vva
     if "widths" in only:  # the same matrix at N = 64 and 256 (fp32 and bf16)
         for n2 in (64, 256):
             B2 = torch.empty((K, n2), device=dev)

@@ -210,6 +210,12 @@ int armsp_convert_f32_to_bf16(int64_t n, const float* x, uint16_t* y, void* stre
  * is zero), Bq = RNE(B / scale) saturated to ±448. */
 int armsp_quantize_fp8_rows(int K, int N, const float* B, int ldb,
                             uint8_t* Bq, float* scale, void* stream);
+
+/* Inverse of armsp_quantize_fp8_rows: out[k][j] = e4m3(Bq[k][j])·scale[k]
+ * (scale NULL: 1), decoded by the gfx950 packed fp8 conversion that the fp8
+ * SpMM uses.  Extension (the reference is fp32 only, spmm.cpp:112). */
+int armsp_dequantize_fp8_rows(int K, int N, const uint8_t* Bq, int ldq, const float* scale,
+                              float* out, int ldo, void* stream);
 /* Counter-based fills (same streams as the host generators in armsp_gen.h). */
 int armsp_fill_uniform_f32(int64_t n, uint64_t seed, uint64_t stream_id,
                            float lo, float hi, float* out, void* stream);

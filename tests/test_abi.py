@@ -3,6 +3,7 @@ and rejects bad arguments on the host before any GPU work (no GPU needed)."""
 import os
 import re
 import subprocess
+import sys
 import textwrap
 
 import pytest
@@ -112,3 +113,13 @@ def test_compat_shims_reject_bad_input(tmp_path, case, msg):
     r = subprocess.run([str(exe), case], capture_output=True, text=True, timeout=60)
     assert r.returncode == 1
     assert msg in r.stderr and r.stderr.startswith("armsparse: ")
+
+
+def test_one_hip_runtime_whatever_the_import_order():
+    """Importing the package before torch must not map a second HIP runtime next to
+    torch's (launches on torch memory from the other one fail)."""
+    code = ("import armsparse_amd, torch, re\n"
+            "print(len(set(re.findall(r'\\S*libamdhip64\\S*', open('/proc/self/maps').read()))))")
+    out = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, cwd=ROOT, timeout=300)
+    assert out.returncode == 0, out.stderr
+    assert out.stdout.strip() == "1"

@@ -185,6 +185,28 @@ def test_gemm_bf16_mfma_layout(m, n, k):
     assert bool(((c.cpu().double() - ref - 1).abs() <= TOL * (bound + 1) + 1e-30).all())
 
 
+@pytest.mark.parametrize("m,n,k,layout", [
+    (20000, 256, 100, "nn"), (20011, 250, 99, "nn"), (20000, 3, 256, "nn"), (20000, 64, 37, "nn"),
+    (20000, 256, 3, "nt"), (12000, 100, 128, "tn"),              # tall: B resident, A streamed once
+    (100, 256, 70000, "tn"), (256, 3, 70001, "tn"), (37, 70, 66000, "nn")])  # reduction over a long K
+def test_gemm_bf16_tall_and_reduction(m, n, k, layout):
+    """The GCN layer's shapes: x·W (tall), the gradient back to x (tall, Wᵀ view) and
+    xᵀ·dY (reduction, xᵀ view); against an FP64 GEMM of the same bf16 inputs."""
+    case(f"gemm {layout} {m}x{n}x{k}")
+    r = np.random.default_rng(m + 3 * n + 7 * k)
+    a = torch.from_numpy(r.standard_normal((m, k)).astype(np.float32)).to(torch.bfloat16)
+    b = torch.from_numpy(r.standard_normal((k, n)).astype(np.float32)).to(torch.bfloat16)
+    ref = a.double() @ b.double()
+    bound = a.double().abs() @ b.double().abs()
+    da = a.cuda() if layout[0] == "n" else a.t().contiguous().cuda().t()
+    db = b.cuda() if layout[1] == "n" else b.t().contiguous().cuda().t()
+    out = kernels.gemm_bf16(da, db).cpu().double()
+    assert bool(((out - ref).abs() <= TOL * bound + 1e-30).all())
+    c = torch.full((m, n), 2.0, device="cuda")
+    kernels.gemm_bf16(da, db, out=c, accumulate=True)
+    assert bool(((c.cpu().double() - ref - 2).abs() <= TOL * (bound + 2) + 1e-30).all())
+
+
 # ----------------------------------------------------------------- fills
 def test_device_uniform_fill_matches_host():
     t = torch.empty(100_003, device="cuda")

@@ -151,6 +151,41 @@ def test_split_widths(rmat14, N, sched):
         assert np.all(np.abs(out - ref) <= TOL * scale + 1e-30)
 
 
+@pytest.mark.parametrize("N", [1, 2, 3, 4, 5, 7, 8])
+@pytest.mark.parametrize("chunk", [256, 512, 1024])
+def test_narrow_merge(rmat14, N, chunk):
+    """N ≤ 8 with the merge schedule: lanes walk runs of merge items (hub rows of the
+    power-law matrix are cut across lanes and waves); overwrite and accumulate."""
+    case(f"narrow N={N} chunk={chunk}")
+    a, (d_rp, d_ci, d_va) = rmat14
+    r = np.random.default_rng(100 + N)
+    B = r.uniform(-1, 1, (a.shape[1], N)).astype(np.float32)
+    C0 = r.uniform(-1, 1, (a.shape[0], N)).astype(np.float32)
+    d_B, d_C = dev(B, C0)
+    cfg = dict(schedule="merge", chunk=chunk)
+    scale = oracle.spmm_f64(a.rowptr, a.colind, np.abs(a.values), np.abs(B)) + 1e-30
+    out = kernels.spmm(d_rp, d_ci, d_va, d_B, config=cfg).cpu().numpy()
+    assert np.all(np.abs(out - oracle.spmm_f64(a.rowptr, a.colind, a.values, B)) <= TOL * scale)
+    kernels.spmm(d_rp, d_ci, d_va, d_B, out=d_C, accumulate=True, config=cfg)
+    ref = oracle.spmm_f64(a.rowptr, a.colind, a.values, B, C0)
+    assert np.all(np.abs(d_C.cpu().numpy() - ref) <= TOL * (scale + np.abs(C0)))
+
+
+@pytest.mark.parametrize("dtype", ["bf16", "fp8"])
+def test_narrow_merge_low_precision(rmat14, dtype):
+    a, (d_rp, d_ci, d_va) = rmat14
+    B = torch.from_numpy(np.random.default_rng(21).uniform(-1, 1, (a.shape[1], 4)).astype(np.float32)).cuda()
+    if dtype == "bf16":
+        Bq, sc = B.to(torch.bfloat16), None
+        deq = Bq.float().cpu().numpy()
+    else:
+        Bq, sc = kernels.quantize_fp8_rows(B)
+        deq = (Bq.cpu().view(torch.float8_e4m3fn).float() * sc.cpu()[:, None]).numpy()
+    out = kernels.spmm(d_rp, d_ci, d_va, Bq, dense_scale=sc, config="merge").cpu().numpy()
+    scale = oracle.spmm_f64(a.rowptr, a.colind, np.abs(a.values), np.abs(deq)) + 1e-30
+    assert np.all(np.abs(out - oracle.spmm_f64(a.rowptr, a.colind, a.values, deq)) <= TOL * scale)
+
+
 def test_value_less_and_empty():
     rp, ci, _ = random_csr(300, 200, 0.05, 9, empty_rows=0.3)
     B = np.random.default_rng(9).uniform(-1, 1, (200, 64)).astype(np.float32)
@@ -196,6 +231,21 @@ def test_fp8_quantiser_matches_torch():
     assert torch.equal(q, expect_q)
 
 
+def test_fp8_hardware_decode_matches_torch():
+    """All 256 e4m3 codes through the packed fp8 conversion the fp8 SpMM gathers use,
+    bit for bit against torch's float8_e4m3fn (NaN codes as NaN); and a scaled matrix."""
+    codes = torch.arange(256, dtype=torch.uint8).reshape(4, 64).cuda()
+    out = kernels.dequantize_fp8_rows(codes).cpu()
+    ref = codes.cpu().view(torch.float8_e4m3fn).float()
+    nan = torch.isnan(ref)
+    assert torch.equal(torch.isnan(out), nan)
+    assert torch.equal(out[~nan].view(torch.int32), ref[~nan].view(torch.int32))
+    B = torch.from_numpy(np.random.default_rng(4).standard_normal((300, 77)).astype(np.float32)).cuda()
+    q, sc = kernels.quantize_fp8_rows(B)
+    deq = kernels.dequantize_fp8_rows(q, sc).cpu()
+    assert torch.equal(deq, q.cpu().view(torch.float8_e4m3fn).float() * sc.cpu()[:, None])
+
+
 @pytest.mark.parametrize("sched", ["row", "merge", "merge-slice"])
 def test_fp8_dense(rmat14, sched):
     a, (d_rp, d_ci, d_va) = rmat14
@@ -227,7 +277,7 @@ def test_full_size_rmat_1m_n128():
     check_rows(sliced, rowwise, f64, "1M R-MAT merge, column slices")
 
 
-@pytest.mark.parametrize("sched", ["row", "merge", "nnz", "slice"])
+@pytest.mark.parametrize("sched", ["row", "merge", "nnz", "slice", "narrow"])
 @pytest.mark.parametrize("accumulate", [False, True])
 def test_epilogue_fusion(rmat14, sched, accumulate):
     """row_scale·(A·diag(col_scale)·B) + bias, ReLU — ReLU on rows cut across waves is applied
@@ -236,13 +286,15 @@ def test_epilogue_fusion(rmat14, sched, accumulate):
     a, (d_rp, d_ci, d_va) = rmat14
     M, K = a.shape
     r = np.random.default_rng(12)
-    B = r.uniform(-1, 1, (K, 64)).astype(np.float32)
+    W = 3 if sched == "narrow" else 64  # narrow: the N ≤ 8 merge kernel
+    B = r.uniform(-1, 1, (K, W)).astype(np.float32)
     cs = r.uniform(0.5, 1.5, K).astype(np.float32)
     rs = r.uniform(0.5, 1.5, M).astype(np.float32)
-    bias = r.uniform(-0.5, 0.5, 64).astype(np.float32)
-    C0 = r.uniform(-1, 1, (M, 64)).astype(np.float32)
+    bias = r.uniform(-0.5, 0.5, W).astype(np.float32)
+    C0 = r.uniform(-1, 1, (M, W)).astype(np.float32)
     d_B, d_cs, d_rs, d_bias, d_C = dev(B, cs, rs, bias, C0)
-    cfg = {"row": "row", "slice": dict(SLICE, chunk=256)}.get(sched, dict(schedule=sched, chunk=256))
+    cfg = {"row": "row", "slice": dict(SLICE, chunk=256), "narrow": dict(schedule="merge", chunk=256)}.get(
+        sched, dict(schedule=sched, chunk=256))
     out = kernels.spmm(d_rp, d_ci, d_va, d_B, out=d_C if accumulate else None, accumulate=accumulate,
                        col_scale=d_cs, row_scale=d_rs, bias=d_bias, relu=True, config=cfg).cpu().numpy()
     prod = oracle.spmm_f64(a.rowptr, a.colind, a.values * cs[a.colind], B)

@@ -1,10 +1,11 @@
-"""CPU model of the merge-path control flow of the LDS-staged SpMM kernels
-(k_spmm_stream: one nonzero per step; k_spmm_slice: a pair of nonzeros per
-step, lower and upper half-wave), step for step as the kernels walk a chunk:
-full batches with one row-end test per batch, then the tail.  Each chunk of
-the merge path must emit exactly the rows it completes, read only the row ends
-it staged, and leave the partial sum of its last row to the carry; the sum of
-emitted rows and carries must equal A·1 exactly (small-integer values)."""
+"""CPU model of the merge-path control flow of the SpMM kernels, step for step
+as they walk a chunk: k_spmm_stream (one nonzero per step) and k_spmm_slice (a
+pair of nonzeros per step, lower and upper half-wave) — full batches with one
+row-end test per batch, then the tail — and k_spmm_narrow (each lane walks a
+run of merge items; a segmented scan joins rows cut between lanes).  Each chunk
+must write exactly the rows it completes, read only the row ends it staged,
+and leave the partial sum of its last row to the carry; the sum of written rows
+and carries must equal A·1 exactly (small-integer values)."""
 import numpy as np
 import pytest
 
@@ -109,3 +110,73 @@ def test_model_catches_the_unbounded_tail(power_law):
     rp, vals = power_law
     with pytest.raises(AssertionError, match="carry row or past"):
         run_model(rp, vals, 64, 4, True, last_bound="old")
+
+
+def run_narrow_model(rp, vals, ipt):
+    """k_spmm_narrow: each of 64 lanes walks ipt items of the wave's merge tile;
+    rows that start and end in a lane are written by it, a lane's first finished
+    row also gets the partials of the lanes before it (segmented scan), and the
+    tile's last open row goes to the carry."""
+    M, nnz = len(rp) - 1, int(rp[-1])
+    tile = 64 * ipt
+    out = np.zeros(M, np.int64)
+    written = np.zeros(M, np.int64)
+    carries = {}
+    for d0 in range(0, M + nnz, tile):
+        d1 = min(d0 + tile, M + nnz)
+        i0, i1 = merge_search(rp, d0, M, nnz), merge_search(rp, d1, M, nnz)
+        j0, j1 = d0 - i0, d1 - i1
+        nz, nrows = j1 - j0, i1 - i0
+        re = [int(rp[i0 + k + 1]) - j0 for k in range(nrows)]
+        items = nrows + nz
+        keys, accs, firsts, starts, ends = [], [], [], [], []
+        for lane in range(64):
+            ld = min(lane * ipt, items)
+            lo, hi = max(0, ld - nz), min(ld, nrows)
+            while lo < hi:
+                mid = (lo + hi) // 2
+                if re[mid] <= ld - mid - 1:
+                    lo = mid + 1
+                else:
+                    hi = mid
+            ri, rj = lo, ld - lo
+            ri_start, acc, first = ri, 0, 0
+            for _ in range(ipt):
+                if ri + rj < items:
+                    if ri < nrows and re[ri] <= rj:
+                        if ri == ri_start:
+                            first = acc
+                        else:
+                            out[i0 + ri] += acc
+                            written[i0 + ri] += 1
+                        acc = 0
+                        ri += 1
+                    else:
+                        acc += int(vals[j0 + rj])
+                        rj += 1
+            keys.append(ri); accs.append(acc); firsts.append(first); starts.append(ri_start); ends.append(ri)
+        S = list(accs)  # segmented inclusive scan by key
+        for lane in range(1, 64):
+            if keys[lane - 1] == keys[lane]:
+                S[lane] += S[lane - 1]
+        for lane in range(64):
+            if ends[lane] > starts[lane]:
+                f = firsts[lane] + (S[lane - 1] if lane > 0 and keys[lane - 1] == starts[lane] else 0)
+                out[i0 + starts[lane]] += f
+                written[i0 + starts[lane]] += 1
+        if i1 < M and j1 > max(int(rp[i1]), j0):
+            assert keys[63] == nrows
+            carries.setdefault(i1, []).append(S[63])
+    for r, cs in carries.items():
+        out[r] += sum(cs)
+    return out, written
+
+
+@pytest.mark.parametrize("ipt", [4, 8, 16])
+def test_narrow_control_flow(power_law, ipt):
+    rp, vals = power_law
+    out, written = run_narrow_model(rp, vals, ipt)
+    expect = np.add.reduceat(np.append(vals, 0), rp[:-1])
+    expect[np.diff(rp) == 0] = 0
+    assert np.array_equal(out, expect)
+    assert np.all(written == 1)
