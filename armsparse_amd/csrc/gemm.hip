@@ -203,51 +203,138 @@ __global__ __launch_bounds__(256) void k_gemm_tall(int M, int N, int K, int KP, 
 // ------------------------------------------------------- reduction GEMM
 // C[M×N] = Σ_k A(m,k)·B(k,n) over a long K with M, N ≤ 256 (the weight
 // gradient xᵀ·dY): block b reduces k ∈ [b·ks, (b+1)·ks) into part[b] with the
-// whole M×N output in MFMA accumulators (TPW 32×32 tiles per wave); operand
-// slabs of 32 k are staged in LDS with the global reads running along the
-// contiguous dimension.  The partials are summed in block order afterwards.
+// whole M×N output in MFMA accumulators (TPW 32×32 tiles per wave).  Operand
+// slabs of 32 k go through LDS; each thread fetches its share of the next slab
+// as 8-byte vectors along the operand's contiguous dimension (m for A = xᵀ,
+// n for B = dY, or k) into registers while the current slab is multiplied,
+// so the loads of one slab are all in flight together and overlap the MFMAs.
+// The partials are summed in block order afterwards (deterministic).
+//
+// vector along the contiguous dimension: 4 bf16 (8 bytes)
+struct Slab {
+    int along;     // 0: vectors along the row dimension (m or n), 1: along k
+    int64_t srow;  // stride of the row dimension
+    int64_t sk;    // stride of k
+};
+
+// element (r, k) of a slab operand, 4 consecutive along the vector dimension
+__device__ __forceinline__ uint2 load4(const uint16_t* __restrict__ P, const Slab& sl, int r, int gk, int R, int ke) {
+    uint2 v = make_uint2(0u, 0u);
+    if (sl.along == 2) {  // any strides: four element loads along the row dimension
+        uint16_t t[4] = {0, 0, 0, 0};
+        if (gk < ke) {
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+                if (r + i < R) t[i] = P[(int64_t)(r + i) * sl.srow + (int64_t)gk * sl.sk];
+        }
+        return make_uint2(t[0] | ((uint32_t)t[1] << 16), t[2] | ((uint32_t)t[3] << 16));
+    }
+    if (sl.along == 0) {  // r..r+3 contiguous (srow == 1)
+        if (r < R && gk < ke) {
+            const uint16_t* p = P + r + (int64_t)gk * sl.sk;
+            if (r + 4 <= R) {
+                v = *reinterpret_cast<const uint2*>(p);
+            } else {
+                uint16_t t[4] = {0, 0, 0, 0};
+                for (int i = 0; i < 4 && r + i < R; ++i) t[i] = p[i];
+                v = make_uint2(t[0] | ((uint32_t)t[1] << 16), t[2] | ((uint32_t)t[3] << 16));
+            }
+        }
+    } else {  // k..k+3 contiguous (sk == 1)
+        if (r < R && gk < ke) {
+            const uint16_t* p = P + (int64_t)r * sl.srow + gk;
+            if (gk + 4 <= ke) {
+                v = *reinterpret_cast<const uint2*>(p);
+            } else {
+                uint16_t t[4] = {0, 0, 0, 0};
+                for (int i = 0; i < 4 && gk + i < ke; ++i) t[i] = p[i];
+                v = make_uint2(t[0] | ((uint32_t)t[1] << 16), t[2] | ((uint32_t)t[3] << 16));
+            }
+        }
+    }
+    return v;
+}
+
+// slab vector q of this thread: (r, kk) of its first element
+__device__ __forceinline__ void slab_pos(const Slab& sl, int v, int RP, int& r, int& kk) {
+    if (sl.along != 1) {  // RP/4 vectors per k
+        kk = v / (RP / 4);
+        r = (v - kk * (RP / 4)) * 4;
+    } else {  // TK/4 vectors per row
+        r = v / (TK / 4);
+        kk = (v - r * (TK / 4)) * 4;
+    }
+}
+
+__device__ __forceinline__ void store4(uint16_t* __restrict__ S, const Slab& sl, int r, int kk, uint2 v) {
+    const uint16_t e[4] = {(uint16_t)(v.x & 0xffffu), (uint16_t)(v.x >> 16), (uint16_t)(v.y & 0xffffu),
+                           (uint16_t)(v.y >> 16)};
+    if (sl.along != 1) {
+#pragma unroll
+        for (int i = 0; i < 4; ++i) S[(r + i) * TKP + kk] = e[i];
+    } else {
+        *reinterpret_cast<uint2*>(&S[r * TKP + kk]) = v;
+    }
+}
+
+constexpr int RW = 16;  // waves per reduction block: ≤ 2 accumulator tiles per wave for 256×256
+
 template <int TPW>
-__global__ __launch_bounds__(256) void k_gemm_reduce(int M, int N, int K, int ks, const uint16_t* __restrict__ A,
-                                                     int64_t sam, int64_t sak, const uint16_t* __restrict__ B,
-                                                     int64_t sbk, int64_t sbn, float* __restrict__ part) {
+__global__ __launch_bounds__(64 * RW) void k_gemm_reduce(int M, int N, int K, int ks, const uint16_t* __restrict__ A,
+                                                     Slab sa, const uint16_t* __restrict__ B, Slab sb,
+                                                     float* __restrict__ part) {
     extern __shared__ __attribute__((aligned(16))) uint16_t lds[];
     const int MT = (M + 31) / 32, NT = (N + 31) / 32;
     const int MP = 32 * MT, NP = 32 * NT;
-    uint16_t* sA = lds;              // [MP][TKP]
-    uint16_t* sB = lds + MP * TKP;   // [NP][TKP]
+    uint16_t* sA = lds;             // [MP][TKP]
+    uint16_t* sB = lds + MP * TKP;  // [NP][TKP]
     const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
     const int kb = blockIdx.x * ks, ke = min(K, kb + ks);
+    // vectors per slab: MP·TK/4 = 256·MT for A, 256·NT for B → ≤ 2 per thread each
+    constexpr int NT_ = 64 * RW;
+    uint2 ra[2], rb[2];
+    auto fetch = [&](int k0) {
+#pragma unroll
+        for (int q = 0; q < 2; ++q) {
+            const int v = tid + NT_ * q;
+            if (v < 256 * MT) {
+                int r, kk;
+                slab_pos(sa, v, MP, r, kk);
+                ra[q] = load4(A, sa, r, k0 + kk, M, ke);
+            }
+            if (v < 256 * NT) {
+                int r, kk;
+                slab_pos(sb, v, NP, r, kk);
+                rb[q] = load4(B, sb, r, k0 + kk, N, ke);
+            }
+        }
+    };
+    auto commit = [&]() {
+#pragma unroll
+        for (int q = 0; q < 2; ++q) {
+            const int v = tid + NT_ * q;
+            if (v < 256 * MT) {
+                int r, kk;
+                slab_pos(sa, v, MP, r, kk);
+                store4(sA, sa, r, kk, ra[q]);
+            }
+            if (v < 256 * NT) {
+                int r, kk;
+                slab_pos(sb, v, NP, r, kk);
+                store4(sB, sb, r, kk, rb[q]);
+            }
+        }
+    };
     f32x16 acc[TPW];
 #pragma unroll
     for (int j = 0; j < TPW; ++j)
 #pragma unroll
         for (int i = 0; i < 16; ++i) acc[j][i] = 0.0f;
+    if (kb < ke) fetch(kb);
     for (int k0 = kb; k0 < ke; k0 += TK) {
-        for (int idx = tid; idx < MP * TK; idx += 256) {
-            int m, kk;
-            if (sam == 1) {
-                kk = idx / MP;
-                m = idx - kk * MP;
-            } else {
-                m = idx / TK;
-                kk = idx - m * TK;
-            }
-            const int gk = k0 + kk;
-            sA[m * TKP + kk] = (m < M && gk < ke) ? A[(int64_t)m * sam + (int64_t)gk * sak] : (uint16_t)0;
-        }
-        for (int idx = tid; idx < NP * TK; idx += 256) {
-            int n, kk;
-            if (sbn == 1) {
-                kk = idx / NP;
-                n = idx - kk * NP;
-            } else {
-                n = idx / TK;
-                kk = idx - n * TK;
-            }
-            const int gk = k0 + kk;
-            sB[n * TKP + kk] = (n < N && gk < ke) ? B[(int64_t)gk * sbk + (int64_t)n * sbn] : (uint16_t)0;
-        }
+        commit();
         __syncthreads();
+        if (k0 + TK < ke) fetch(k0 + TK);  // next slab in flight during the MFMAs
 #pragma unroll
         for (int kq = 0; kq < TK; kq += 16) {
             const int koff = kq + (lane >> 5) * 8;
@@ -344,10 +431,23 @@ extern "C" int armsp_gemm_bf16(int M, int N, int Kd, const uint16_t* A, int64_t 
         }
     }
     // reduction: K ≫ M, N — each block's K slab into a partial, summed in block order
-    if (Kd >= 65536 && M <= 256 && N <= 256) {
+    // operand slabs as 8-byte vectors: along the row dimension when it is contiguous
+    // (A = xᵀ: sam == 1; B = dY: sbn == 1), else along k; 8-byte aligned
+    auto slab = [](const uint16_t* P, int64_t srow, int64_t sk, Slab& sl) {
+        const bool al = ((uintptr_t)P % 8) == 0;
+        if (al && srow == 1 && sk % 4 == 0)
+            sl = Slab{0, srow, sk};
+        else if (al && sk == 1 && srow % 4 == 0)
+            sl = Slab{1, srow, sk};
+        else
+            sl = Slab{2, srow, sk};  // element loads
+        return true;
+    };
+    Slab sa{}, sb{};
+    if (Kd >= 65536 && M <= 256 && N <= 256 && slab(A, sam, sak, sa) && slab(B, sbn, sbk, sb)) {
         const int MT = (M + 31) / 32, NT = (N + 31) / 32;
-        const int tpw = (MT * NT + 3) / 4;
-        if (tpw <= 8) {
+        const int tpw = (MT * NT + RW - 1) / RW;
+        if (tpw <= 4) {
             int nblk = 2 * device_cus();
             int ks = (Kd + nblk - 1) / nblk;
             ks = (ks + TK - 1) / TK * TK;
@@ -359,15 +459,13 @@ extern "C" int armsp_gemm_bf16(int M, int N, int Kd, const uint16_t* A, int64_t 
             float* part = static_cast<float*>(ws);
             TimedLaunch t("gemm", s);
 #define ARMSP_RED(TT)                                                                                              \
-    hipLaunchKernelGGL((k_gemm_reduce<TT>), dim3(nblk), dim3(256), lds, s, M, N, Kd, ks, A, sam, sak, B, sbk, sbn, part)
+    hipLaunchKernelGGL((k_gemm_reduce<TT>), dim3(nblk), dim3(64 * RW), lds, s, M, N, Kd, ks, A, sa, B, sb, part)
             if (tpw <= 1)
                 ARMSP_RED(1);
             else if (tpw <= 2)
                 ARMSP_RED(2);
-            else if (tpw <= 4)
-                ARMSP_RED(4);
             else
-                ARMSP_RED(8);
+                ARMSP_RED(4);
 #undef ARMSP_RED
             hipError_t e = hipGetLastError();
             if (e == hipSuccess) {
