@@ -100,6 +100,7 @@ _SIGS = {
     "armsp_spgemm_dense_f32": (c_int, [_I, _I, _I, _I, _P, _P, _P, _I, _P, _P, _P, _P, _I, _P]),
     "armsp_csr_transpose": (c_int, [_I, _I, _I, _P, _P, _P, _P, _P, _P, _P]),
     "armsp_gemm_bf16": (c_int, [_I, _I, _I, _P, c_int64, c_int64, _P, c_int64, c_int64, _P, c_int64, _I, _P]),
+    "armsp_gemm_bf16_ex": (c_int, [_I, _I, _I, _P, c_int64, c_int64, _P, c_int64, c_int64, _P, _I, c_int64, _I, _P]),
     "armsp_convert_f32_to_bf16": (c_int, [c_int64, _P, _P, _P]),
     "armsp_quantize_fp8_rows": (c_int, [_I, _I, _P, _I, _P, _P, _P]),
     "armsp_dequantize_fp8_rows": (c_int, [_I, _I, _P, _I, _P, _P, _I, _P]),

@@ -112,10 +112,10 @@ __global__ __launch_bounds__(256) void k_gemm_bf16(int M, int N, int K, int kchu
 // along k when A's rows allow (va = 8, 4 or 1 elements).
 constexpr int TK = 32, TKP = TK + 8;  // A stage: 32 k, row pitch 40 bf16
 
-template <int WR, int NTW>
+template <int WR, int NTW, bool OUTBF>
 __global__ __launch_bounds__(256) void k_gemm_tall(int M, int N, int K, int KP, const uint16_t* __restrict__ A,
                                                    int64_t sam, int64_t sak, const uint16_t* __restrict__ B,
-                                                   int64_t sbk, int64_t sbn, float* __restrict__ C, int64_t ldc,
+                                                   int64_t sbk, int64_t sbn, void* __restrict__ Cv, int64_t ldc,
                                                    int accumulate, int va) {
     constexpr int TR = 32 * WR;               // rows per tile
     constexpr int NP = 32 * NTW * (4 / WR);  // columns held
@@ -191,13 +191,25 @@ __global__ __launch_bounds__(256) void k_gemm_tall(int M, int N, int K, int KP, 
                 for (int i = 0; i < 16; ++i) {
                     const int gm = m0 + wr * 32 + 8 * (i >> 2) + 4 * (lane >> 5) + (i & 3);
                     if (gm < M) {
-                        float* p = C + (int64_t)gm * ldc + gn;
-                        *p = accumulate ? *p + acc[j][i] : acc[j][i];
+                        if constexpr (OUTBF) {  // bf16 output (round to nearest even), overwrite
+                            static_cast<uint16_t*>(Cv)[(int64_t)gm * ldc + gn] = f32_to_bf16_rne(acc[j][i]);
+                        } else {
+                            float* p = static_cast<float*>(Cv) + (int64_t)gm * ldc + gn;
+                            *p = accumulate ? *p + acc[j][i] : acc[j][i];
+                        }
                     }
                 }
             }
         }
     }
+}
+
+// fp32 [M×N] (pitch N) → bf16 rows of pitch ldc
+__global__ void k_rows_to_bf16(int M, int N, const float* __restrict__ X, uint16_t* __restrict__ Y, int64_t ldc) {
+    const int64_t idx = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (idx >= (int64_t)M * N) return;
+    const int m = (int)(idx / N), n = (int)(idx % N);
+    Y[(int64_t)m * ldc + n] = f32_to_bf16_rne(X[idx]);
 }
 
 // ------------------------------------------------------- reduction GEMM
@@ -383,54 +395,113 @@ __global__ void k_splitk_reduce(int M, int N, int splits, const float* __restric
 }  // namespace
 }  // namespace armsp
 
+namespace armsp {
+namespace {
+int gemm_f32_out(int M, int N, int Kd, const uint16_t* A, int64_t sam, int64_t sak, const uint16_t* B, int64_t sbk,
+                 int64_t sbn, float* C, int64_t ldc, int accumulate, hipStream_t s);
+int gemm_tall(int M, int N, int Kd, const uint16_t* A, int64_t sam, int64_t sak, const uint16_t* B, int64_t sbk,
+              int64_t sbn, void* C, bool outbf, int64_t ldc, int accumulate, hipStream_t s, bool& done);
+}  // namespace
+}  // namespace armsp
+
 extern "C" int armsp_gemm_bf16(int M, int N, int Kd, const uint16_t* A, int64_t sam, int64_t sak, const uint16_t* B,
                                int64_t sbk, int64_t sbn, float* C, int64_t ldc, int accumulate, void* stream) {
     using namespace armsp;
     if (M < 0 || N < 0 || Kd < 0 || ldc < N) return ARMSP_ERROR_INVALID_VALUE;
     if (M == 0 || N == 0) return ARMSP_SUCCESS;
     if (!A || !B || !C) return ARMSP_ERROR_INVALID_VALUE;
+    return gemm_f32_out(M, N, Kd, A, sam, sak, B, sbk, sbn, C, ldc, accumulate, as_stream(stream));
+}
+
+extern "C" int armsp_gemm_bf16_ex(int M, int N, int Kd, const uint16_t* A, int64_t sam, int64_t sak,
+                                  const uint16_t* B, int64_t sbk, int64_t sbn, void* C, int c_dtype, int64_t ldc,
+                                  int accumulate, void* stream) {
+    using namespace armsp;
+    if (c_dtype == ARMSP_DTYPE_F32)
+        return armsp_gemm_bf16(M, N, Kd, A, sam, sak, B, sbk, sbn, static_cast<float*>(C), ldc, accumulate, stream);
+    if (c_dtype != ARMSP_DTYPE_BF16 || accumulate) return ARMSP_ERROR_INVALID_VALUE;
+    if (M < 0 || N < 0 || Kd < 0 || ldc < N) return ARMSP_ERROR_INVALID_VALUE;
+    if (M == 0 || N == 0) return ARMSP_SUCCESS;
+    if (!A || !B || !C) return ARMSP_ERROR_INVALID_VALUE;
     hipStream_t s = as_stream(stream);
-    // tall: M ≫ K, N — B resident in LDS, A streamed once
-    if (M >= 4096 && N <= 256 && Kd <= 256 && Kd > 0) {
-        const int KP = (Kd + TK - 1) / TK * TK;
-        int WR = 2, NTW = 4;
-        if (N <= 32) {
-            WR = 4;
-            NTW = 1;
-        } else if (N <= 64) {
-            NTW = 1;
-        } else if (N <= 128) {
-            NTW = 2;
-        }
-        const int NP = 32 * NTW * (4 / WR);
-        const size_t lds = ((size_t)NP * (KP + 8) + (size_t)32 * WR * TKP) * sizeof(uint16_t);
-        if (lds <= 96 * 1024) {
-            int va = 1;
-            if (sak == 1 && sam % 8 == 0 && ((uintptr_t)A % 16) == 0)
-                va = 8;
-            else if (sak == 1 && sam % 4 == 0 && ((uintptr_t)A % 8) == 0)
-                va = 4;
-            const int ntiles = (M + 32 * WR - 1) / (32 * WR);
-            const int nblk = ntiles < 2 * device_cus() ? ntiles : 2 * device_cus();
-            TimedLaunch t("gemm", s);
-#define ARMSP_TALL(WW, NN)                                                                                     \
-    hipLaunchKernelGGL((k_gemm_tall<WW, NN>), dim3(nblk), dim3(256), lds, s, M, N, Kd, KP, A, sam, sak, B, sbk, sbn, C, \
-                       (int64_t)ldc, accumulate, va)
-            if (WR == 4)
-                ARMSP_TALL(4, 1);
-            else if (NTW == 1)
-                ARMSP_TALL(2, 1);
-            else if (NTW == 2)
-                ARMSP_TALL(2, 2);
-            else
-                ARMSP_TALL(2, 4);
-#undef ARMSP_TALL
-            ARMSP_LAUNCHED();
-            t.stop();
-            return ARMSP_SUCCESS;
-        }
+    bool done = false;
+    int st = gemm_tall(M, N, Kd, A, sam, sak, B, sbk, sbn, C, true, ldc, 0, s, done);
+    if (st || done) return st;
+    // other shapes: fp32 into a workspace, then rounded into C
+    void* ws = nullptr;
+    st = ws_alloc(&ws, (size_t)M * N * sizeof(float), s);
+    if (st) return st;
+    st = gemm_f32_out(M, N, Kd, A, sam, sak, B, sbk, sbn, static_cast<float*>(ws), N, 0, s);
+    if (!st) {
+        const int64_t n = (int64_t)M * N;
+        hipLaunchKernelGGL(k_rows_to_bf16, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, M, N,
+                           static_cast<const float*>(ws), static_cast<uint16_t*>(C), ldc);
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) st = hip_fail(e);
     }
-    // reduction: K ≫ M, N — each block's K slab into a partial, summed in block order
+    ws_free(ws, s);
+    return st;
+}
+
+namespace armsp {
+namespace {
+// tall: M ≫ K, N — B resident in LDS, A streamed once; done = false when the shape does not fit
+int gemm_tall(int M, int N, int Kd, const uint16_t* A, int64_t sam, int64_t sak, const uint16_t* B, int64_t sbk,
+              int64_t sbn, void* C, bool outbf, int64_t ldc, int accumulate, hipStream_t s, bool& done) {
+    done = false;
+    if (!(M >= 4096 && N <= 256 && Kd <= 256 && Kd > 0)) return ARMSP_SUCCESS;
+    const int KP = (Kd + TK - 1) / TK * TK;
+    int WR = 2, NTW = 4;
+    if (N <= 32) {
+        WR = 4;
+        NTW = 1;
+    } else if (N <= 64) {
+        NTW = 1;
+    } else if (N <= 128) {
+        NTW = 2;
+    }
+    const int NP = 32 * NTW * (4 / WR);
+    const size_t lds = ((size_t)NP * (KP + 8) + (size_t)32 * WR * TKP) * sizeof(uint16_t);
+    if (lds > 96 * 1024) return ARMSP_SUCCESS;
+    int va = 1;
+    if (sak == 1 && sam % 8 == 0 && ((uintptr_t)A % 16) == 0)
+        va = 8;
+    else if (sak == 1 && sam % 4 == 0 && ((uintptr_t)A % 8) == 0)
+        va = 4;
+    const int ntiles = (M + 32 * WR - 1) / (32 * WR);
+    const int nblk = ntiles < 2 * device_cus() ? ntiles : 2 * device_cus();
+    TimedLaunch t("gemm", s);
+#define ARMSP_TALL(WW, NN)                                                                                         \
+    do {                                                                                                           \
+        if (outbf)                                                                                                 \
+            hipLaunchKernelGGL((k_gemm_tall<WW, NN, true>), dim3(nblk), dim3(256), lds, s, M, N, Kd, KP, A, sam, sak, \
+                               B, sbk, sbn, C, ldc, accumulate, va);                                               \
+        else                                                                                                       \
+            hipLaunchKernelGGL((k_gemm_tall<WW, NN, false>), dim3(nblk), dim3(256), lds, s, M, N, Kd, KP, A, sam,  \
+                               sak, B, sbk, sbn, C, ldc, accumulate, va);                                          \
+    } while (0)
+    if (WR == 4)
+        ARMSP_TALL(4, 1);
+    else if (NTW == 1)
+        ARMSP_TALL(2, 1);
+    else if (NTW == 2)
+        ARMSP_TALL(2, 2);
+    else
+        ARMSP_TALL(2, 4);
+#undef ARMSP_TALL
+    ARMSP_LAUNCHED();
+    t.stop();
+    done = true;
+    return ARMSP_SUCCESS;
+}
+
+int gemm_f32_out(int M, int N, int Kd, const uint16_t* A, int64_t sam, int64_t sak, const uint16_t* B, int64_t sbk,
+                 int64_t sbn, float* C, int64_t ldc, int accumulate, hipStream_t s) {
+    {
+        bool done = false;
+        const int st = gemm_tall(M, N, Kd, A, sam, sak, B, sbk, sbn, C, false, ldc, accumulate, s, done);
+        if (st || done) return st;
+    }
     // operand slabs as 8-byte vectors: along the row dimension when it is contiguous
     // (A = xᵀ: sam == 1; B = dY: sbn == 1), else along k; 8-byte aligned
     auto slab = [](const uint16_t* P, int64_t srow, int64_t sk, Slab& sl) {
@@ -516,3 +587,5 @@ extern "C" int armsp_gemm_bf16(int M, int N, int Kd, const uint16_t* A, int64_t 
     t.stop();
     return ARMSP_SUCCESS;
 }
+}  // namespace
+}  // namespace armsp

@@ -280,20 +280,25 @@ def csr_transpose(rowptr: torch.Tensor, colind: torch.Tensor, values: Optional[t
 
 # ------------------------------------------------------- dense helpers
 def gemm_bf16(a: torch.Tensor, b: torch.Tensor, *, out: Optional[torch.Tensor] = None,
-              accumulate: bool = False) -> torch.Tensor:
-    """out (=|+=) a @ b for 2-D bfloat16 tensors of any strides, fp32 result (MFMA)."""
+              accumulate: bool = False, out_dtype: torch.dtype = torch.float32) -> torch.Tensor:
+    """out (=|+=) a @ b for 2-D bfloat16 tensors of any strides on MFMA with fp32
+    accumulation; the result is float32, or bfloat16 (rounded) with out_dtype."""
     fn = "gemm_bf16"
     _require(a.dim() == 2 and b.dim() == 2 and a.shape[1] == b.shape[0], f"{fn}: shape mismatch")
     _require(a.dtype == torch.bfloat16 and b.dtype == torch.bfloat16, f"{fn}: operands must be bfloat16")
     _require(a.is_cuda and b.is_cuda, f"{fn}: operands must be device tensors")
+    _require(out_dtype in (torch.float32, torch.bfloat16), f"{fn}: out_dtype must be float32 or bfloat16")
     m, kd = a.shape
     n = b.shape[1]
     if out is None:
         _require(not accumulate, f"{fn}: accumulate needs an output tensor")
-        out = torch.empty((m, n), dtype=torch.float32, device=a.device)
-    _require(out.dtype == torch.float32 and out.shape == (m, n) and out.stride(1) == 1, f"{fn}: bad out")
-    call("armsp_gemm_bf16", m, n, kd, _ptr(a), a.stride(0), a.stride(1), _ptr(b), b.stride(0), b.stride(1),
-         _ptr(out), out.stride(0), 1 if accumulate else 0, _stream(a.device))
+        out = torch.empty((m, n), dtype=out_dtype, device=a.device)
+    _require(out.dtype == out_dtype and out.shape == (m, n) and out.stride(1) == 1, f"{fn}: bad out")
+    _require(not (accumulate and out_dtype == torch.bfloat16), f"{fn}: accumulate needs a float32 output")
+    call("armsp_gemm_bf16_ex", m, n, kd, _ptr(a), a.stride(0), a.stride(1), _ptr(b), b.stride(0), b.stride(1),
+         _ptr(out), _lib.DTYPE_BF16 if out_dtype == torch.bfloat16 else _lib.DTYPE_F32, out.stride(0),
+         1 if accumulate else 0,
+         _stream(a.device))
     return out
 
 

@@ -104,24 +104,29 @@ def gcn_edge_coefficients(colind, rowind, w_csr, w_csc, in_norm, out_norm):
 
 
 class _BF16Linear(torch.autograd.Function):
-    """y = x·W with x bfloat16 [M×F] and W float32 master weights [F×H]; MFMA GEMM, fp32 y."""
+    """y = x·W on the MFMA GEMM with W float32 master weights [F×H] and x [M×F]
+    in bfloat16, or float32 (rounded to bfloat16 here, and given a float32
+    gradient back, so no conversion round trip sits at the autograd boundary).
+    y is bfloat16 when out_bf16 (written so by the GEMM), else float32."""
 
     @staticmethod
-    def forward(ctx, x, weight):
+    def forward(ctx, x, weight, out_bf16=False):
+        xb = x if x.dtype == torch.bfloat16 else kernels.to_bf16(x.contiguous())
         wb = kernels.to_bf16(weight.detach())
-        ctx.save_for_backward(x, wb)
-        return kernels.gemm_bf16(x, wb)
+        ctx.save_for_backward(xb, wb)
+        ctx.x_dtype = x.dtype
+        return kernels.gemm_bf16(xb, wb, out_dtype=torch.bfloat16 if out_bf16 else torch.float32)
 
     @staticmethod
     def backward(ctx, gy):
-        x, wb = ctx.saved_tensors
-        gyb = kernels.to_bf16(gy.contiguous())
+        xb, wb = ctx.saved_tensors
+        gyb = gy.contiguous() if gy.dtype == torch.bfloat16 else kernels.to_bf16(gy.contiguous())
         gx = gw = None
         if ctx.needs_input_grad[0]:
-            gx = kernels.gemm_bf16(gyb, wb.t()).to(x.dtype)  # [M×H]·[H×F]
+            gx = kernels.gemm_bf16(gyb, wb.t(), out_dtype=ctx.x_dtype)  # [M×H]·[H×F]
         if ctx.needs_input_grad[1]:
-            gw = kernels.gemm_bf16(x.t(), gyb)  # [F×M]·[M×H]
-        return gx, gw
+            gw = kernels.gemm_bf16(xb.t(), gyb)  # [F×M]·[M×H]
+        return gx, gw, None
 
 
 class GCNConv(torch.nn.Module):
@@ -167,8 +172,10 @@ class GCNConv(torch.nn.Module):
 
     def forward(self, x, rowptr, colind, colptr, rowind, edge_weight_csr=None, edge_weight_csc=None):
         if self.precision == "bf16":
-            xb = x if x.dtype == torch.bfloat16 else x.to(torch.bfloat16)
-            x = _BF16Linear.apply(xb.contiguous(), self.weight)
+            # bf16 features for the aggregation straight from the GEMM (fused path with
+            # more than 8 output channels); fp32 otherwise
+            out_bf16 = self.fused and self.out_channels > 8
+            x = _BF16Linear.apply(x.contiguous(), self.weight, out_bf16)
         else:
             x = torch.matmul(x, self.weight)
         if not self.cached or self.cached_result is None:
@@ -190,7 +197,7 @@ class GCNConv(torch.nn.Module):
                     self.cached_coef = coefs
             # bf16 features halve the gather bytes; a handful of output channels gather
             # next to nothing, so those stay fp32 (no conversions either way)
-            h = x.to(torch.bfloat16) if self.precision == "bf16" and self.out_channels > 8 else x
+            h = x
             return GCNAggregate.apply(h.contiguous(), rowptr, colind, colptr, rowind, coefs[0], coefs[1], in_n,
                                       out_n, self.bias)
         if self.normalize:

@@ -204,6 +204,13 @@ int armsp_gemm_bf16(int M, int N, int Kd,
                     const uint16_t* A, int64_t sam, int64_t sak,
                     const uint16_t* B, int64_t sbk, int64_t sbn,
                     float* C, int64_t ldc, int accumulate, void* stream);
+/* The same product written as C of type c_dtype: ARMSP_DTYPE_F32 (as
+ * armsp_gemm_bf16) or ARMSP_DTYPE_BF16 (rounded to nearest even; accumulate
+ * must be 0), so a bf16 layer output needs no separate conversion pass. */
+int armsp_gemm_bf16_ex(int M, int N, int Kd,
+                       const uint16_t* A, int64_t sam, int64_t sak,
+                       const uint16_t* B, int64_t sbk, int64_t sbn,
+                       void* C, int c_dtype, int64_t ldc, int accumulate, void* stream);
 /* fp32 → bf16 (round to nearest even), n elements. */
 int armsp_convert_f32_to_bf16(int64_t n, const float* x, uint16_t* y, void* stream);
 /* Per-row fp8-e4m3 quantisation: scale[k] = max|B[k,:]| / 448 (1 if the row

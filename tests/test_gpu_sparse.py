@@ -207,6 +207,18 @@ def test_gemm_bf16_tall_and_reduction(m, n, k, layout):
     assert bool(((c.cpu().double() - ref - 2).abs() <= TOL * (bound + 2) + 1e-30).all())
 
 
+@pytest.mark.parametrize("m,n,k", [(20000, 256, 100), (20000, 3, 256), (37, 70, 50), (100, 256, 70000)])
+def test_gemm_bf16_output(m, n, k):
+    """bf16 output = the fp32 product rounded to nearest even (tall kernel writes it
+    directly, other shapes round a workspace)."""
+    r = np.random.default_rng(m + n + k + 1)
+    a = torch.from_numpy(r.standard_normal((m, k)).astype(np.float32)).to(torch.bfloat16).cuda()
+    b = torch.from_numpy(r.standard_normal((k, n)).astype(np.float32)).to(torch.bfloat16).cuda()
+    ref = kernels.gemm_bf16(a, b).to(torch.bfloat16)
+    out = kernels.gemm_bf16(a, b, out_dtype=torch.bfloat16)
+    assert out.dtype == torch.bfloat16 and torch.equal(out, ref)
+
+
 # ----------------------------------------------------------------- fills
 def test_device_uniform_fill_matches_host():
     t = torch.empty(100_003, device="cuda")
