@@ -514,14 +514,14 @@ __global__ __launch_bounds__(256) void k_spmm_stream(int M, int N, int nnz, int 
 // slice (when the slice count divides 8): each XCD's L2 then holds one line
 // per B row instead of all of them, i.e. several times more distinct rows of
 // the hot set.  The numbering only affects speed, never results.
-template <typename TB, int U>
+template <typename TB, int V, int U>
 __global__ __launch_bounds__(256) void k_spmm_slice(int M, int N, int nnz, int chunk, int S, int G,
                                                     const int* __restrict__ rowptr, const int* __restrict__ colind,
                                                     const float* __restrict__ val, const TB* __restrict__ B, int ldb,
                                                     Epi ep, float* __restrict__ C, int ldc, int flags,
                                                     int* __restrict__ carry_row, float* __restrict__ carry_val) {
-    constexpr int V = 4 / (int)sizeof(TB);  // columns per lane
-    constexpr int SW = 32 * V;              // columns per slice
+    // V columns per lane (4, 8 or 16 bytes); a slice is the 32·V columns of a half wave
+    constexpr int SW = 32 * V;
     extern __shared__ __align__(16) unsigned char smem[];
     const int lane = threadIdx.x & 63;
     const int wid = threadIdx.x >> 6;
@@ -567,7 +567,7 @@ __global__ __launch_bounds__(256) void k_spmm_slice(int M, int N, int nnz, int c
     const bool active = col < N;
     const __amdgpu_buffer_rsrc_t rs =
         __builtin_amdgcn_make_buffer_rsrc(const_cast<TB*>(B + sl * SW), (short)0, (int)0xffffffff, 0x00020000);
-    const uint32_t voff = active ? (uint32_t)(l * 4) : 0u;
+    const uint32_t voff = active ? (uint32_t)(l * V * (int)sizeof(TB)) : 0u;
 
     float acc[V];
 #pragma unroll
@@ -990,19 +990,19 @@ int launch_stream_vu(int U, dim3 grid, hipStream_t s, int M, int N, int nnz, int
     return ARMSP_SUCCESS;
 }
 
-template <typename TB>
-int launch_slice(int U, int S, int G, hipStream_t s, int M, int N, int nnz, int chunk, const int* rp, const int* ci,
-                 const float* va, const TB* B, int ldb, const Epi& bs, float* C, int ldc, int flags, int* crow,
-                 float* cval) {
+template <typename TB, int V>
+int launch_slice_v(int U, int S, int G, hipStream_t s, int M, int N, int nnz, int chunk, const int* rp,
+                   const int* ci, const float* va, const TB* B, int ldb, const Epi& bs, float* C, int ldc, int flags,
+                   int* crow, float* cval) {
     const size_t lds = stream_lds_bytes(chunk);
     const int64_t nblk = (8 % S == 0) ? ((int64_t)(G + 8 / S - 1) / (8 / S)) * 8 : (int64_t)G * S;
     if (nblk > INT_MAX) return ARMSP_ERROR_NOT_SUPPORTED;
     const dim3 grid((unsigned)nblk);
     switch (U) {
-#define ARMSP_SLICE_CASE(UU)                                                                                  \
-    case UU:                                                                                                  \
-        hipLaunchKernelGGL((k_spmm_slice<TB, UU>), grid, dim3(256), lds, s, M, N, nnz, chunk, S, G, rp, ci, va, B, \
-                           ldb, bs, C, ldc, flags, crow, cval);                                               \
+#define ARMSP_SLICE_CASE(UU)                                                                                     \
+    case UU:                                                                                                     \
+        hipLaunchKernelGGL((k_spmm_slice<TB, V, UU>), grid, dim3(256), lds, s, M, N, nnz, chunk, S, G, rp, ci, va, B, \
+                           ldb, bs, C, ldc, flags, crow, cval);                                                  \
         break;
         ARMSP_SLICE_CASE(4)
         ARMSP_SLICE_CASE(8)
@@ -1011,6 +1011,25 @@ int launch_slice(int U, int S, int G, hipStream_t s, int M, int N, int nnz, int 
         default: return ARMSP_ERROR_INVALID_VALUE;
     }
     return ARMSP_SUCCESS;
+}
+
+// lane width sv columns: 4, 8 or 16 bytes per lane
+template <typename TB>
+int launch_slice(int sv, int U, int S, int G, hipStream_t s, int M, int N, int nnz, int chunk, const int* rp,
+                 const int* ci, const float* va, const TB* B, int ldb, const Epi& bs, float* C, int ldc, int flags,
+                 int* crow, float* cval) {
+    switch (sv * (int)sizeof(TB)) {
+        case 4:
+            return launch_slice_v<TB, 4 / sizeof(TB)>(U, S, G, s, M, N, nnz, chunk, rp, ci, va, B, ldb, bs, C, ldc,
+                                                      flags, crow, cval);
+        case 8:
+            return launch_slice_v<TB, 8 / sizeof(TB)>(U, S, G, s, M, N, nnz, chunk, rp, ci, va, B, ldb, bs, C, ldc,
+                                                      flags, crow, cval);
+        case 16:
+            return launch_slice_v<TB, 16 / sizeof(TB)>(U, S, G, s, M, N, nnz, chunk, rp, ci, va, B, ldb, bs, C, ldc,
+                                                       flags, crow, cval);
+        default: return ARMSP_ERROR_INVALID_VALUE;
+    }
 }
 
 template <typename TB, int NV>
@@ -1119,6 +1138,11 @@ int spmm_impl(int M, int K, int N, int nnz, const int* rowptr, const int* colind
     // N ≤ 8 (group 0): lanes walk runs of 4, 8 or 16 merge items (k_spmm_narrow); the
     // wave tile is 64 × that run, so the chunk is rounded to 256, 512 or 1024
     const bool narrow = merge && !wide && N <= 8 && c.group == 0;
+    // AUTO half-wave slices (see below) run best with 256 items per wave and 4 gathers per batch
+    if (merge && !wide && !narrow && c.group == 0 && N * (int)sizeof(TB) >= 512) {
+        if (c.chunk <= 0) chunk = 256;
+        if (c.unroll <= 0) U = 4;
+    }
     if (narrow) {  // default: runs of 8 items for N ≤ 2, of 4 above (the staged products fill the LDS)
         if (c.chunk <= 0) chunk = N <= 2 ? 512 : 256;
         chunk = chunk <= 256 ? 256 : (chunk <= 512 ? 512 : 1024);
@@ -1136,10 +1160,21 @@ int spmm_impl(int M, int K, int N, int nnz, const int* rowptr, const int* colind
     int* crow = static_cast<int*>(ws);
     float* cval = reinterpret_cast<float*>(static_cast<char*>(ws) + row_bytes);
 
-    // group 32: half a wave per nonzero over 128-byte column slices of B (k_spmm_slice)
-    constexpr int SV = 4 / (int)sizeof(TB);
-    const bool slice = merge && !wide && c.group == 32 && chunk <= 2048 && N % SV == 0 && ldb % SV == 0 &&
-                       ldc % SV == 0 && aligned(B, 4) && aligned(C, 4 * SV);
+    // group 32: half a wave per nonzero over column slices of B (k_spmm_slice), each lane
+    // 4 bytes of B (vec 0) or vec columns of 4, 8 or 16 bytes.
+    // AUTO (group 0): when a row of B spans two or more 256-byte slices, half a wave per
+    // nonzero with 8-byte lanes, the slices spread over the XCDs — on power-law matrices
+    // each XCD's L2 then holds half (or less) of every hot row (measured: 18–25% faster
+    // on the 1M R-MAT at N=128/256 fp32, 8% at N=256 bf16, within 6% elsewhere)
+    const int esz = (int)sizeof(TB);
+    const bool auto_slice = merge && !wide && !narrow && c.group == 0 && chunk <= 2048 && N * esz >= 512 &&
+                            N % (8 / esz) == 0 &&
+                            ldb % (8 / esz) == 0 && ldc % (8 / esz) == 0 && aligned(B, 8) && aligned(C, 4 * (8 / esz));
+    const int SV = auto_slice ? 8 / esz : (c.vec > 0 ? c.vec : 4 / esz);
+    const int slb = SV * esz;
+    const bool slice = auto_slice ||
+                       (merge && !wide && c.group == 32 && chunk <= 2048 && (slb == 4 || slb == 8 || slb == 16) &&
+                        N % SV == 0 && ldb % SV == 0 && ldc % SV == 0 && aligned(B, slb) && aligned(C, 4 * SV));
     if (c.group == 32 && !slice) {
         ws_free(ws, s);
         return ARMSP_ERROR_NOT_SUPPORTED;
@@ -1158,8 +1193,8 @@ int spmm_impl(int M, int K, int N, int nnz, const int* rowptr, const int* colind
     } else if (slice) {
         TimedLaunch t("spmm", s);
         const int S = (N + 32 * SV - 1) / (32 * SV);
-        st = launch_slice<TB>(U, S, (int)((nwaves + 3) / 4), s, M, N, nnz, chunk, rowptr, colind, val, B, ldb, bscale,
-                              C, ldc, flags, crow, cval);
+        st = launch_slice<TB>(SV, U, S, (int)((nwaves + 3) / 4), s, M, N, nnz, chunk, rowptr, colind, val, B, ldb,
+                              bscale, C, ldc, flags, crow, cval);
         hipError_t e = st ? hipSuccess : hipGetLastError();
         if (st || e != hipSuccess) {
             ws_free(ws, s);

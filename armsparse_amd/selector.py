@@ -44,12 +44,14 @@ WEIGHTS_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "selecto
 SPMM_CONFIGS = [None,
                 dict(schedule="row"), dict(schedule="row", group=16),
                 dict(schedule="nnz", chunk=256, unroll=8),
-                dict(schedule="merge", chunk=128, unroll=16), dict(schedule="merge", chunk=256, unroll=4),
-                dict(schedule="merge", chunk=256, unroll=16), dict(schedule="merge", chunk=512, unroll=8),
+                dict(schedule="merge", group=64, chunk=128, unroll=16),
+                dict(schedule="merge", group=64, chunk=256, unroll=4),
+                dict(schedule="merge", group=64, chunk=256, unroll=16),
+                dict(schedule="merge", group=64, chunk=512, unroll=8),
                 dict(schedule="merge", chunk=1024, unroll=8),
-                dict(schedule="merge", group=32, chunk=256, unroll=4),
                 dict(schedule="merge", group=32, chunk=512, unroll=8),
-                dict(schedule="merge", group=64, chunk=256, unroll=8)]
+                dict(schedule="merge", group=32, vec=2, chunk=256, unroll=4),
+                dict(schedule="merge", group=32, vec=2, chunk=512, unroll=8)]
 SDDMM_CONFIGS = ([None] + [dict(schedule="merge", group=g, chunk=c) for g in (4, 8, 16) for c in (128, 256, 512)]
                  + [dict(schedule="nnz", group=g, chunk=256) for g in (8, 16)])
 CONFIGS = {"spmm": SPMM_CONFIGS, "sddmm": SDDMM_CONFIGS}
@@ -283,11 +285,22 @@ def autotune(prob: Problem, op: str, n: int, configs=None) -> np.ndarray:
 
 
 # ----------------------------------------------------------------- corpus
-def corpus_specs(n: int, seed: int = 2024, nnz_cap: int = 1 << 22) -> list:
+def corpus_specs(n: int, seed: int = 2024, nnz_cap: int = 1 << 22, big_every: int = 20) -> list:
+    """Seeded corpus; every big_every-th matrix is graph-scale (2^18–2^20 rows, up
+    to 2^24 nonzeros, uniform or R-MAT) so the regime where B far exceeds the L2
+    is represented."""
     r = np.random.default_rng(seed)
     kinds = ["uniform", "rmat", "banded", "blockdiag"]
     specs = []
     for i in range(n):
+        if big_every and i % big_every == big_every - 1:
+            kind = ["uniform", "rmat"][(i // big_every) % 2]
+            m = 1 << int(r.integers(18, 21))
+            avg = float(np.exp(r.uniform(math.log(4), math.log(32))))
+            avg = min(avg, (1 << 24) / m)
+            param = float(r.uniform(0.45, 0.75)) if kind == "rmat" else 0.0
+            specs.append(dict(kind=kind, m=m, avg=avg, param=param, seed=seed * 100003 + i))
+            continue
         kind = kinds[i % 4]
         m = 1 << int(r.integers(12, 22))
         avg = float(np.exp(r.uniform(math.log(2), math.log(256))))

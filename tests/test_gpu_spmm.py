@@ -97,12 +97,15 @@ def test_reference_tail_semantics():
         assert np.array_equal(bits(d_C), bits(oracle.irregular(rp, ci, va, B, C0.copy())))
 
 
-# MERGE with chunk ≤ 2048 runs the LDS-staged 32-bit-offset kernel; group=1 or a
-# larger chunk runs the 64-bit-address kernel that NNZ always uses
+# MERGE with chunk ≤ 2048 runs the LDS-staged 32-bit-offset kernels — at N=128 fp32
+# AUTO (group 0) is the half-wave 256-byte-slice kernel, group=64 the whole-wave one;
+# group=1 or a larger chunk runs the 64-bit-address kernel that NNZ always uses
 SPLIT_CFGS = ([dict(schedule=s, chunk=c, unroll=u) for s in ("merge", "nnz") for c in (64, 512, 2048)
                for u in (4, 8, 16)]
+              + [dict(schedule="merge", chunk=c, unroll=u, group=64) for c in (64, 512, 2048) for u in (4, 8, 16)]
               + [dict(schedule="merge", chunk=c, unroll=u, group=1) for c in (64, 512) for u in (4, 8, 16)]
               + [dict(schedule="merge", chunk=c, unroll=u, group=32) for c in (64, 512, 2048) for u in (4, 8, 16)]
+              + [dict(schedule="merge", chunk=c, unroll=u, group=32, vec=4) for c in (64, 512) for u in (4, 8, 16)]
               + [dict(schedule="merge", chunk=4096, unroll=8)])
 
 SLICE = dict(schedule="merge", group=32)  # half a wave per nonzero over 128-byte column slices
@@ -116,7 +119,8 @@ def rmat14():
 
 @pytest.mark.parametrize("cfg", SPLIT_CFGS,
                          ids=lambda c: f"{c['schedule']}-c{c['chunk']}-u{c['unroll']}"
-                         + {1: "-wide", 32: "-slice"}.get(c.get("group"), ""))
+                         + {1: "-wide", 32: "-slice", 64: "-wave"}.get(c.get("group"), "")
+                         + (f"-v{c['vec']}" if "vec" in c else ""))
 def test_split_schedules_power_law(rmat14, cfg):
     case(f"split {cfg}")
     a, (d_rp, d_ci, d_va) = rmat14
@@ -136,12 +140,15 @@ def test_split_schedules_power_law(rmat14, cfg):
 
 
 @pytest.mark.parametrize("N", [1, 16, 32, 64, 96, 256, 512, 520])
-@pytest.mark.parametrize("sched", ["merge", "nnz", "merge-wide", "merge-slice"])
+@pytest.mark.parametrize("sched", ["merge", "nnz", "merge-wide", "merge-slice", "merge-slice16B", "merge-wave"])
 def test_split_widths(rmat14, N, sched):
     case(f"split width N={N} {sched}")
     a, (d_rp, d_ci, d_va) = rmat14
     B = np.random.default_rng(N).uniform(-1, 1, (a.shape[1], N)).astype(np.float32)
-    cfg = {"merge-wide": dict(schedule="merge", group=1), "merge-slice": SLICE}.get(sched, sched)
+    if sched == "merge-slice16B" and N % 4:
+        pytest.skip("16-byte lanes need N % 4 == 0")
+    cfg = {"merge-wide": dict(schedule="merge", group=1), "merge-slice": SLICE,
+           "merge-slice16B": dict(SLICE, vec=4), "merge-wave": dict(schedule="merge", group=64)}.get(sched, sched)
     out = kernels.spmm(d_rp, d_ci, d_va, dev(B)[0], config=cfg).cpu().numpy()
     ref = oracle.spmm_f64(a.rowptr, a.colind, a.values, B)
     if N >= 16:
@@ -275,6 +282,10 @@ def test_full_size_rmat_1m_n128():
     check_rows(merged, rowwise, f64, "1M R-MAT merge")
     sliced = kernels.spmm(d_rp, d_ci, d_va, B, config=SLICE).cpu().numpy()
     check_rows(sliced, rowwise, f64, "1M R-MAT merge, column slices")
+    paired = kernels.spmm(d_rp, d_ci, d_va, B, config=dict(SLICE, vec=4)).cpu().numpy()
+    check_rows(paired, rowwise, f64, "1M R-MAT merge, half wave per nonzero, 16-byte lanes")
+    wave = kernels.spmm(d_rp, d_ci, d_va, B, config=dict(schedule="merge", group=64)).cpu().numpy()
+    check_rows(wave, rowwise, f64, "1M R-MAT merge, whole wave per nonzero")
 
 
 @pytest.mark.parametrize("sched", ["row", "merge", "nnz", "slice", "narrow"])
