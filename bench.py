@@ -285,7 +285,9 @@ def main():
                    "hbm_pct_of_peak": round(100 * value / HBM_PEAK_GBS, 2)},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                     "kernel": "k_spmm_split (merge-path main kernel)", "avg_launch_us": round(k_avg * 1e6, 2)},
+                     "kernel": "SpMM main kernel of the AUTO schedule (k_spmm_slice: merge path, half a wave per "
+                               "nonzero, 256-byte column slices)" if cfg is None else "SpMM main kernel",
+                     "avg_launch_us": round(k_avg * 1e6, 2)},
         "cpu_baseline": cpu,
         "extras": extras,
     }
