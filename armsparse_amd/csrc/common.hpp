@@ -53,6 +53,10 @@ struct TimedLaunch {
 // Number of compute units on the current device (cached).
 int device_cus();
 
+// part[w] = rows consumed by the merge path of (row ends, nonzeros) at diagonal
+// min(w·chunk, M + nnz), w = 0..nwaves (nwaves + 1 entries), on stream s.
+int merge_partition(int M, int nnz, int chunk, int64_t nwaves, const int* rowptr, int* part, hipStream_t s);
+
 // ------------------------------------------------------------ device utils
 __device__ __forceinline__ int wave_lane() { return __lane_id(); }
 
@@ -139,6 +143,16 @@ __device__ __forceinline__ int merge_path_search_wave(int d, const int* __restri
         }
     }
     return lo;
+}
+
+// Row boundary w of a precomputed merge-path partition (merge_partition) for
+// diagonal d, clamped to the interval a search could return so that a
+// malformed rowptr cannot carry a kernel outside its chunk.
+__device__ __forceinline__ int partition_at(const int* __restrict__ part, int w, int d, int M, int nnz) {
+    const int lo = d - nnz > 0 ? d - nnz : 0;
+    const int hi = d < M ? d : M;
+    const int i = part[w];
+    return i < lo ? lo : (i > hi ? hi : i);
 }
 
 // Number of rows whose end is ≤ j (rows fully before nonzero j).

@@ -178,7 +178,7 @@ __global__ __launch_bounds__(256) void k_sddmm_stream(int M, int Kd, int nnz, in
                                                       const int* __restrict__ rowidx, const int* __restrict__ colind,
                                                       const float* __restrict__ scale, const T* __restrict__ U,
                                                       int ldu, const T* __restrict__ V, int ldv,
-                                                      float* __restrict__ out) {
+                                                      const int* __restrict__ part, float* __restrict__ out) {
     extern __shared__ __align__(16) unsigned char smem[];
     const int lane = threadIdx.x & 63;
     const int wid = threadIdx.x >> 6;
@@ -191,8 +191,8 @@ __global__ __launch_bounds__(256) void k_sddmm_stream(int M, int Kd, int nnz, in
     const int d1 = (int)min(d0l + chunk, total);
     int i0 = 0, j0 = d0, j1 = d1;
     if constexpr (!COO) {
-        i0 = uniform(merge_path_search_wave(d0, rowidx, M, nnz));
-        const int i1 = uniform(merge_path_search_wave(d1, rowidx, M, nnz));
+        i0 = uniform(partition_at(part, wave, d0, M, nnz));  // merge_partition
+        const int i1 = uniform(partition_at(part, wave + 1, d1, M, nnz));
         j0 = d0 - i0;
         j1 = d1 - i1;
     }
@@ -297,12 +297,13 @@ inline size_t sddmm_lds_bytes(int chunk) {
 
 template <typename T, int G, int W>
 void launch_sddmm_stream(bool coo, dim3 grid, hipStream_t s, int M, int Kd, int nnz, int chunk, const int* ri,
-                         const int* ci, const float* sc, const T* U, int ldu, const T* V, int ldv, float* out) {
+                         const int* ci, const float* sc, const T* U, int ldu, const T* V, int ldv, const int* part,
+                         float* out) {
     const size_t lds = sddmm_lds_bytes(chunk);
     const bool full = Kd % (G * W) == 0;
 #define ARMSP_SDDMM_ST(CO, FU)                                                                                 \
     hipLaunchKernelGGL((k_sddmm_stream<T, G, W, CO, FU>), grid, dim3(256), lds, s, M, Kd, nnz, chunk, ri, ci, sc, U, \
-                       ldu, V, ldv, out)
+                       ldu, V, ldv, part, out)
     if (coo) {
         if (full) ARMSP_SDDMM_ST(true, true); else ARMSP_SDDMM_ST(true, false);
     } else {
@@ -359,14 +360,29 @@ int sddmm_impl(bool coo, int M, int K, int Kd, int nnz, const int* rowidx, const
     const int64_t nwaves = (items + chunk - 1) / chunk;
     if (nwaves > INT_MAX / 4) return ARMSP_ERROR_NOT_SUPPORTED;
     const dim3 grid((unsigned)((nwaves + 3) / 4));
+    // the staged CSR kernel starts each wave from a precomputed merge-path partition
+    void* ws = nullptr;
+    int* part = nullptr;
+    if (staged && !coo) {
+        int st = ws_alloc(&ws, (size_t)(nwaves + 1) * sizeof(int), s);
+        if (st) return st;
+        part = static_cast<int*>(ws);
+        TimedLaunch tp("merge_partition", s);
+        st = merge_partition(M, nnz, chunk, nwaves, rowidx, part, s);
+        if (st) {
+            ws_free(ws, s);
+            return st;
+        }
+        tp.stop();
+    }
     TimedLaunch t("sddmm", s);
 #define ARMSP_SDDMM_G(WW)                                                                                    \
     if (staged) {                                                                                            \
         switch (G) {                                                                                         \
-            case 4: launch_sddmm_stream<T, 4, WW>(coo, grid, s, M, Kd, nnz, chunk, rowidx, colind, scale, U, ldu, V, ldv, out); break; \
-            case 8: launch_sddmm_stream<T, 8, WW>(coo, grid, s, M, Kd, nnz, chunk, rowidx, colind, scale, U, ldu, V, ldv, out); break; \
-            case 16: launch_sddmm_stream<T, 16, WW>(coo, grid, s, M, Kd, nnz, chunk, rowidx, colind, scale, U, ldu, V, ldv, out); break; \
-            default: launch_sddmm_stream<T, 32, WW>(coo, grid, s, M, Kd, nnz, chunk, rowidx, colind, scale, U, ldu, V, ldv, out); break; \
+            case 4: launch_sddmm_stream<T, 4, WW>(coo, grid, s, M, Kd, nnz, chunk, rowidx, colind, scale, U, ldu, V, ldv, part, out); break; \
+            case 8: launch_sddmm_stream<T, 8, WW>(coo, grid, s, M, Kd, nnz, chunk, rowidx, colind, scale, U, ldu, V, ldv, part, out); break; \
+            case 16: launch_sddmm_stream<T, 16, WW>(coo, grid, s, M, Kd, nnz, chunk, rowidx, colind, scale, U, ldu, V, ldv, part, out); break; \
+            default: launch_sddmm_stream<T, 32, WW>(coo, grid, s, M, Kd, nnz, chunk, rowidx, colind, scale, U, ldu, V, ldv, part, out); break; \
         }                                                                                                    \
     } else {                                                                                                 \
         switch (G) {                                                                                         \
@@ -382,7 +398,9 @@ int sddmm_impl(bool coo, int M, int K, int Kd, int nnz, const int* rowidx, const
         ARMSP_SDDMM_G(WMAX)
     }
 #undef ARMSP_SDDMM_G
-    ARMSP_LAUNCHED();
+    const hipError_t e = hipGetLastError();
+    if (ws) ws_free(ws, s);
+    if (e != hipSuccess) return hip_fail(e);
     t.stop();
     return ARMSP_SUCCESS;
 }

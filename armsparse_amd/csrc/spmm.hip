@@ -379,7 +379,8 @@ __global__ __launch_bounds__(256) void k_spmm_stream(int M, int N, int nnz, int 
                                                      const int* __restrict__ rowptr, const int* __restrict__ colind,
                                                      const float* __restrict__ val, const TB* __restrict__ B,
                                                      int ldb, Epi ep, float* __restrict__ C, int ldc, int flags,
-                                                     int* __restrict__ carry_row, float* __restrict__ carry_val) {
+                                                     const int* __restrict__ part, int* __restrict__ carry_row,
+                                                     float* __restrict__ carry_val) {
     extern __shared__ __align__(16) unsigned char smem[];
     const int lane = threadIdx.x & 63;
     const int wid = threadIdx.x >> 6;
@@ -389,8 +390,8 @@ __global__ __launch_bounds__(256) void k_spmm_stream(int M, int N, int nnz, int 
     if (d0l >= total) return;
     const int d0 = (int)d0l;
     const int d1 = (int)min(d0l + chunk, total);
-    const int i0 = uniform(merge_path_search_wave(d0, rowptr, M, nnz));
-    const int i1 = uniform(merge_path_search_wave(d1, rowptr, M, nnz));
+    const int i0 = uniform(partition_at(part, wave, d0, M, nnz));  // merge_partition
+    const int i1 = uniform(partition_at(part, wave + 1, d1, M, nnz));
     const int j0 = d0 - i0, j1 = d1 - i1;
     const int nz = j1 - j0;      // ≤ chunk
     const int nr = i1 - i0 + 1;  // rows i0..i1, ≤ chunk + 1
@@ -519,7 +520,8 @@ __global__ __launch_bounds__(256) void k_spmm_slice(int M, int N, int nnz, int c
                                                     const int* __restrict__ rowptr, const int* __restrict__ colind,
                                                     const float* __restrict__ val, const TB* __restrict__ B, int ldb,
                                                     Epi ep, float* __restrict__ C, int ldc, int flags,
-                                                    int* __restrict__ carry_row, float* __restrict__ carry_val) {
+                                                    const int* __restrict__ part, int* __restrict__ carry_row,
+                                                    float* __restrict__ carry_val) {
     // V columns per lane (4, 8 or 16 bytes); a slice is the 32·V columns of a half wave
     constexpr int SW = 32 * V;
     extern __shared__ __align__(16) unsigned char smem[];
@@ -542,8 +544,8 @@ __global__ __launch_bounds__(256) void k_spmm_slice(int M, int N, int nnz, int c
     if (d0l >= total) return;
     const int d0 = (int)d0l;
     const int d1 = (int)min(d0l + chunk, total);
-    const int i0 = uniform(merge_path_search_wave(d0, rowptr, M, nnz));
-    const int i1 = uniform(merge_path_search_wave(d1, rowptr, M, nnz));
+    const int i0 = uniform(partition_at(part, wave, d0, M, nnz));  // merge_partition
+    const int i1 = uniform(partition_at(part, wave + 1, d1, M, nnz));
     const int j0 = d0 - i0, j1 = d1 - i1;
     const int nz = j1 - j0;
     const int nr = i1 - i0 + 1;
@@ -692,8 +694,8 @@ template <typename TB, int NV, int IPT>
 __global__ __launch_bounds__(256) void k_spmm_narrow(int M, int N, int nnz, const int* __restrict__ rowptr,
                                                      const int* __restrict__ colind, const float* __restrict__ val,
                                                      const TB* __restrict__ B, int ldb, Epi ep, float* __restrict__ C,
-                                                     int ldc, int flags, int* __restrict__ carry_row,
-                                                     float* __restrict__ carry_val) {
+                                                     int ldc, int flags, const int* __restrict__ part,
+                                                     int* __restrict__ carry_row, float* __restrict__ carry_val) {
     constexpr int TILE = 64 * IPT;
     __shared__ float s_prod[4][TILE * NV];
     __shared__ int s_re[4][TILE + 1];
@@ -705,8 +707,8 @@ __global__ __launch_bounds__(256) void k_spmm_narrow(int M, int N, int nnz, cons
     if (d0l >= total) return;
     const int d0 = (int)d0l;
     const int d1 = (int)min(d0l + TILE, total);
-    const int i0 = uniform(merge_path_search_wave(d0, rowptr, M, nnz));
-    const int i1 = uniform(merge_path_search_wave(d1, rowptr, M, nnz));
+    const int i0 = uniform(partition_at(part, wave, d0, M, nnz));  // merge_partition
+    const int i1 = uniform(partition_at(part, wave + 1, d1, M, nnz));
     const int j0 = d0 - i0, j1 = d1 - i1;
     const int nz = j1 - j0, nrows = i1 - i0;  // rows i0..i1-1 end inside the tile
     const uint32_t rowbytes = (uint32_t)ldb * (uint32_t)sizeof(TB);
@@ -857,58 +859,83 @@ inline size_t stream_lds_bytes(int chunk) {
 }
 
 // Adds each run of carries (consecutive waves cut inside the same row) into
-// that row, summing the run in wave order.
+// that row: a wave per carry, and the one that heads a run sums it.  One load
+// round reads the 64 row tags from the previous wave's on (head test and run
+// length together); the lanes cover the columns with lpr lanes a row, so
+// 64 / lpr carries share a load instruction, 8 loads in flight per lane, and
+// a butterfly over the sub-rows joins them in a fixed order — a hub row cut
+// into hundreds of chunks costs a few load round trips, and the sum does not
+// depend on timing.
 template <int V>
 __global__ __launch_bounds__(256) void k_carry_fixup(int nwaves, int N, const int* __restrict__ carry_row,
                                                      const float* __restrict__ carry_val, float* __restrict__ C,
                                                      int ldc, Epi ep) {
+    constexpr int UF = 8;
     const int lane = threadIdx.x & 63;
     const int w = uniform((int)(blockIdx.x * 4u + (threadIdx.x >> 6)));
     if (w >= nwaves) return;
-    const int r = carry_row[w];
-    if (r < 0) return;
-    if (w > 0 && carry_row[w - 1] == r) return;
-    // run length
-    int run = 0;
-    for (int base = w;; base += 64) {
-        const int cr = (base + lane < nwaves) ? carry_row[base + lane] : -2;
-        const uint64_t m = __ballot(cr == r);
-        const int cnt = (~m == 0ull) ? 64 : __builtin_ctzll(~m);
-        run += cnt;
-        if (cnt < 64) break;
+    const int idx = w - 1 + lane;  // lane 0: the previous wave, lane 1: this one
+    const int cr = (idx >= 0 && idx < nwaves) ? carry_row[idx] : -2;
+    const int r = read_lane(cr, 1);
+    if (r < 0 || read_lane(cr, 0) == r) return;
+    const uint64_t m = __ballot(cr == r) >> 1;  // bit k: wave w + k carries row r, k < 63
+    int run = __builtin_ctzll(~m);
+    if (run == 63) {
+        for (int base = w + 63;; base += 64) {
+            const int c2 = (base + lane < nwaves) ? carry_row[base + lane] : -2;
+            const uint64_t m2 = __ballot(c2 == r);
+            const int cnt = (~m2 == 0ull) ? 64 : __builtin_ctzll(~m2);
+            run += cnt;
+            if (cnt < 64) break;
+        }
     }
-    for (int col = (int)(blockIdx.y * 64u + lane) * V; col < N; col += (int)gridDim.y * 64 * V) {
+    const int cols = (N + V - 1) / V;  // lanes a row needs
+    int lpr = 1;
+    while (lpr < 64 && lpr < cols) lpr <<= 1;
+    const int P = 64 / lpr;  // carries per load instruction
+    const int p = lane / lpr, cl = lane % lpr;
+    const float* cv = carry_val + (size_t)w * N;
+    for (int col0 = 0; col0 < N; col0 += lpr * V) {
+        const int col = col0 + cl * V;
+        const bool cok = col < N;
+        float* cp = C + (size_t)r * ldc + col;
+        float o[V];  // C's row, loaded together with the first carries
+        if (p == 0 && cok) load_c<V>(cp, o);
         float s[V];
 #pragma unroll
         for (int v = 0; v < V; ++v) s[v] = 0.0f;
-        int k = 0;
-        for (; k + 4 <= run; k += 4) {
-            float t[4][V];
+        if (cok) {
+            int k = p;
+            for (; k + (UF - 1) * P < run; k += UF * P) {
+                float t[UF][V];
 #pragma unroll
-            for (int q = 0; q < 4; ++q) load_c<V>(carry_val + (size_t)(w + k + q) * N + col, t[q]);
+                for (int q = 0; q < UF; ++q) load_c<V>(cv + (size_t)(k + q * P) * N + col, t[q]);
 #pragma unroll
-            for (int q = 0; q < 4; ++q)
+                for (int q = 0; q < UF; ++q)
 #pragma unroll
-                for (int v = 0; v < V; ++v) s[v] += t[q][v];
+                    for (int v = 0; v < V; ++v) s[v] += t[q][v];
+            }
+            for (; k < run; k += P) {
+                float t[V];
+                load_c<V>(cv + (size_t)k * N + col, t);
+#pragma unroll
+                for (int v = 0; v < V; ++v) s[v] += t[v];
+            }
         }
-        for (; k < run; ++k) {
-            float t[V];
-            load_c<V>(carry_val + (size_t)(w + k) * N + col, t);
+        for (int o = lpr; o < 64; o <<= 1)
 #pragma unroll
-            for (int v = 0; v < V; ++v) s[v] += t[v];
+            for (int v = 0; v < V; ++v) s[v] += __shfl_xor(s[v], o);
+        if (p == 0 && cok) {
+            const float rs = ep.rscale ? ep.rscale[r] : 1.0f;
+#pragma unroll
+            for (int v = 0; v < V; ++v) o[v] += rs * s[v];
+            if (ep.relu) {
+#pragma unroll
+                for (int v = 0; v < V; ++v) o[v] = fmaxf(o[v], 0.0f);
+            }
+#pragma unroll
+            for (int v = 0; v < V; ++v) cp[v] = o[v];
         }
-        float* cp = C + (size_t)r * ldc + col;
-        float o[V];
-        load_c<V>(cp, o);
-        const float rs = ep.rscale ? ep.rscale[r] : 1.0f;
-#pragma unroll
-        for (int v = 0; v < V; ++v) o[v] += rs * s[v];
-        if (ep.relu) {
-#pragma unroll
-            for (int v = 0; v < V; ++v) o[v] = fmaxf(o[v], 0.0f);
-        }
-#pragma unroll
-        for (int v = 0; v < V; ++v) cp[v] = o[v];
     }
 }
 
@@ -972,14 +999,14 @@ int launch_split_vu(int U, dim3 grid, hipStream_t s, int M, int N, int nnz, int 
 
 template <typename TB, int V>
 int launch_stream_vu(int U, dim3 grid, hipStream_t s, int M, int N, int nnz, int chunk, const int* rp, const int* ci,
-                     const float* va, const TB* B, int ldb, const Epi& bs, float* C, int ldc, int flags, int* crow,
-                     float* cval) {
+                     const float* va, const TB* B, int ldb, const Epi& bs, float* C, int ldc, int flags,
+                     const int* part, int* crow, float* cval) {
     const size_t lds = stream_lds_bytes(chunk);
     switch (U) {
 #define ARMSP_STREAM_CASE(UU)                                                                                  \
     case UU:                                                                                                   \
         hipLaunchKernelGGL((k_spmm_stream<TB, V, UU>), grid, dim3(256), lds, s, M, N, nnz, chunk, rp, ci, va, B, ldb, \
-                           bs, C, ldc, flags, crow, cval);                                                     \
+                           bs, C, ldc, flags, part, crow, cval);                                               \
         break;
         ARMSP_STREAM_CASE(4)
         ARMSP_STREAM_CASE(8)
@@ -993,7 +1020,7 @@ int launch_stream_vu(int U, dim3 grid, hipStream_t s, int M, int N, int nnz, int
 template <typename TB, int V>
 int launch_slice_v(int U, int S, int G, hipStream_t s, int M, int N, int nnz, int chunk, const int* rp,
                    const int* ci, const float* va, const TB* B, int ldb, const Epi& bs, float* C, int ldc, int flags,
-                   int* crow, float* cval) {
+                   const int* part, int* crow, float* cval) {
     const size_t lds = stream_lds_bytes(chunk);
     const int64_t nblk = (8 % S == 0) ? ((int64_t)(G + 8 / S - 1) / (8 / S)) * 8 : (int64_t)G * S;
     if (nblk > INT_MAX) return ARMSP_ERROR_NOT_SUPPORTED;
@@ -1002,7 +1029,7 @@ int launch_slice_v(int U, int S, int G, hipStream_t s, int M, int N, int nnz, in
 #define ARMSP_SLICE_CASE(UU)                                                                                     \
     case UU:                                                                                                     \
         hipLaunchKernelGGL((k_spmm_slice<TB, V, UU>), grid, dim3(256), lds, s, M, N, nnz, chunk, S, G, rp, ci, va, B, \
-                           ldb, bs, C, ldc, flags, crow, cval);                                                  \
+                           ldb, bs, C, ldc, flags, part, crow, cval);                                            \
         break;
         ARMSP_SLICE_CASE(4)
         ARMSP_SLICE_CASE(8)
@@ -1017,31 +1044,31 @@ int launch_slice_v(int U, int S, int G, hipStream_t s, int M, int N, int nnz, in
 template <typename TB>
 int launch_slice(int sv, int U, int S, int G, hipStream_t s, int M, int N, int nnz, int chunk, const int* rp,
                  const int* ci, const float* va, const TB* B, int ldb, const Epi& bs, float* C, int ldc, int flags,
-                 int* crow, float* cval) {
+                 const int* part, int* crow, float* cval) {
     switch (sv * (int)sizeof(TB)) {
         case 4:
             return launch_slice_v<TB, 4 / sizeof(TB)>(U, S, G, s, M, N, nnz, chunk, rp, ci, va, B, ldb, bs, C, ldc,
-                                                      flags, crow, cval);
+                                                      flags, part, crow, cval);
         case 8:
             return launch_slice_v<TB, 8 / sizeof(TB)>(U, S, G, s, M, N, nnz, chunk, rp, ci, va, B, ldb, bs, C, ldc,
-                                                      flags, crow, cval);
+                                                      flags, part, crow, cval);
         case 16:
             return launch_slice_v<TB, 16 / sizeof(TB)>(U, S, G, s, M, N, nnz, chunk, rp, ci, va, B, ldb, bs, C, ldc,
-                                                       flags, crow, cval);
+                                                       flags, part, crow, cval);
         default: return ARMSP_ERROR_INVALID_VALUE;
     }
 }
 
 template <typename TB, int NV>
 int launch_narrow_nv(int ipt, int64_t nwaves, hipStream_t s, int M, int N, int nnz, const int* rp, const int* ci,
-                     const float* va, const TB* B, int ldb, const Epi& bs, float* C, int ldc, int flags, int* crow,
-                     float* cval) {
+                     const float* va, const TB* B, int ldb, const Epi& bs, float* C, int ldc, int flags,
+                     const int* part, int* crow, float* cval) {
     const dim3 grid((unsigned)((nwaves + 3) / 4));
     switch (ipt) {
 #define ARMSP_NARROW_CASE(II)                                                                                   \
     case II:                                                                                                    \
         hipLaunchKernelGGL((k_spmm_narrow<TB, NV, II>), grid, dim3(256), 0, s, M, N, nnz, rp, ci, va, B, ldb, bs, C, \
-                           ldc, flags, crow, cval);                                                             \
+                           ldc, flags, part, crow, cval);                                                       \
         break;
         ARMSP_NARROW_CASE(4)
         ARMSP_NARROW_CASE(8)
@@ -1054,23 +1081,24 @@ int launch_narrow_nv(int ipt, int64_t nwaves, hipStream_t s, int M, int N, int n
 
 template <typename TB>
 int launch_narrow(int ipt, int64_t nwaves, hipStream_t s, int M, int N, int nnz, const int* rp, const int* ci,
-                  const float* va, const TB* B, int ldb, const Epi& bs, float* C, int ldc, int flags, int* crow,
-                  float* cval) {
-    if (N <= 1) return launch_narrow_nv<TB, 1>(ipt, nwaves, s, M, N, nnz, rp, ci, va, B, ldb, bs, C, ldc, flags, crow, cval);
-    if (N <= 2) return launch_narrow_nv<TB, 2>(ipt, nwaves, s, M, N, nnz, rp, ci, va, B, ldb, bs, C, ldc, flags, crow, cval);
-    if (N <= 4) return launch_narrow_nv<TB, 4>(ipt, nwaves, s, M, N, nnz, rp, ci, va, B, ldb, bs, C, ldc, flags, crow, cval);
-    return launch_narrow_nv<TB, 8>(ipt, nwaves, s, M, N, nnz, rp, ci, va, B, ldb, bs, C, ldc, flags, crow, cval);
+                  const float* va, const TB* B, int ldb, const Epi& bs, float* C, int ldc, int flags,
+                  const int* part, int* crow, float* cval) {
+    if (N <= 1) return launch_narrow_nv<TB, 1>(ipt, nwaves, s, M, N, nnz, rp, ci, va, B, ldb, bs, C, ldc, flags, part, crow, cval);
+    if (N <= 2) return launch_narrow_nv<TB, 2>(ipt, nwaves, s, M, N, nnz, rp, ci, va, B, ldb, bs, C, ldc, flags, part, crow, cval);
+    if (N <= 4) return launch_narrow_nv<TB, 4>(ipt, nwaves, s, M, N, nnz, rp, ci, va, B, ldb, bs, C, ldc, flags, part, crow, cval);
+    return launch_narrow_nv<TB, 8>(ipt, nwaves, s, M, N, nnz, rp, ci, va, B, ldb, bs, C, ldc, flags, part, crow, cval);
 }
 
 // MERGE gathers through 32-bit buffer offsets unless a dense-row offset could exceed them
 template <typename TB, int V, bool MERGE>
 int launch_split(bool wide, int U, dim3 grid, hipStream_t s, int M, int N, int nnz, int chunk, const int* rp,
                  const int* ci, const float* va, const TB* B, int ldb, const Epi& bs, float* C, int ldc, int flags,
-                 int* crow, float* cval) {
+                 const int* part, int* crow, float* cval) {
     if (wide || !MERGE || chunk > 2048)  // the LDS staging holds at most 2048 items per wave
         return launch_split_vu<TB, V, MERGE>(U, grid, s, M, N, nnz, chunk, rp, ci, va, B, ldb, bs, C, ldc, flags, crow,
                                              cval);
-    return launch_stream_vu<TB, V>(U, grid, s, M, N, nnz, chunk, rp, ci, va, B, ldb, bs, C, ldc, flags, crow, cval);
+    return launch_stream_vu<TB, V>(U, grid, s, M, N, nnz, chunk, rp, ci, va, B, ldb, bs, C, ldc, flags, part, crow,
+                                   cval);
 }
 
 template <typename TB>
@@ -1153,12 +1181,27 @@ int spmm_impl(int M, int K, int N, int nnz, const int* rowptr, const int* colind
     const unsigned tiles = (unsigned)((N + 64 * V - 1) / (64 * V));
     if (tiles > 65535) return ARMSP_ERROR_NOT_SUPPORTED;
 
+    // the LDS-staged merge kernels (stream, slice, narrow) start from a precomputed
+    // merge-path partition: nwaves + 1 row boundaries
+    const bool staged = merge && !wide && chunk <= 2048;
     void* ws = nullptr;
     const size_t row_bytes = ((size_t)nwaves * sizeof(int) + 255) & ~(size_t)255;
-    int st = ws_alloc(&ws, row_bytes + (size_t)nwaves * N * sizeof(float), s);
+    const size_t val_bytes = ((size_t)nwaves * N * sizeof(float) + 255) & ~(size_t)255;
+    const size_t part_bytes = staged ? (size_t)(nwaves + 1) * sizeof(int) : 0;
+    int st = ws_alloc(&ws, row_bytes + val_bytes + part_bytes, s);
     if (st) return st;
     int* crow = static_cast<int*>(ws);
     float* cval = reinterpret_cast<float*>(static_cast<char*>(ws) + row_bytes);
+    int* part = staged ? reinterpret_cast<int*>(static_cast<char*>(ws) + row_bytes + val_bytes) : nullptr;
+    if (staged) {
+        TimedLaunch t("merge_partition", s);
+        st = merge_partition(M, nnz, chunk, nwaves, rowptr, part, s);
+        if (st) {
+            ws_free(ws, s);
+            return st;
+        }
+        t.stop();
+    }
 
     // group 32: half a wave per nonzero over column slices of B (k_spmm_slice), each lane
     // 4 bytes of B (vec 0) or vec columns of 4, 8 or 16 bytes.
@@ -1183,7 +1226,7 @@ int spmm_impl(int M, int K, int N, int nnz, const int* rowptr, const int* colind
     if (narrow) {
         TimedLaunch t("spmm", s);
         st = launch_narrow<TB>(chunk / 64, nwaves, s, M, N, nnz, rowptr, colind, val, B, ldb, bscale, C, ldc, flags,
-                               crow, cval);
+                               part, crow, cval);
         hipError_t e = st ? hipSuccess : hipGetLastError();
         if (st || e != hipSuccess) {
             ws_free(ws, s);
@@ -1194,7 +1237,7 @@ int spmm_impl(int M, int K, int N, int nnz, const int* rowptr, const int* colind
         TimedLaunch t("spmm", s);
         const int S = (N + 32 * SV - 1) / (32 * SV);
         st = launch_slice<TB>(SV, U, S, (int)((nwaves + 3) / 4), s, M, N, nnz, chunk, rowptr, colind, val, B, ldb,
-                              bscale, C, ldc, flags, crow, cval);
+                              bscale, C, ldc, flags, part, crow, cval);
         hipError_t e = st ? hipSuccess : hipGetLastError();
         if (st || e != hipSuccess) {
             ws_free(ws, s);
@@ -1206,9 +1249,9 @@ int spmm_impl(int M, int K, int N, int nnz, const int* rowptr, const int* colind
 #define ARMSP_SPLIT_V(VV)                                                                                          \
     case VV:                                                                                                       \
         st = merge ? launch_split<TB, VV, true>(wide, U, grid, s, M, N, nnz, chunk, rowptr, colind, val, B, ldb,    \
-                                                bscale, C, ldc, flags, crow, cval)                                 \
+                                                bscale, C, ldc, flags, part, crow, cval)                           \
                    : launch_split<TB, VV, false>(wide, U, grid, s, M, N, nnz, chunk, rowptr, colind, val, B, ldb,   \
-                                                 bscale, C, ldc, flags, crow, cval);                               \
+                                                 bscale, C, ldc, flags, part, crow, cval);                         \
         break;
         switch (V) {
             ARMSP_SPLIT_V(1)
@@ -1217,9 +1260,9 @@ int spmm_impl(int M, int K, int N, int nnz, const int* rowptr, const int* colind
             case 8:
                 if constexpr (sizeof(TB) < 4) {
                     st = merge ? launch_split<TB, 8, true>(wide, U, grid, s, M, N, nnz, chunk, rowptr, colind, val, B,
-                                                           ldb, bscale, C, ldc, flags, crow, cval)
+                                                           ldb, bscale, C, ldc, flags, part, crow, cval)
                                : launch_split<TB, 8, false>(wide, U, grid, s, M, N, nnz, chunk, rowptr, colind, val, B,
-                                                            ldb, bscale, C, ldc, flags, crow, cval);
+                                                            ldb, bscale, C, ldc, flags, part, crow, cval);
                     break;
                 }
                 [[fallthrough]];

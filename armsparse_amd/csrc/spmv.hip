@@ -43,7 +43,8 @@ template <int IPT>
 __global__ __launch_bounds__(256) void k_spmv_merge(int M, int nnz, const int* __restrict__ rowptr,
                                                     const int* __restrict__ colind, const float* __restrict__ val,
                                                     const float* __restrict__ x, float* __restrict__ y, int flags,
-                                                    int* __restrict__ carry_row, float* __restrict__ carry_val) {
+                                                    const int* __restrict__ part, int* __restrict__ carry_row,
+                                                    float* __restrict__ carry_val) {
     constexpr int TILE = 64 * IPT;
     __shared__ int s_re[4][TILE + 1];
     __shared__ float s_prod[4][TILE];
@@ -56,17 +57,35 @@ __global__ __launch_bounds__(256) void k_spmv_merge(int M, int nnz, const int* _
     if (d0l >= total) return;
     const int d0 = (int)d0l;
     const int d1 = (int)(d0l + TILE < total ? d0l + TILE : total);
-    const int i0 = uniform(merge_path_search_wave(d0, rowptr, M, nnz));
-    const int i1 = uniform(merge_path_search_wave(d1, rowptr, M, nnz));
+    const int i0 = uniform(partition_at(part, wave, d0, M, nnz));  // merge_partition
+    const int i1 = uniform(partition_at(part, wave + 1, d1, M, nnz));
     const int j0 = d0 - i0, j1 = d1 - i1;
     const int nrows = i1 - i0, nz = j1 - j0;
     int* re = s_re[wid];
     float* prod = s_prod[wid];
     float* out = s_out[wid];
     for (int k = lane; k < nrows; k += 64) re[k] = rowptr[i0 + k + 1];
-    for (int k = lane; k < nz; k += 64) {
-        const int p = j0 + k;
-        prod[k] = (val ? val[p] : 1.0f) * x[colind[p]];
+    {
+        // nonzero k = lane + 64·q: all IPT index loads, then all IPT gathers of x, in flight together
+        int cs[IPT];
+        float vs[IPT], xs[IPT];
+#pragma unroll
+        for (int q = 0; q < IPT; ++q) {
+            const int k = lane + 64 * q;
+            cs[q] = 0;
+            vs[q] = 0.0f;
+            if (k < nz) {
+                cs[q] = __builtin_nontemporal_load(colind + j0 + k);
+                vs[q] = val ? __builtin_nontemporal_load(val + j0 + k) : 1.0f;
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < IPT; ++q) xs[q] = (lane + 64 * q < nz) ? x[cs[q]] : 0.0f;
+#pragma unroll
+        for (int q = 0; q < IPT; ++q) {
+            const int k = lane + 64 * q;
+            if (k < nz) prod[k] = vs[q] * xs[q];
+        }
     }
     wave_sync_lds();
 
@@ -124,15 +143,45 @@ __global__ __launch_bounds__(256) void k_spmv_merge(int M, int nnz, const int* _
     }
 }
 
+// Adds each run of carries (consecutive waves cut inside the same row) into
+// y.  A wave takes 64 consecutive carries; a segmented suffix sum (rows are
+// non-decreasing along the carries, so equal tags mark one run) leaves each
+// run's in-wave total on its head lane.  A run that goes on past the wave is
+// finished by the wave that holds its head, 64 carries a load, and a run that
+// began in an earlier wave is left to that wave.  Fixed order, no atomics.
 __global__ __launch_bounds__(256) void k_spmv_fixup(int nwaves, const int* __restrict__ carry_row,
                                                     const float* __restrict__ carry_val, float* __restrict__ y) {
-    const int w = (int)(blockIdx.x * 256u + threadIdx.x);
-    if (w >= nwaves) return;
-    const int r = carry_row[w];
-    if (r < 0 || (w > 0 && carry_row[w - 1] == r)) return;
-    float s = 0.0f;
-    for (int k = w; k < nwaves && carry_row[k] == r; ++k) s += carry_val[k];
-    y[r] += s;
+    const int lane = threadIdx.x & 63;
+    const int base = uniform((int)(blockIdx.x * 4u + (threadIdx.x >> 6))) * 64;
+    if (base >= nwaves) return;
+    const int w = base + lane;
+    const int r = w < nwaves ? carry_row[w] : -2;
+    const int rprev = (w == 0 || w > nwaves) ? -2 : carry_row[w - 1];
+    float s = (w < nwaves && r >= 0) ? carry_val[w] : 0.0f;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const float t = __shfl_down(s, o);
+        const int rt = __shfl_down(r, o);
+        if (lane + o < 64 && rt == r) s += t;
+    }
+    // the run of lane 63 may go on into the next waves; the wave holding its head sums the rest
+    const int r63 = read_lane(r, 63);
+    const bool owner = __ballot(r == r63 && rprev != r63) != 0ull;
+    float ext = 0.0f;
+    if (r63 >= 0 && owner && base + 64 < nwaves && carry_row[base + 64] == r63) {
+        float acc = 0.0f;
+        for (int b = base + 64;; b += 64) {
+            const int cr = (b + lane < nwaves) ? carry_row[b + lane] : -2;
+            const uint64_t m = __ballot(cr == r63);
+            const int cnt = (~m == 0ull) ? 64 : __builtin_ctzll(~m);
+            if (lane < cnt) acc += carry_val[b + lane];
+            if (cnt < 64) break;
+        }
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o);
+        ext = acc;
+    }
+    if (r >= 0 && rprev != r) y[r] += (r == r63 ? s + ext : s);
 }
 
 __global__ __launch_bounds__(256) void k_spmv_vcsr(int M, const int* __restrict__ rowptr,
@@ -167,14 +216,24 @@ int launch_merge(int M, int nnz, const int* rp, const int* ci, const float* va, 
     if (nwaves > INT_MAX / 4) return ARMSP_ERROR_NOT_SUPPORTED;
     void* ws = nullptr;
     const size_t rb = ((size_t)nwaves * 4 + 255) & ~(size_t)255;
-    int st = ws_alloc(&ws, rb + (size_t)nwaves * 4, s);
+    int st = ws_alloc(&ws, 2 * rb + (size_t)(nwaves + 1) * 4, s);
     if (st) return st;
     int* crow = static_cast<int*>(ws);
     float* cval = reinterpret_cast<float*>(static_cast<char*>(ws) + rb);
+    int* part = reinterpret_cast<int*>(static_cast<char*>(ws) + 2 * rb);
+    {
+        TimedLaunch t("merge_partition", s);
+        st = merge_partition(M, nnz, TILE, nwaves, rp, part, s);
+        if (st) {
+            ws_free(ws, s);
+            return st;
+        }
+        t.stop();
+    }
     {
         TimedLaunch t("spmv", s);
         hipLaunchKernelGGL((k_spmv_merge<IPT>), dim3((unsigned)((nwaves + 3) / 4)), dim3(256), 0, s, M, nnz, rp, ci,
-                           va, x, y, flags, crow, cval);
+                           va, x, y, flags, part, crow, cval);
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) {
             ws_free(ws, s);

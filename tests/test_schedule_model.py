@@ -180,3 +180,89 @@ def test_narrow_control_flow(power_law, ipt):
     expect[np.diff(rp) == 0] = 0
     assert np.array_equal(out, expect)
     assert np.all(written == 1)
+
+
+def scatter_partition(rp, chunk):
+    """k_merge_partition: row i (and the extra i = M) writes the boundaries w
+    with e(i-1) <= w·chunk < e(i), e(i) = rowptr[i+1] + i + 1."""
+    M, nnz = len(rp) - 1, int(rp[-1])
+    nwaves = -(-(M + nnz) // chunk)
+    part = np.full(nwaves + 1, -1, np.int64)
+    for i in range(M + 1):
+        lo = 0 if i == 0 else int(rp[i]) + i
+        hi = int(rp[i + 1]) + i + 1 if i < M else nwaves * chunk + 1
+        wb, we = -(-lo // chunk), min(-(-hi // chunk), nwaves + 1)
+        part[wb:we] = i
+    return part, nwaves
+
+
+def _matrices(power_law):
+    rp, _ = power_law
+    yield "rmat", rp
+    yield "empty_rows_at_ends", np.array([0, 0, 0, 5, 5, 9, 64, 64, 64], np.int64)
+    yield "one_dense_row", np.array([0, 4096], np.int64)
+    yield "no_nonzeros", np.zeros(300, np.int64)
+    lens = np.random.default_rng(5).integers(0, 3, 5000)  # many empty and short rows
+    yield "short_rows", np.concatenate([[0], np.cumsum(lens)])
+
+
+@pytest.mark.parametrize("chunk", [64, 256, 512, 1024])
+def test_scatter_partition_equals_the_search(power_law, chunk):
+    """Every boundary is written exactly once and equals the merge-path search
+    at diagonal min(w·chunk, M + nnz)."""
+    for name, rp in _matrices(power_law):
+        M, nnz = len(rp) - 1, int(rp[-1])
+        part, nwaves = scatter_partition(rp, chunk)
+        want = [merge_search(rp, min(w * chunk, M + nnz), M, nnz) for w in range(nwaves + 1)]
+        assert part.min() >= 0, name
+        assert np.array_equal(part, want), name
+
+
+def spmv_fixup_model(crow, cval):
+    """k_spmv_fixup: a wave per 64 carries, segmented suffix sums by row tag,
+    head lanes add their run; the wave holding the head of a run that goes on
+    past its lane 63 sums the rest of it."""
+    nw = len(crow)
+    y = {}
+    for base in range(0, nw, 64):
+        w = np.arange(base, base + 64)
+        r = np.where(w < nw, crow[np.minimum(w, nw - 1)], -2)
+        rprev = np.array([-2 if (x == 0 or x > nw) else crow[x - 1] for x in w])
+        s = np.where((w < nw) & (r >= 0), cval[np.minimum(w, nw - 1)], 0.0)
+        o = 1
+        while o < 64:
+            t = np.concatenate([s[o:], np.zeros(o)])
+            rt = np.concatenate([r[o:], np.full(o, -3)])
+            s = s + np.where(rt == r, t, 0.0)
+            o <<= 1
+        r63 = r[63]
+        owner = np.any((r == r63) & (rprev != r63))
+        ext = 0.0
+        if r63 >= 0 and owner and base + 64 < nw and crow[base + 64] == r63:
+            b = base + 64
+            while b < nw and crow[b] == r63:
+                ext += cval[b]
+                b += 1
+        for lane in range(64):
+            if r[lane] >= 0 and rprev[lane] != r[lane]:
+                y[r[lane]] = y.get(r[lane], 0.0) + s[lane] + (ext if r[lane] == r63 else 0.0)
+    return y
+
+
+def test_spmv_fixup_model_sums_every_run_once():
+    rng = np.random.default_rng(11)
+    crow, row = [], 0
+    for _ in range(400):  # runs of 1..300 carries, some waves without a carry
+        n = int(rng.choice([1, 1, 2, 3, 63, 64, 65, 130, 300]))
+        crow += [row] * n
+        if rng.random() < 0.3:
+            crow.append(-1)
+        row += int(rng.integers(1, 4))
+    crow = np.array(crow)
+    cval = rng.integers(-9, 10, len(crow)).astype(np.float64)
+    y = spmv_fixup_model(crow, cval)
+    want = {}
+    for r, v in zip(crow, cval):
+        if r >= 0:
+            want[r] = want.get(r, 0.0) + v
+    assert y == want
