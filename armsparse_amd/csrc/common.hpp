@@ -103,6 +103,15 @@ __device__ __forceinline__ void e4m3x4_to_f32(uint32_t w, float (&o)[4]) {
     o[3] = hi[1];
 }
 
+// x[l] + x[l ^ 32] in every lane l: v_permlane32_swap of x with itself gives
+// one copy whose upper half holds the lower lanes and one whose lower half holds
+// the upper lanes, so the two results always add to the pair's sum (a VALU op;
+// a 32-lane shuffle would go through the LDS crossbar).
+__device__ __forceinline__ float half_sum(float x) {
+    const auto r = __builtin_amdgcn_permlane32_swap(__float_as_uint(x), __float_as_uint(x), false, false);
+    return __uint_as_float(r[0]) + __uint_as_float(r[1]);
+}
+
 // Merge-path search: for diagonal d over (row ends rowptr[1..M], nnz 0..nnz-1)
 // returns the number of rows consumed; nnz consumed is d - result.
 __device__ __forceinline__ int merge_path_search(int d, const int* __restrict__ rowptr, int M, int nnz) {

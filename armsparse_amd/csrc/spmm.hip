@@ -130,6 +130,66 @@ __device__ __forceinline__ void finish(float (&o)[V], const float (&acc)[V], con
     }
 }
 
+// LDS of one wave of the staged merge kernels: per nonzero (row byte offset,
+// coefficient), per row its end relative to the chunk, and with a row-scale
+// epilogue per row its scale (so writing a row needs no memory load).
+__host__ __device__ inline size_t merge_wave_lds(int chunk, bool row_scales) {
+    return ((size_t)chunk * 8 + (size_t)(chunk + 1) * 4 * (row_scales ? 2 : 1) + 15) & ~(size_t)15;
+}
+
+// Writes one finished row of a merge kernel: acc·rs + bias (+ C when
+// accumulating), ReLU when the row is final.  `rs` is the row's scale (staged
+// in LDS), `bc` the lane's bias columns, loaded once per wave.  The C load and its use sit in one branch:
+// were the loaded value merged with a default after the branch, the wait for
+// it would run on every row and, the memory counter being in order, drain the
+// B gathers already in flight for the next rows.
+template <int V>
+__device__ __forceinline__ void write_row(float* cp, const float (&acc)[V], const Epi& ep, float rs,
+                                          const float (&bc)[V], bool acc_mode, bool final) {
+    float o[V];
+#pragma unroll
+    for (int v = 0; v < V; ++v) o[v] = acc[v];
+    if (ep.rscale) {
+#pragma unroll
+        for (int v = 0; v < V; ++v) o[v] *= rs;
+    }
+    if (ep.bias) {
+#pragma unroll
+        for (int v = 0; v < V; ++v) o[v] += bc[v];
+    }
+    const bool relu = final && ep.relu;
+    if (acc_mode) {
+        float cold[V];
+        load_c<V>(cp, cold);
+#pragma unroll
+        for (int v = 0; v < V; ++v) o[v] += cold[v];
+        if (relu) {
+#pragma unroll
+            for (int v = 0; v < V; ++v) o[v] = fmaxf(o[v], 0.0f);
+        }
+        store_c<V>(cp, o);
+    } else {
+        if (relu) {
+#pragma unroll
+            for (int v = 0; v < V; ++v) o[v] = fmaxf(o[v], 0.0f);
+        }
+        store_c<V>(cp, o);
+    }
+}
+
+// The lane's bias columns, loaded once per wave.  The empty asm statements
+// consume them, so their load completes here, before the gather loop: a load
+// still pending at the loop entry would otherwise make the compiler wait for
+// it (and, the memory counter being in order, for every gather) at each row.
+template <int V>
+__device__ __forceinline__ void load_bias(float (&bc)[V], const Epi& ep, int col, bool active) {
+#pragma unroll
+    for (int v = 0; v < V; ++v) {
+        bc[v] = (ep.bias && active) ? ep.bias[col + v] : 0.0f;
+        asm volatile("" ::"v"(bc[v]));
+    }
+}
+
 // ------------------------------------------------------------- ROW split
 template <typename TB, int V, int G>
 __global__ __launch_bounds__(256) void k_spmm_row(int M, int N, const int* __restrict__ rowptr,
@@ -398,15 +458,20 @@ __global__ __launch_bounds__(256) void k_spmm_stream(int M, int N, int nnz, int 
 
     // stage the chunk: (row byte offset, coefficient) per nonzero, end per row
     const uint32_t rowbytes = (uint32_t)ldb * (uint32_t)sizeof(TB);
-    const size_t wstride = ((size_t)chunk * 8 + (size_t)(chunk + 1) * 4 + 15) & ~(size_t)15;
+    const bool has_rs = ep.rscale != nullptr;
+    const size_t wstride = merge_wave_lds(chunk, has_rs);
     uint2* s_nz = reinterpret_cast<uint2*>(smem + wid * wstride);
     int* s_re = reinterpret_cast<int*>(smem + wid * wstride + (size_t)chunk * 8);
+    float* s_rs = reinterpret_cast<float*>(smem + wid * wstride + (size_t)chunk * 8 + (size_t)(chunk + 1) * 4);
     for (int k = lane; k < nz; k += 64) {
         const int c = __builtin_nontemporal_load(colind + j0 + k);
         const float a = coef(val ? __builtin_nontemporal_load(val + j0 + k) : 1.0f, ep, c);
         s_nz[k] = make_uint2((uint32_t)c * rowbytes, __float_as_uint(a));
     }
-    for (int k = lane; k < nr; k += 64) s_re[k] = (i0 + k < M) ? rowptr[i0 + k + 1] - j0 : INT_MAX;
+    for (int k = lane; k < nr; k += 64) {
+        s_re[k] = (i0 + k < M) ? rowptr[i0 + k + 1] - j0 : INT_MAX;
+        if (has_rs) s_rs[k] = (i0 + k < M) ? ep.rscale[i0 + k] : 1.0f;
+    }
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
@@ -426,15 +491,13 @@ __global__ __launch_bounds__(256) void k_spmm_stream(int M, int N, int nnz, int 
     int rr = 0;
     int re = uniform(s_re[0]);
     const int carry_in_row = (i0 < M && rowptr[i0] < j0) ? i0 : -1;
+    float bc[V];
+    load_bias<V>(bc, ep, col, active);
     auto emit = [&]() {
         const int r = i0 + rr;
-        if (active) {
-            float* cp = C + (size_t)r * ldc + col;
-            float o[V], cold[V];
-            if (acc_mode) load_c<V>(cp, cold);
-            finish<V>(o, acc, ep, r, col, cold, acc_mode, r != carry_in_row);
-            store_c<V>(cp, o);
-        }
+        if (active)
+            write_row<V>(C + (size_t)r * ldc + col, acc, ep, has_rs ? s_rs[rr] : 1.0f, bc, acc_mode,
+                         r != carry_in_row);
 #pragma unroll
         for (int v = 0; v < V; ++v) acc[v] = 0.0f;
         ++rr;
@@ -538,7 +601,7 @@ __global__ __launch_bounds__(256) void k_spmm_slice(int M, int N, int nnz, int c
         g = (int)(blockIdx.x / (unsigned)S);
     }
     if (g >= G) return;
-    const int wave = g * 4 + wid;
+    const int wave = uniform(g * 4 + wid);  // keeps every chunk bound and loop test scalar
     const int64_t total = (int64_t)M + nnz;
     const int64_t d0l = (int64_t)wave * chunk;
     if (d0l >= total) return;
@@ -551,15 +614,20 @@ __global__ __launch_bounds__(256) void k_spmm_slice(int M, int N, int nnz, int c
     const int nr = i1 - i0 + 1;
 
     const uint32_t rowbytes = (uint32_t)ldb * (uint32_t)sizeof(TB);
-    const size_t wstride = ((size_t)chunk * 8 + (size_t)(chunk + 1) * 4 + 15) & ~(size_t)15;
+    const bool has_rs = ep.rscale != nullptr;
+    const size_t wstride = merge_wave_lds(chunk, has_rs);
     uint2* s_nz = reinterpret_cast<uint2*>(smem + wid * wstride);
     int* s_re = reinterpret_cast<int*>(smem + wid * wstride + (size_t)chunk * 8);
+    float* s_rs = reinterpret_cast<float*>(smem + wid * wstride + (size_t)chunk * 8 + (size_t)(chunk + 1) * 4);
     for (int k = lane; k < nz; k += 64) {
         const int c = __builtin_nontemporal_load(colind + j0 + k);
         const float a = coef(val ? __builtin_nontemporal_load(val + j0 + k) : 1.0f, ep, c);
         s_nz[k] = make_uint2((uint32_t)c * rowbytes, __float_as_uint(a));
     }
-    for (int k = lane; k < nr; k += 64) s_re[k] = (i0 + k < M) ? rowptr[i0 + k + 1] - j0 : INT_MAX;
+    for (int k = lane; k < nr; k += 64) {
+        s_re[k] = (i0 + k < M) ? rowptr[i0 + k + 1] - j0 : INT_MAX;
+        if (has_rs) s_rs[k] = (i0 + k < M) ? ep.rscale[i0 + k] : 1.0f;
+    }
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
@@ -577,18 +645,16 @@ __global__ __launch_bounds__(256) void k_spmm_slice(int M, int N, int nnz, int c
     int rr = 0;
     int re = uniform(s_re[0]);
     const int carry_in_row = (i0 < M && rowptr[i0] < j0) ? i0 : -1;
+    float bc[V];
+    load_bias<V>(bc, ep, col, active);
     auto emit = [&]() {  // called by the whole wave
         const int r = i0 + rr;
         float sum[V];
 #pragma unroll
-        for (int v = 0; v < V; ++v) sum[v] = acc[v] + __shfl_xor(acc[v], 32);
-        if (h == 0 && active) {
-            float* cp = C + (size_t)r * ldc + col;
-            float o[V], cold[V];
-            if (acc_mode) load_c<V>(cp, cold);
-            finish<V>(o, sum, ep, r, col, cold, acc_mode, r != carry_in_row);
-            store_c<V>(cp, o);
-        }
+        for (int v = 0; v < V; ++v) sum[v] = half_sum(acc[v]);
+        if (h == 0 && active)
+            write_row<V>(C + (size_t)r * ldc + col, sum, ep, has_rs ? s_rs[rr] : 1.0f, bc, acc_mode,
+                         r != carry_in_row);
 #pragma unroll
         for (int v = 0; v < V; ++v) acc[v] = 0.0f;
         ++rr;
@@ -671,7 +737,7 @@ __global__ __launch_bounds__(256) void k_spmm_slice(int M, int N, int nnz, int c
     if (has) {
         float sum[V];
 #pragma unroll
-        for (int v = 0; v < V; ++v) sum[v] = acc[v] + __shfl_xor(acc[v], 32);
+        for (int v = 0; v < V; ++v) sum[v] = half_sum(acc[v]);
         if (h == 0 && active) {
             float* cp = carry_val + (size_t)wave * N + col;
 #pragma unroll
@@ -854,9 +920,7 @@ __global__ __launch_bounds__(256) void k_spmm_narrow(int M, int N, int nnz, cons
     }
 }
 
-inline size_t stream_lds_bytes(int chunk) {
-    return 4 * (((size_t)chunk * 8 + (size_t)(chunk + 1) * 4 + 15) & ~(size_t)15);
-}
+inline size_t stream_lds_bytes(int chunk, bool row_scales) { return 4 * merge_wave_lds(chunk, row_scales); }
 
 // Adds each run of carries (consecutive waves cut inside the same row) into
 // that row: a wave per carry, and the one that heads a run sums it.  One load
@@ -1001,7 +1065,7 @@ template <typename TB, int V>
 int launch_stream_vu(int U, dim3 grid, hipStream_t s, int M, int N, int nnz, int chunk, const int* rp, const int* ci,
                      const float* va, const TB* B, int ldb, const Epi& bs, float* C, int ldc, int flags,
                      const int* part, int* crow, float* cval) {
-    const size_t lds = stream_lds_bytes(chunk);
+    const size_t lds = stream_lds_bytes(chunk, bs.rscale != nullptr);
     switch (U) {
 #define ARMSP_STREAM_CASE(UU)                                                                                  \
     case UU:                                                                                                   \
@@ -1021,7 +1085,7 @@ template <typename TB, int V>
 int launch_slice_v(int U, int S, int G, hipStream_t s, int M, int N, int nnz, int chunk, const int* rp,
                    const int* ci, const float* va, const TB* B, int ldb, const Epi& bs, float* C, int ldc, int flags,
                    const int* part, int* crow, float* cval) {
-    const size_t lds = stream_lds_bytes(chunk);
+    const size_t lds = stream_lds_bytes(chunk, bs.rscale != nullptr);
     const int64_t nblk = (8 % S == 0) ? ((int64_t)(G + 8 / S - 1) / (8 / S)) * 8 : (int64_t)G * S;
     if (nblk > INT_MAX) return ARMSP_ERROR_NOT_SUPPORTED;
     const dim3 grid((unsigned)nblk);
