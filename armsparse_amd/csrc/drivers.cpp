@@ -10,6 +10,13 @@
 //   balanceNUMAAwareSPDM M K N      (numaAware/balanceNUMAAwareSPDM.cpp)
 //   numaAwareSpMV M K threads       (numaAware/numaAwareSpMV.cpp:126-)
 //   numaAwareSpMM M K N threads     (numaAware/numaAwareSpMM.cpp)
+//   suitesparse_sdmm threads [file] (suitesparse_sdmm.cpp:63-147: rowBaseSpmm, N = 1024)
+//   suitesparse_spmm threads [file] (suitesparse_spmm.cpp:90-172: sparse_matrix_mul3, A·A)
+//   suitesparse_spmv threads [file] (suitesparse_spmv.cpp:73-132: blocked-CSR SpMV, x = 5)
+//
+// The suitesparse drivers read a Matrix Market file (default: the reference's
+// hard-coded datasets/ path) as the reference does — stored entries only, a
+// later entry overwriting an earlier one, zeros dropped — and print its lines.
 //
 // Inputs come from the seeded CSR-native generators instead of
 // srand(time(NULL)) + a dense M×K array (the reference's densities and value
@@ -241,6 +248,102 @@ int run_spmv(int M, int K, double ratio, float threshold, bool numa_lines) {
     return EXIT_SUCCESS;
 }
 
+// ------------------------------------------------------ Matrix Market inputs
+Csr read_mtx_file(const char* path, int* declared) {
+    Csr a;
+    int64_t nnz = 0;
+    int st = armsp_read_mtx(path, ARMSP_MTX_STORED_TRIANGLE, &a.M, &a.K, &nnz, nullptr, nullptr, nullptr);
+    if (st) {
+        std::fprintf(stderr, "armsparse: %s: Matrix Market read error %d\n", path, st);
+        std::exit(EXIT_FAILURE);
+    }
+    if (nnz > 0x7fffffff) die("matrix too large");
+    a.rp.resize(a.M + 1);
+    a.ci.resize(nnz);
+    a.va.resize(nnz);
+    st = armsp_read_mtx(path, ARMSP_MTX_STORED_TRIANGLE, &a.M, &a.K, &nnz, a.rp.data(), a.ci.data(), a.va.data());
+    if (st) die("Matrix Market read failed");
+    // the header's count, which the reference prints
+    *declared = (int)nnz;
+    if (FILE* f = std::fopen(path, "r")) {
+        char line[4096];
+        while (std::fgets(line, sizeof line, f))
+            if (line[0] != '%') {
+                long long m = 0, k = 0, n = 0;
+                if (std::sscanf(line, "%lld %lld %lld", &m, &k, &n) == 3) *declared = (int)n;
+                break;
+            }
+        std::fclose(f);
+    }
+    return a;
+}
+
+int run_suitesparse(const std::string& name, const char* path) {
+    int declared = 0;
+    Csr a = read_mtx_file(path, &declared);
+    std::printf("%d %d %d\n", a.M, a.K, declared);
+    int *rp = upload(a.rp), *ci = upload(a.ci);
+    float* va = upload(a.va);
+    const int ite = name == "suitesparse_spmm" ? 10 : 30;
+    if (name == "suitesparse_sdmm") {  // rowBaseSpmm: whole groups of 4 nonzeros and columns
+        const int N = 1024;
+        std::vector<float> B = dense_uniform((int64_t)a.K * N, 0xA5A50000ULL, 0.0f, 10.0f);
+        float* dB = upload(B);
+        float* dC = nullptr;
+        hip_check(hipMalloc(&dC, std::max<size_t>((size_t)a.M * N, 1) * 4), "hipMalloc");
+        hip_check(hipMemset(dC, 0, (size_t)a.M * N * 4), "memset");
+        armsp_config row{ARMSP_SCHED_ROW, 0, 0, 0, 0};
+        const double t = time_kernel([&] {
+            st_check(armsp_spmm_f32(a.M, a.K, N, a.nnz(), rp, ci, va, dB, N, dC, N, ARMSP_ACCUMULATE | ARMSP_TRUNC4,
+                                    &row, nullptr), "spmm");
+        }, ite);
+        std::printf("m k n ratio:%d %d %d  nnz:%d\n", a.M, a.K, N, declared);
+        std::printf("rowBaseSpmmTime %g\n", t);
+        hipFree(dB);
+        hipFree(dC);
+    } else if (name == "suitesparse_spmm") {  // A·A into a dense C, then toCSR of C
+        const int N = a.M;
+        if (a.K != a.M) die("suitesparse_spmm multiplies the matrix by itself: it must be square");
+        float* dC = nullptr;
+        hip_check(hipMalloc(&dC, std::max<size_t>((size_t)a.M * N, 1) * 4), "hipMalloc");
+        hip_check(hipMemset(dC, 0, (size_t)a.M * N * 4), "memset");
+        const double t = time_kernel([&] {
+            st_check(armsp_spgemm_dense_f32(a.M, a.K, N, a.nnz(), rp, ci, va, a.nnz(), rp, ci, va, dC, N, nullptr),
+                     "spgemm");
+        }, ite);
+        std::vector<float> C((size_t)a.M * N);
+        hip_check(hipMemcpy(C.data(), dC, C.size() * 4, hipMemcpyDeviceToHost), "download");
+        const double t0 = now();
+        int64_t cnnz = 0;
+        for (float v : C) cnnz += v != 0.0f;
+        const double tocsr = now() - t0;
+        std::printf("tocsrTime: %g\n", tocsr);
+        std::printf("m k n ratio:%d %d %d  nnz:%d\n", a.M, a.K, N, declared);
+        std::printf("SpMMTime %g\n", t);
+        std::printf("C nnz:%lld\n", (long long)cnnz);
+        hipFree(dC);
+    } else {  // blocked CSR (toVectorAwareCSR), x = 5
+        std::vector<int> vrp(a.M + 1);
+        const int64_t np = armsp_csr_to_vcsr(a.M, a.K, a.rp.data(), a.ci.data(), a.va.data(), 0.0f, vrp.data(),
+                                             nullptr, nullptr);
+        std::vector<int> vci(np);
+        std::vector<float> vva(np);
+        armsp_csr_to_vcsr(a.M, a.K, a.rp.data(), a.ci.data(), a.va.data(), 0.0f, vrp.data(), vci.data(), vva.data());
+        const int Kp = (a.K + 3) / 4 * 4;
+        std::vector<float> x(Kp, 5.0f), y(a.M, 0.0f);
+        int *vr = upload(vrp), *vc = upload(vci);
+        float *vv = upload(vva), *dx = upload(x), *dy = upload(y);
+        std::printf("123");
+        const double t = time_kernel(
+            [&] { st_check(armsp_spmv_vcsr_f32(a.M, (int)np, vr, vc, vv, dx, dy, 0, nullptr), "spmv"); }, ite);
+        std::printf("m k: %d %d \n", a.M, a.K);
+        std::printf("time take: %g\n", t);
+        hipFree(vr); hipFree(vc); hipFree(vv); hipFree(dx); hipFree(dy);
+    }
+    hipFree(rp); hipFree(ci); hipFree(va);
+    return EXIT_SUCCESS;
+}
+
 std::string base_name(const char* p) {
     const char* s = std::strrchr(p, '/');
     return s ? s + 1 : p;
@@ -282,6 +385,14 @@ int main(int argc, char** argv) {
         arg(argc, argv, 3, u);
         return run_spmv(M, K, 0.6, name == "SpMV" ? 0.0f : 0.1f, name != "SpMV");
     }
+    if (name == "suitesparse_sdmm" || name == "suitesparse_spmm" || name == "suitesparse_spmv") {
+        const char* u = "usage: suitesparse_{sdmm,spmm,spmv} threads [file.mtx]";
+        arg(argc, argv, 1, u);
+        const char* dflt = name == "suitesparse_sdmm" ? "datasets/wang3.mtx"
+                           : name == "suitesparse_spmm" ? "datasets/big.mtx"
+                                                        : "datasets/mouse_gene.mtx";
+        return run_suitesparse(name, argc > 2 ? argv[2] : dflt);
+    }
     die("unknown driver name (SDMM, SpMM, SpMV, numaAwareSDMM, numaAwareSPDM, balanceNUMAAwareSPDM, "
-        "numaAwareSpMV, numaAwareSpMM)");
+        "numaAwareSpMV, numaAwareSpMM, suitesparse_sdmm, suitesparse_spmm, suitesparse_spmv)");
 }

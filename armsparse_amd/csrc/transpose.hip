@@ -1,4 +1,4 @@
-// Stable CSR → CSC transpose for gfx950.
+// Stable CSR → CSC transpose and COO → CSR builder for gfx950.
 //
 // Reference: toCSC's column-major scan (SpMM.cpp:80-95) and scipy's tocsc in
 // the GCN script (gcn_custom.py:48-55) produce, for every column, its entries
@@ -9,6 +9,13 @@
 // scatter whose in-block ranks come from wave ballots in item order), so rows
 // stay ascending inside each column.  The last pass gathers row ids and
 // values; colptr is read off the sorted column keys.
+//
+// COO → CSR (the canonical CSR of toCSR, SDMM.cpp:355-370 / SpMM.cpp:64-79,
+// built from unordered triplets on the device): the same stable sort by
+// column, then by row, so entries end in (row, column) order with duplicates
+// in input order; duplicates are summed in that order, entries whose sum is
+// zero are dropped on request (toCSR keeps only `!= 0`), and rowptr is read off
+// the sorted rows.
 #include <climits>
 
 #include "common.hpp"
@@ -151,6 +158,66 @@ __global__ void k_colptr(int nnz, int K, const int* __restrict__ keys, int* __re
     for (int c = kp + 1; c <= k; ++c) colptr[c] = (int)p;
 }
 
+// rowptr[r] = first position whose row is ≥ r, for the *n_dev sorted rows.
+__global__ void k_rowptr_dev(const int* __restrict__ n_dev, int M, const int* __restrict__ keys,
+                             int* __restrict__ rowptr) {
+    const int n = *n_dev;
+    const int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (p > n) return;
+    const int k = p < n ? keys[p] : M;
+    const int kp = p > 0 ? keys[p - 1] : -1;
+    for (int c = kp + 1; c <= k; ++c) rowptr[c] = (int)p;
+}
+
+__global__ void k_coo_check(int nnz, int M, int K, const int* __restrict__ rows, const int* __restrict__ cols,
+                            int* __restrict__ bad) {
+    const int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (p >= nnz) return;
+    const int r = rows[p], c = cols[p];
+    if (r < 0 || r >= M || c < 0 || c >= K) atomicOr(bad, 1);
+}
+
+__global__ void k_gather_keys(int n, const int* __restrict__ idx, const int* __restrict__ src,
+                              int* __restrict__ dst) {
+    const int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (p < n) dst[p] = src[idx[p]];
+}
+
+// keep[p] = 1 where p heads a run of equal (row, column) and the run is kept;
+// sums[p] = the run's values added in order (runs of duplicates are short).
+__global__ void k_coo_runs(int n, const int* __restrict__ srow, const int* __restrict__ scol,
+                           const float* __restrict__ sval, float* __restrict__ sums, int* __restrict__ keep,
+                           int drop_zeros) {
+    const int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    const int r = srow[p], c = scol[p];
+    const bool head = p == 0 || srow[p - 1] != r || scol[p - 1] != c;
+    int k = 0;
+    if (head) {
+        float s = sval ? sval[p] : 1.0f;
+        for (int64_t q = p + 1; q < n && srow[q] == r && scol[q] == c; ++q)
+            if (sval) s += sval[q];
+        if (sval) sums[p] = s;
+        k = (!drop_zeros || !sval || s != 0.0f) ? 1 : 0;
+    }
+    keep[p] = k;
+}
+
+__global__ void k_coo_compact(int n, const int* __restrict__ srow, const int* __restrict__ scol,
+                              const float* __restrict__ sums, const int* __restrict__ keep,
+                              const int* __restrict__ pos, int* __restrict__ urow, int* __restrict__ colind,
+                              float* __restrict__ val, int* __restrict__ n_out) {
+    const int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    if (keep[p]) {
+        const int d = pos[p];
+        urow[d] = srow[p];
+        colind[d] = scol[p];
+        if (val) val[d] = sums[p];
+    }
+    if (p == n - 1) *n_out = pos[p] + keep[p];
+}
+
 int64_t scan_ws_ints(int64_t n) {
     int64_t total = 0;
     while (n > 1) {
@@ -167,6 +234,37 @@ void exclusive_scan(int64_t n, int* data, int* ws, hipStream_t s) {
         exclusive_scan(nb, ws, ws + nb, s);
         hipLaunchKernelGGL(k_scan_add, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, n, data, ws);
     }
+}
+
+// One stable LSD sort of keys (values in [0, range)) carrying positions:
+// idx_in == nullptr sorts positions 0..n-1.  Non-final: sorted keys and
+// positions land in *kout_final / *iout_final (ping-pong buffers A/B).  Final
+// (gather_src != nullptr): the last pass writes gathered[dst] = gather_src[pos]
+// and gvals[dst] = vals[pos] instead of positions.
+void radix_sort(int n, int range, const int* keys_in, const int* idx_in, int* keysA, int* keysB, int* idxA,
+                int* idxB, int* hist, int* scanws, const int* gather_src, const float* vals, int* gathered,
+                float* gvals, const int** kout_final, const int** iout_final, hipStream_t s) {
+    const int nblocks = (int)(((int64_t)n + TILE - 1) / TILE);
+    const int64_t nhist = (int64_t)NB * nblocks;
+    int passes = 1;
+    while (range > 1 && ((range - 1) >> (8 * passes)) > 0) ++passes;
+    const int* kin = keys_in;
+    const int* iin = idx_in;
+    int* kout = keysA;
+    int* iout = idxA;
+    for (int pass = 0; pass < passes; ++pass) {
+        const bool last = gather_src && pass == passes - 1;
+        hipLaunchKernelGGL(k_radix_hist, dim3(nblocks), dim3(TPB), 0, s, n, 8 * pass, kin, hist, nblocks);
+        exclusive_scan(nhist, hist, scanws, s);
+        hipLaunchKernelGGL(k_radix_scatter, dim3(nblocks), dim3(TPB), 0, s, n, 8 * pass, kin, iin, hist, nblocks,
+                           kout, iout, gather_src, vals, gathered, gvals, last ? 1 : 0);
+        kin = kout;
+        iin = iout;
+        kout = (kout == keysA) ? keysB : keysA;
+        iout = (iout == idxA) ? idxB : idxA;
+    }
+    *kout_final = kin;
+    *iout_final = iin;
 }
 
 }  // namespace
@@ -189,8 +287,6 @@ extern "C" int armsp_csr_transpose(int M, int K, int nnz, const int* rowptr, con
     const int nblocks = (int)(((int64_t)nnz + TILE - 1) / TILE);
     const int64_t nhist = (int64_t)NB * nblocks;
     if (nhist > INT_MAX) return ARMSP_ERROR_NOT_SUPPORTED;
-    int passes = 1;
-    while (K > 1 && ((K - 1) >> (8 * passes)) > 0) ++passes;
     const size_t n4 = ((size_t)nnz * 4 + 255) & ~(size_t)255;
     const size_t h4 = ((size_t)nhist * 4 + 255) & ~(size_t)255;
     const size_t s4 = ((size_t)scan_ws_ints(nhist) * 4 + 255) & ~(size_t)255;
@@ -207,26 +303,108 @@ extern "C" int armsp_csr_transpose(int M, int K, int nnz, const int* rowptr, con
     int* scanws = reinterpret_cast<int*>(p + 5 * n4 + h4);
 
     hipLaunchKernelGGL(k_expand_rows, dim3((unsigned)((M + 3) / 4)), dim3(256), 0, s, M, rowptr, rows);
-    const int* kin = colind;
-    const int* iin = nullptr;
-    int* kout = keysA;
-    int* iout = idxA;
-    for (int pass = 0; pass < passes; ++pass) {
-        const int shift = 8 * pass;
-        const bool last = pass == passes - 1;
-        hipLaunchKernelGGL(k_radix_hist, dim3(nblocks), dim3(TPB), 0, s, nnz, shift, kin, hist, nblocks);
-        exclusive_scan(nhist, hist, scanws, s);
-        hipLaunchKernelGGL(k_radix_scatter, dim3(nblocks), dim3(TPB), 0, s, nnz, shift, kin, iin, hist, nblocks,
-                           kout, iout, rows, val, rowind, valT, last ? 1 : 0);
-        kin = kout;
-        iin = iout;
-        kout = (kout == keysA) ? keysB : keysA;
-        iout = (iout == idxA) ? idxB : idxA;
-    }
-    const int* sorted_keys = kin;
+    const int* sorted_keys = nullptr;
+    const int* sorted_idx = nullptr;
+    radix_sort(nnz, K, colind, nullptr, keysA, keysB, idxA, idxB, hist, scanws, rows, val, rowind, valT,
+               &sorted_keys, &sorted_idx, s);
     hipLaunchKernelGGL(k_colptr, dim3((unsigned)(((int64_t)nnz + 1 + 255) / 256)), dim3(256), 0, s, nnz, K,
                        sorted_keys, colptr);
     hipError_t e = hipGetLastError();
+    ws_free(ws, s);
+    if (e != hipSuccess) return hip_fail(e);
+    t.stop();
+    return ARMSP_SUCCESS;
+}
+
+extern "C" int armsp_coo_to_csr(int M, int K, int nnz, const int* rows, const int* cols, const float* vals,
+                                int drop_zeros, int* rowptr, int* colind, float* val, int* nnz_out, void* stream) {
+    using namespace armsp;
+    if (M < 0 || K < 0 || nnz < 0) return ARMSP_ERROR_INVALID_VALUE;
+    if (!rowptr || !nnz_out || (nnz > 0 && (!rows || !cols || !colind))) return ARMSP_ERROR_INVALID_VALUE;
+    if (nnz > 0 && (vals == nullptr) != (val == nullptr)) return ARMSP_ERROR_INVALID_VALUE;
+    if (nnz > 0 && (M == 0 || K == 0)) return ARMSP_ERROR_INVALID_VALUE;
+    hipStream_t s = as_stream(stream);
+    TimedLaunch t("coo_to_csr", s);
+    const int nblocks = (int)(((int64_t)nnz + TILE - 1) / TILE);
+    const int64_t nhist = (int64_t)NB * (nblocks > 0 ? nblocks : 1);
+    if (nhist > INT_MAX) return ARMSP_ERROR_NOT_SUPPORTED;
+    const int64_t nscan = nhist > nnz ? nhist : (int64_t)nnz;
+    const size_t n4 = ((size_t)(nnz > 0 ? nnz : 1) * 4 + 255) & ~(size_t)255;
+    const size_t h4 = ((size_t)nhist * 4 + 255) & ~(size_t)255;
+    const size_t s4 = ((size_t)scan_ws_ints(nscan) * 4 + 255) & ~(size_t)255;
+    void* ws = nullptr;
+    int st = ws_alloc(&ws, 11 * n4 + h4 + s4 + 256, s);
+    if (st) return st;
+    char* p = static_cast<char*>(ws);
+    int* keysA = reinterpret_cast<int*>(p);
+    int* keysB = reinterpret_cast<int*>(p + n4);
+    int* idxA = reinterpret_cast<int*>(p + 2 * n4);
+    int* idxB = reinterpret_cast<int*>(p + 3 * n4);
+    int* rperm = reinterpret_cast<int*>(p + 4 * n4);
+    int* scol = reinterpret_cast<int*>(p + 5 * n4);
+    float* sval = reinterpret_cast<float*>(p + 6 * n4);
+    float* sums = reinterpret_cast<float*>(p + 7 * n4);
+    int* keep = reinterpret_cast<int*>(p + 8 * n4);
+    int* pos = reinterpret_cast<int*>(p + 9 * n4);
+    int* urow = reinterpret_cast<int*>(p + 10 * n4);
+    int* hist = reinterpret_cast<int*>(p + 11 * n4);
+    int* scanws = reinterpret_cast<int*>(p + 11 * n4 + h4);
+    int* bad = reinterpret_cast<int*>(p + 11 * n4 + h4 + s4);
+    auto fail = [&](int code) {
+        ws_free(ws, s);
+        return code;
+    };
+    if (nnz == 0) {
+        hipError_t e = hipMemsetAsync(nnz_out, 0, sizeof(int), s);
+        if (e == hipSuccess) {
+            hipLaunchKernelGGL(k_rowptr_dev, dim3(1), dim3(64), 0, s, (const int*)nnz_out, M, (const int*)nullptr,
+                               rowptr);
+            e = hipGetLastError();
+        }
+        if (e != hipSuccess) return fail(hip_fail(e));
+        ws_free(ws, s);
+        t.stop();
+        return ARMSP_SUCCESS;
+    }
+    // every index in range, or an error before anything is written through one
+    int bad_h = 0;
+    hipError_t e = hipMemsetAsync(bad, 0, sizeof(int), s);
+    const unsigned g1 = (unsigned)(((int64_t)nnz + 255) / 256);
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(k_coo_check, dim3(g1), dim3(256), 0, s, nnz, M, K, rows, cols, bad);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipMemcpyAsync(&bad_h, bad, sizeof(int), hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e != hipSuccess) return fail(hip_fail(e));
+    if (bad_h) return fail(ARMSP_ERROR_INVALID_VALUE);
+
+    // by column, then (stably) by row: (row, column) order, duplicates in input order
+    const int* ck = nullptr;
+    const int* ci = nullptr;
+    radix_sort(nnz, K, cols, nullptr, keysA, keysB, idxA, idxB, hist, scanws, nullptr, nullptr, nullptr, nullptr,
+               &ck, &ci, s);
+    hipLaunchKernelGGL(k_gather_keys, dim3(g1), dim3(256), 0, s, nnz, ci, rows, rperm);
+    // the column pass left its positions in idxA or idxB; sort rows into the other pair
+    int* k2a = (ci == idxA) ? keysA : keysB;
+    int* k2b = (ci == idxA) ? keysB : keysA;
+    int* i2a = (ci == idxA) ? idxB : idxA;
+    int* i2b = (ci == idxA) ? idxA : idxB;
+    (void)i2b;
+    const int* srow = nullptr;
+    const int* si = nullptr;
+    radix_sort(nnz, M, rperm, ci, k2a, k2b, i2a, const_cast<int*>(ci), hist, scanws, cols, vals, scol,
+               vals ? sval : nullptr, &srow, &si, s);
+    hipLaunchKernelGGL(k_coo_runs, dim3(g1), dim3(256), 0, s, nnz, srow, scol, vals ? sval : nullptr, sums, keep,
+                       drop_zeros);
+    e = hipMemcpyAsync(pos, keep, (size_t)nnz * sizeof(int), hipMemcpyDeviceToDevice, s);
+    if (e != hipSuccess) return fail(hip_fail(e));
+    exclusive_scan(nnz, pos, scanws, s);
+    hipLaunchKernelGGL(k_coo_compact, dim3(g1), dim3(256), 0, s, nnz, srow, scol, sums, keep, pos, urow, colind,
+                       val, nnz_out);
+    hipLaunchKernelGGL(k_rowptr_dev, dim3((unsigned)(((int64_t)nnz + 1 + 255) / 256)), dim3(256), 0, s,
+                       (const int*)nnz_out, M, urow, rowptr);
+    e = hipGetLastError();
     ws_free(ws, s);
     if (e != hipSuccess) return hip_fail(e);
     t.stop();

@@ -8,6 +8,7 @@ GCN scripts (gcn/pytorch-custom/gcn_custom.py:19-58), which needs a download.
 from __future__ import annotations
 
 import ctypes
+import os
 from dataclasses import dataclass
 from typing import Optional
 
@@ -110,6 +111,26 @@ def csr_check(a: CSR) -> int:
 
 
 # ------------------------------------------------------------ workloads
+def read_matrix_market(path: str, mirror: bool = True) -> CSR:
+    """Matrix Market coordinate file → canonical CSR (armsp_read_mtx): a later
+    entry for a position overwrites an earlier one, zeros are dropped, pattern
+    entries are 1.0, symmetric files are mirrored unless mirror=False."""
+    flags = 0 if mirror else 0x1  # ARMSP_MTX_STORED_TRIANGLE
+    m, k, n = ctypes.c_int(0), ctypes.c_int(0), ctypes.c_int64(0)
+    p = os.fsencode(path)
+    st = lib.armsp_read_mtx(p, flags, ctypes.byref(m), ctypes.byref(k), ctypes.byref(n), None, None, None)
+    if st:
+        raise ValueError(f"armsp_read_mtx({path!r}): error {st}")
+    rowptr = np.empty(m.value + 1, np.int32)
+    colind = np.empty(n.value, np.int32)
+    values = np.empty(n.value, np.float32)
+    st = lib.armsp_read_mtx(p, flags, ctypes.byref(m), ctypes.byref(k), ctypes.byref(n), _p(rowptr), _p(colind),
+                            _p(values))
+    if st:
+        raise ValueError(f"armsp_read_mtx({path!r}): error {st}")
+    return CSR(rowptr, colind, values, (m.value, k.value))
+
+
 def spmm_small(seed: int = CFG_SEED + 0):
     """4096×4096 uniform 1% (167,772 nnz), B 4096×64 U(-1,1), x[4096] U(-1,1)."""
     A = uniform_csr(4096, 4096, 167_772, seed)

@@ -278,6 +278,66 @@ def csr_transpose(rowptr: torch.Tensor, colind: torch.Tensor, values: Optional[t
     return tuple(None if x is None else x.cpu() for x in res) if host else res
 
 
+MAX_INIT = -10000.0  # max_init() of gcn/dgl-custom/binary_reduce_max.cu:22-24
+
+
+def spmm_max(rowptr: torch.Tensor, colind: torch.Tensor, values: Optional[torch.Tensor], dense: torch.Tensor,
+             init: float = MAX_INIT, return_argmax: bool = False):
+    """out[r, j] = max(init, max_p values[p]·dense[colind[p], j]) over row r's
+    nonzeros (values None: dense rows as they are), the DGL max reducer; with
+    return_argmax also the winning nonzero positions (int32, -1 where init won)."""
+    fn = "spmm_max"
+    _check_csr(fn, rowptr, colind, values)
+    _require(dense.dim() == 2 and dense.dtype == torch.float32, f"{fn}: dense must be 2-D float32")
+    _require(dense.is_cuda and rowptr.is_cuda, f"{fn}: operands must be device tensors")
+    m, nnz = rowptr.numel() - 1, colind.numel()
+    k, n = dense.shape
+    d = dense.contiguous()
+    out = torch.empty((m, n), device=dense.device)
+    arg = torch.empty((m, n), dtype=torch.int32, device=dense.device) if return_argmax else None
+    call("armsp_spmm_max_f32", m, k, n, nnz, _ptr(rowptr), _ptr(colind), _ptr(values), _ptr(d), n, float(init),
+         _ptr(out), n, _ptr(arg), _stream(dense.device))
+    return (out, arg) if return_argmax else out
+
+
+def spmm_max_backward(colind: torch.Tensor, values: Optional[torch.Tensor], grad_out: torch.Tensor,
+                      argmax: torch.Tensor, num_src: int) -> torch.Tensor:
+    """d dense of spmm_max: grad_out[r, j]·values[p] added to row colind[p] for p = argmax[r, j]."""
+    fn = "spmm_max_backward"
+    _require(argmax.shape == grad_out.shape and argmax.dtype == torch.int32, f"{fn}: argmax must match grad_out")
+    m, n = grad_out.shape
+    g = grad_out.contiguous().float()
+    db = torch.zeros((num_src, n), device=grad_out.device)
+    call("armsp_spmm_max_backward_f32", m, num_src, n, colind.numel(), _ptr(colind), _ptr(values), _ptr(g), n,
+         _ptr(argmax.contiguous()), _ptr(db), n, _stream(grad_out.device))
+    return db
+
+
+def coo_to_csr(rows: torch.Tensor, cols: torch.Tensor, values: Optional[torch.Tensor], num_rows: int,
+               num_cols: int, drop_zeros: bool = False):
+    """(rowptr, colind, values): canonical CSR of unordered COO triplets built on
+    the device — (row, column) order, duplicates summed in input order, zero
+    entries dropped when drop_zeros (toCSR's `!= 0` rule)."""
+    fn = "coo_to_csr"
+    _require(rows.dtype == torch.int32 and cols.dtype == torch.int32, f"{fn}: indices must be int32")
+    _require(rows.dim() == 1 and rows.shape == cols.shape, f"{fn}: rows and cols must be 1-D of one length")
+    _require(values is None or (values.dtype == torch.float32 and values.shape == rows.shape),
+             f"{fn}: values must be float32 of the same length")
+    nnz = rows.numel()
+    host = not rows.is_cuda
+    dev = _device() if host else rows.device
+    r, c, v = (None if t is None else _on_device(t, dev).contiguous() for t in (rows, cols, values))
+    rowptr = torch.empty(num_rows + 1, dtype=torch.int32, device=dev)
+    colind = torch.empty(max(nnz, 1), dtype=torch.int32, device=dev)
+    val = torch.empty(max(nnz, 1), dtype=torch.float32, device=dev) if values is not None else None
+    nout = torch.empty(1, dtype=torch.int32, device=dev)
+    call("armsp_coo_to_csr", num_rows, num_cols, nnz, _ptr(r), _ptr(c), _ptr(v), int(bool(drop_zeros)),
+         _ptr(rowptr), _ptr(colind), _ptr(val), _ptr(nout), _stream(dev))
+    n = int(nout.item())
+    res = (rowptr, colind[:n], val[:n] if val is not None else None)
+    return tuple(None if x is None else x.cpu() for x in res) if host else res
+
+
 # ------------------------------------------------------- dense helpers
 def gemm_bf16(a: torch.Tensor, b: torch.Tensor, *, out: Optional[torch.Tensor] = None,
               accumulate: bool = False, out_dtype: torch.dtype = torch.float32) -> torch.Tensor:

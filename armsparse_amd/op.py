@@ -66,6 +66,30 @@ class SPMMFunction(torch.autograd.Function):
         return None, None, None, None, grad_feat, grad_ew, None
 
 
+class SPMMMaxFunction(torch.autograd.Function):
+    """out[r] = max(init, max over row r's edges of w[e]·feat[colind[e]]) per
+    column: the max reducer of the reference's DGL path
+    (gcn/dgl-custom/binary_reduce_max.cu:151-166, init -10000 as its
+    max_init()).  Backward routes each output's gradient to the feature row
+    that won (no edge-weight gradient)."""
+
+    @staticmethod
+    def forward(ctx, rowptr, colind, feat, edge_weight=None, init=kernels.MAX_INIT):
+        out, arg = kernels.spmm_max(rowptr, colind, edge_weight, feat, init=init, return_argmax=True)
+        ctx.save_for_backward(colind, arg)
+        ctx.edge_weight = edge_weight
+        ctx.num_src = feat.shape[0]
+        return out
+
+    @staticmethod
+    def backward(ctx, grad_out):
+        colind, arg = ctx.saved_tensors
+        gf = None
+        if ctx.needs_input_grad[2]:
+            gf = kernels.spmm_max_backward(colind, ctx.edge_weight, grad_out, arg, ctx.num_src)
+        return None, None, gf, None, None
+
+
 class GCNAggregate(torch.autograd.Function):
     """y = diag(in_norm)·A·diag(out_norm)·h + bias in one fused SpMM (armsp_spmm_ex);
     backward dh = diag(out_norm)·Aᵀ·diag(in_norm)·dy in one SpMM on (colptr, rowind).

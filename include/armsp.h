@@ -186,6 +186,24 @@ int armsp_spgemm_dense_f32(int M, int K, int N,
                            int nnzB, const int* rowptrB, const int* colindB, const float* valB,
                            float* C, int ldc, void* stream);
 
+/* ------------------------------------------------------- max aggregation */
+/* C[r][j] = max(init, max over the row's nonzeros p of val[p]·B[colind[p]][j]),
+ * folded in CSR order with `acc > x ? acc : x`: the DGL max reducer of
+ * gcn/dgl-custom/binary_reduce_max.cu (:151-166 and the tiled kernels
+ * :26-149, whose init is max_init() = -10000, :22-24).  val may be NULL (copy
+ * of the source features, as the reference).  argmax (NULL, or M×N int32,
+ * row-major) receives the winning position p, or -1 where init won (a later
+ * equal value wins). */
+int armsp_spmm_max_f32(int M, int K, int N, int nnz,
+                       const int* rowptr, const int* colind, const float* val,
+                       const float* B, int ldb, float init, float* C, int ldc, int* argmax, void* stream);
+
+/* Backward of armsp_spmm_max_f32: dB[colind[p]][j] += val[p]·dC[r][j] for
+ * p = argmax[r][j] ≥ 0 (atomic adds; the reference has no backward for its
+ * custom max kernel).  dB is accumulated into, not cleared. */
+int armsp_spmm_max_backward_f32(int M, int K, int N, int nnz, const int* colind, const float* val,
+                                const float* dC, int ldc, const int* argmax, float* dB, int ldb, void* stream);
+
 /* -------------------------------------------------------------- transpose */
 /* CSC of A (= CSR of Aᵀ): colptr[K+1], rowind[nnz], valT[nnz] (valT and val
  * may be NULL together).  Stable: rows ascend inside each column, exactly as
@@ -194,6 +212,19 @@ int armsp_spgemm_dense_f32(int M, int K, int N,
 int armsp_csr_transpose(int M, int K, int nnz,
                         const int* rowptr, const int* colind, const float* val,
                         int* colptr, int* rowind, float* valT, void* stream);
+
+/* Canonical CSR from nnz unordered COO triplets (rows[e], cols[e], vals[e]):
+ * entries in (row, column) order, duplicates summed in input order, and with
+ * drop_zeros the entries whose value (after summing) is 0 left out — the
+ * structure toCSR builds from a dense matrix (SDMM.cpp:355-370,
+ * SpMM.cpp:64-79), on the device.  Outputs: rowptr[M+1], colind[nnz] and
+ * val[nnz] (capacity nnz; vals and val may be NULL together for a pattern),
+ * and *nnz_out (a device int) = the entries written.  Every index is checked
+ * (ARMSP_ERROR_INVALID_VALUE when out of range); the check synchronises the
+ * stream once. */
+int armsp_coo_to_csr(int M, int K, int nnz,
+                     const int* rows, const int* cols, const float* vals, int drop_zeros,
+                     int* rowptr, int* colind, float* val, int* nnz_out, void* stream);
 
 /* -------------------------------------------------------- dense helpers */
 /* C[M×N] (=|+=) op(A)[M×Kd] · op(B)[Kd×N] in bfloat16 on MFMA with fp32

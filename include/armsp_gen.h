@@ -89,6 +89,22 @@ int64_t armsp_csr_to_vcsr(int M, int K, const int* rowptr, const int* colind, co
  * 4 column out of range, 5 columns not strictly increasing). */
 int armsp_csr_check(int M, int K, int64_t nnz, const int* rowptr, const int* colind);
 
+/* Matrix Market coordinate file → canonical CSR (host).  Replaces the
+ * SuiteSparse drivers' reader (suitesparse_spmm.cpp:90-121,
+ * suitesparse_sdmm.cpp:63-90, suitesparse_spmv.cpp:73-100: dense scatter, a
+ * later entry overwrites an earlier one, toCSR keeps nonzeros) and readMtx
+ * (gcn/util/util.hpp:286-333: pattern entries 1.0, symmetric files mirrored).
+ * Same rules here: last write wins, zeros dropped, pattern = 1.0, symmetric
+ * files mirrored (skew-symmetric negated) unless ARMSP_MTX_STORED_TRIANGLE.
+ * Call once with rowptr = colind = val = NULL for *M, *K, *nnz, then with
+ * rowptr[M+1], colind[nnz], val[nnz] (val may stay NULL) and *nnz unchanged.
+ * Returns 0, or 1 cannot open, 2 not a real/integer/pattern coordinate
+ * matrix, 3 malformed size line or entry (or fewer entries than declared),
+ * 4 index out of range, 5 bad arguments. */
+#define ARMSP_MTX_STORED_TRIANGLE 0x1
+int armsp_read_mtx(const char* path, int flags, int* M, int* K, int64_t* nnz,
+                   int* rowptr, int* colind, float* val);
+
 /* CSR values are never exactly zero (toCSR drops zeros, SDMM.cpp:362): a drawn
  * 0 is replaced by 2^-24. */
 

@@ -38,3 +38,28 @@ def test_driver(args, keys):
 def test_driver_bad_arguments():
     r = run("SDMM", "512")
     assert r.returncode == 1 and "usage: SDMM" in r.stderr
+
+
+GOLDEN = os.path.join(ROOT, "tests", "golden")
+
+
+@pytest.mark.parametrize("driver,keys", [
+    ("suitesparse_sdmm", ["m k n ratio:", "rowBaseSpmmTime"]),
+    ("suitesparse_spmm", ["tocsrTime:", "m k n ratio:", "SpMMTime", "C nnz:"]),
+    ("suitesparse_spmv", ["m k:", "time take:"]),
+])
+@pytest.mark.parametrize("matrix", ["gre_1107.mtx", "1138_bus.mtx"])
+def test_suitesparse_driver(driver, keys, matrix):
+    """A Matrix Market file from the reference's datasets/ through each
+    SuiteSparse driver; the first line is the file's "rows cols nnz"."""
+    r = run(driver, "8", os.path.join(GOLDEN, matrix))
+    assert r.returncode == 0
+    first = r.stdout.splitlines()[0].split()
+    assert first == {"gre_1107.mtx": ["1107", "1107", "5664"], "1138_bus.mtx": ["1138", "1138", "2596"]}[matrix]
+    for k in keys:
+        assert k in r.stdout
+
+
+def test_suitesparse_driver_missing_file():
+    r = run("suitesparse_spmv", "8", os.path.join(GOLDEN, "no_such.mtx"))
+    assert r.returncode == 1 and "Matrix Market read error 1" in r.stderr

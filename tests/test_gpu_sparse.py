@@ -278,3 +278,141 @@ def test_transpose_full_rmat_1m():
     cp, ri, vt = _check_transpose(a.rowptr, a.colind, a.values, a.shape[1])
     rp2, ci2, v2 = kernels.csr_transpose(cp, ri, vt, a.shape[0])
     assert np.array_equal(rp2.cpu().numpy(), a.rowptr) and np.array_equal(ci2.cpu().numpy(), a.colind)
+
+
+# ------------------------------------------------------------- COO → CSR
+def _scipy_csr(r, c, v, shape, drop_zeros):
+    import scipy.sparse as sp
+    a = sp.coo_matrix((v.astype(np.float64), (r, c)), shape=shape).tocsr()  # sums duplicates
+    a.sort_indices()
+    if drop_zeros:
+        a.eliminate_zeros()
+    return a.indptr.astype(np.int32), a.indices.astype(np.int32), a.data
+
+
+@pytest.mark.parametrize("drop_zeros", [False, True])
+@pytest.mark.parametrize("m,k,n,seed", [(50, 40, 3000, 1), (1000, 70000, 20000, 2), (1, 1, 5, 3), (70000, 3, 9000, 4)])
+def test_coo_to_csr_matches_scipy(m, k, n, seed, drop_zeros):
+    """Unordered triplets with many duplicates, cancelling pairs and explicit
+    zeros (small-integer values, so every sum is exact): structure and values
+    equal scipy's canonical CSR (duplicates summed, indices sorted, zeros
+    eliminated on request)."""
+    case(f"coo_to_csr {m}x{k} n={n} drop={drop_zeros}")
+    r = np.random.default_rng(seed)
+    rows = r.integers(0, m, n).astype(np.int32)
+    cols = r.integers(0, k, n).astype(np.int32)
+    vals = r.integers(-3, 4, n).astype(np.float32)
+    rows = np.concatenate([rows, rows[:50]]).astype(np.int32)  # exact cancellations
+    cols = np.concatenate([cols, cols[:50]]).astype(np.int32)
+    vals = np.concatenate([vals, -vals[:50]]).astype(np.float32)
+    erp, eci, eva = _scipy_csr(rows, cols, vals, (m, k), drop_zeros)
+    rp, ci, va = kernels.coo_to_csr(*dev(rows, cols, vals), m, k, drop_zeros=drop_zeros)
+    assert np.array_equal(rp.cpu().numpy(), erp)
+    assert np.array_equal(ci.cpu().numpy(), eci)
+    assert np.array_equal(va.cpu().numpy().astype(np.float64), eva)
+    # pattern only: the same structure without values (nothing is dropped)
+    if not drop_zeros:
+        rp2, ci2, va2 = kernels.coo_to_csr(*dev(rows, cols), None, m, k)
+        assert va2 is None
+        assert np.array_equal(rp2.cpu().numpy(), erp) and np.array_equal(ci2.cpu().numpy(), eci)
+
+
+def test_coo_to_csr_equals_toCSR_of_the_dense_matrix():
+    """Every entry of a dense matrix as a triplet, shuffled, with drop_zeros: the
+    result is the reference's toCSR of that matrix (restated in the oracle)."""
+    case("coo_to_csr vs toCSR")
+    r = np.random.default_rng(9)
+    dense = np.where(r.random((37, 53)) < 0.2, r.standard_normal((37, 53)), 0.0).astype(np.float32)
+    ii, jj = np.meshgrid(np.arange(37), np.arange(53), indexing="ij")
+    perm = r.permutation(dense.size)
+    rows, cols, vals = (a.reshape(-1)[perm] for a in (ii.astype(np.int32), jj.astype(np.int32), dense))
+    rp, ci, va = kernels.coo_to_csr(*dev(rows, cols, vals), 37, 53, drop_zeros=True)
+    erp, eci, eva = oracle.toCSR(dense)
+    assert np.array_equal(rp.cpu().numpy(), erp)
+    assert np.array_equal(ci.cpu().numpy(), eci)
+    assert np.array_equal(va.cpu().numpy().view(np.uint32), eva.view(np.uint32))
+
+
+def test_coo_to_csr_rebuilds_the_1m_rmat():
+    """The 1M R-MAT's triplets in a random order come back as the same CSR, bit
+    for bit (no duplicates: the values only move)."""
+    case("coo_to_csr 1M R-MAT")
+    a = rmat_1m_cached()
+    rows = np.repeat(np.arange(a.shape[0], dtype=np.int32), np.diff(a.rowptr))
+    perm = np.random.default_rng(5).permutation(a.nnz)
+    rp, ci, va = kernels.coo_to_csr(*dev(rows[perm], a.colind[perm], a.values[perm]), a.shape[0], a.shape[1])
+    assert np.array_equal(rp.cpu().numpy(), a.rowptr)
+    assert np.array_equal(ci.cpu().numpy(), a.colind)
+    assert np.array_equal(va.cpu().numpy().view(np.uint32), a.values.view(np.uint32))
+
+
+def test_coo_to_csr_edges():
+    case("coo_to_csr edges")
+    e = torch.empty(0, dtype=torch.int32, device="cuda")
+    rp, ci, va = kernels.coo_to_csr(e, e, torch.empty(0, device="cuda"), 5, 4)
+    assert rp.cpu().tolist() == [0] * 6 and ci.numel() == 0 and va.numel() == 0
+    # only the last row has entries; all cancel with drop_zeros
+    rows, cols = np.array([4, 4], np.int32), np.array([3, 3], np.int32)
+    rp, ci, va = kernels.coo_to_csr(*dev(rows, cols, np.array([1.5, -1.5], np.float32)), 5, 4, drop_zeros=True)
+    assert rp.cpu().tolist() == [0] * 6 and ci.numel() == 0
+    rp, ci, va = kernels.coo_to_csr(*dev(rows, cols, np.array([1.5, 2.0], np.float32)), 5, 4)
+    assert rp.cpu().tolist() == [0, 0, 0, 0, 0, 1] and ci.cpu().tolist() == [3] and va.cpu().tolist() == [3.5]
+    # an index out of range is an error, found before anything is written through it
+    bad = np.array([0, 5], np.int32)
+    with pytest.raises(RuntimeError, match="invalid"):
+        kernels.coo_to_csr(*dev(bad, np.array([0, 1], np.int32), np.ones(2, np.float32)), 5, 4)
+    with pytest.raises(RuntimeError, match="invalid"):
+        kernels.coo_to_csr(*dev(np.array([0, 1], np.int32), np.array([0, -1], np.int32), np.ones(2, np.float32)),
+                           5, 4)
+
+
+# ------------------------------------------------------- max aggregation
+def max_fold(rp, ci, va, B, init):
+    """Sequential fold of the reference's max reducer (binary_reduce_max.cu:18-24,
+    151-166) in float32: acc = acc > x ? acc : x in CSR order; argmax = the
+    position whose value was taken last."""
+    M, N = len(rp) - 1, B.shape[1]
+    out = np.full((M, N), np.float32(init), np.float32)
+    arg = np.full((M, N), -1, np.int32)
+    for r in range(M):
+        for p in range(rp[r], rp[r + 1]):
+            x = (np.float32(va[p]) if va is not None else np.float32(1)) * B[ci[p]]
+            take = ~(out[r] > x)
+            out[r] = np.where(take, x, out[r])
+            arg[r] = np.where(take, p, arg[r])
+    return out, arg
+
+
+@pytest.mark.parametrize("m,k,n,density,valued", [(300, 200, 40, 0.05, False), (257, 300, 128, 0.02, True),
+                                                  (64, 50, 3, 0.3, True), (100, 100, 130, 0.0, False)])
+def test_spmm_max_matches_the_sequential_fold(m, k, n, density, valued):
+    case(f"spmm_max {m}x{k} n={n} d={density} valued={valued}")
+    rp, ci, va = random_csr(m, k, density, m + n, empty_rows=0.2)
+    r = np.random.default_rng(n)
+    B = r.integers(-4, 5, (k, n)).astype(np.float32)  # many ties: the later position must win
+    va = va if valued else None
+    out, arg = kernels.spmm_max(*dev(rp, ci, va, B), return_argmax=True)
+    eo, ea = max_fold(rp, ci, va, B, kernels.MAX_INIT)
+    assert np.array_equal(out.cpu().numpy().view(np.uint32), eo.view(np.uint32))
+    assert np.array_equal(arg.cpu().numpy(), ea)
+
+
+def test_spmm_max_power_law_and_backward(rmat14):
+    """Hub rows of a power-law matrix, init -inf; the backward adds each
+    output's gradient (integers: exact in any order) into its winning row."""
+    case("spmm_max rmat14 + backward")
+    a, _ = rmat14
+    rp, ci = a.rowptr, a.colind
+    k = len(rp) - 1
+    r = np.random.default_rng(3)
+    B = r.standard_normal((k, 64)).astype(np.float32)
+    out, arg = kernels.spmm_max(*dev(rp, ci, None, B), init=float("-inf"), return_argmax=True)
+    eo, ea = max_fold(rp, ci, None, B, -np.inf)
+    assert np.array_equal(out.cpu().numpy(), eo)
+    assert np.array_equal(arg.cpu().numpy(), ea)
+    g = r.integers(-3, 4, (k, 64)).astype(np.float32)
+    db = kernels.spmm_max_backward(*dev(ci, None, g), arg, k).cpu().numpy()
+    want = np.zeros_like(B)
+    rows, cols = np.nonzero(ea >= 0)
+    np.add.at(want, (ci[ea[rows, cols]], cols), g[rows, cols])
+    assert np.array_equal(db, want)

@@ -1,6 +1,6 @@
 """The shipped selector weights against the recorded MI355X autotuner timings
 (tests/golden/selector_dataset.npz, written by `python -m armsparse_amd.selector
---matrices 800`), evaluated on the CPU: the network's forward pass in FP64
+--matrices 2000`), evaluated on the CPU: the network's forward pass in FP64
 (oracle.mlp) on the matrices held out of its training (the same split rule as
 selector.train_eval), so the regret claim can be checked without a GPU."""
 import json
