@@ -84,6 +84,7 @@ _SIGS = {
     "armsp_status_string": (c_char_p, [_I]),
     "armsp_last_hip_error": (c_int, []),
     "armsp_device_available": (c_int, []),
+    "armsp_workspace_reserve": (c_int, [ctypes.c_size_t, POINTER(ctypes.c_size_t)]),
     "armsp_timing_enable": (c_int, [_I]),
     "armsp_timing_reset": (c_int, []),
     "armsp_timing_read": (c_int, [c_char_p, POINTER(c_double), POINTER(c_int64)]),

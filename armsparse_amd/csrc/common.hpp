@@ -39,6 +39,7 @@ inline hipStream_t as_stream(void* s) {
 // release threshold is raised at first use so freed blocks stay cached.
 int ws_alloc(void** p, size_t bytes, hipStream_t s);
 void ws_free(void* p, hipStream_t s);
+int workspace_reserve(size_t bytes, size_t* reserved);
 
 // Optional per-launch timing (armsp_timing_enable).  `TimedLaunch t(family, s)`
 // records a start event; t.stop() records the stop event.

@@ -138,6 +138,32 @@ __global__ void k_colsum(int M, int N, const float* __restrict__ X, float* __res
     out[n] = s;
 }
 
+// Tall matrices: block b sums rows [b·R, (b+1)·R) — 64 columns by 4 row
+// phases, the phases joined in LDS in a fixed order — into part[b][·];
+// k_colsum_parts then adds the blocks' sums in block order.  Deterministic.
+__global__ __launch_bounds__(256) void k_colsum_blocks(int M, int N, int R, const float* __restrict__ X,
+                                                       float* __restrict__ part) {
+    __shared__ float s[4][64];
+    const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+    const int n = blockIdx.y * 64 + tx;
+    const int64_t r0 = (int64_t)blockIdx.x * R;
+    const int64_t r1 = r0 + R < M ? r0 + R : M;
+    float acc = 0.0f;
+    if (n < N)
+        for (int64_t m = r0 + ty; m < r1; m += 4) acc += X[m * N + n];
+    s[ty][tx] = acc;
+    __syncthreads();
+    if (ty == 0 && n < N) part[(int64_t)blockIdx.x * N + n] = ((s[0][tx] + s[1][tx]) + s[2][tx]) + s[3][tx];
+}
+
+__global__ void k_colsum_parts(int nb, int N, const float* __restrict__ part, float* __restrict__ out) {
+    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= N) return;
+    float s = 0.0f;
+    for (int b = 0; b < nb; ++b) s += part[(int64_t)b * N + n];
+    out[n] = s;
+}
+
 __global__ void k_adam(int64_t n, float* __restrict__ p, const float* __restrict__ g, float* __restrict__ m,
                        float* __restrict__ v, float lr, float b1, float b2, float eps, float wd, float c1, float c2) {
     const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
@@ -247,9 +273,25 @@ int armsp_colsum_f32(int M, int N, const float* X, float* out, void* stream) {
     using namespace armsp;
     if (M < 0 || N < 0 || (N > 0 && (!X || !out))) return ARMSP_ERROR_INVALID_VALUE;
     if (N == 0) return ARMSP_SUCCESS;
-    hipLaunchKernelGGL(k_colsum, dim3(blocks_for(N)), dim3(256), 0, as_stream(stream), M, N, X, out);
-    ARMSP_LAUNCHED();
-    return ARMSP_SUCCESS;
+    hipStream_t s = as_stream(stream);
+    if (M <= 4096 || N > 65535 * 64) {  // short columns: a thread per column
+        hipLaunchKernelGGL(k_colsum, dim3(blocks_for(N)), dim3(256), 0, s, M, N, X, out);
+        ARMSP_LAUNCHED();
+        return ARMSP_SUCCESS;
+    }
+    const int64_t nb64 = ((int64_t)M + 1023) / 1024;
+    const int nb = (int)(nb64 < 1024 ? nb64 : 1024);
+    const int R = (int)(((int64_t)M + nb - 1) / nb);
+    void* ws = nullptr;
+    int st = ws_alloc(&ws, (size_t)nb * N * sizeof(float), s);
+    if (st) return st;
+    float* part = static_cast<float*>(ws);
+    hipLaunchKernelGGL(k_colsum_blocks, dim3((unsigned)nb, (unsigned)((N + 63) / 64)), dim3(256), 0, s, M, N, R, X,
+                       part);
+    hipLaunchKernelGGL(k_colsum_parts, dim3(blocks_for(N)), dim3(256), 0, s, nb, N, part, out);
+    const hipError_t e = hipGetLastError();
+    ws_free(ws, s);
+    return e == hipSuccess ? ARMSP_SUCCESS : hip_fail(e);
 }
 
 int armsp_adam_step(int64_t n, float* param, const float* grad, float* m, float* v, float lr, float beta1,

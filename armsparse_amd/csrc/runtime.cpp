@@ -2,6 +2,7 @@
 // workspace, optional event timing of kernel launches.
 #include <mutex>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "common.hpp"
@@ -14,6 +15,19 @@ thread_local int g_last_hip_error = 0;
 std::mutex g_pool_mu;
 bool g_pool_ready[64] = {};
 hipMemPool_t g_pools[64] = {};
+
+// Workspace for streams under HIP graph capture, where nothing may allocate:
+// one reserved arena per device (armsp_workspace_reserve), carved from the
+// bottom as a stack (an entry point may hold two workspaces at once, freed in
+// reverse order).  A captured graph keeps the arena's addresses, so an arena
+// is never freed: a larger reservation retires the old one.
+std::mutex g_arena_mu;
+char* g_arena[64] = {};
+size_t g_arena_bytes[64] = {};
+size_t g_arena_used[64] = {};
+std::vector<std::pair<size_t, size_t>> g_arena_marks[64];  // [begin, end) of the live blocks, newest last
+size_t g_ws_high[64] = {};              // the largest workspace a call on the device has asked for
+std::vector<void*> g_retired;
 
 struct TimingRecord {
     std::string family;
@@ -69,6 +83,21 @@ int ws_alloc(void** p, size_t bytes, hipStream_t s) {
     int dev = 0;
     ARMSP_HIP(hipGetDevice(&dev));
     if (dev < 0 || dev >= 64) return ARMSP_ERROR_NOT_SUPPORTED;
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    ARMSP_HIP(hipStreamIsCapturing(s, &cs));
+    if (cs != hipStreamCaptureStatusNone) {
+        std::lock_guard<std::mutex> g(g_arena_mu);
+        const size_t off = (g_arena_used[dev] + 255) & ~(size_t)255;
+        if (!g_arena[dev] || off > g_arena_bytes[dev] || bytes > g_arena_bytes[dev] - off) return ARMSP_ERROR_ALLOC;
+        g_arena_marks[dev].push_back({off, off + bytes});
+        g_arena_used[dev] = off + bytes;
+        *p = g_arena[dev] + off;
+        return ARMSP_SUCCESS;
+    }
+    {
+        std::lock_guard<std::mutex> g(g_arena_mu);
+        if (bytes > g_ws_high[dev]) g_ws_high[dev] = bytes;
+    }
     hipMemPool_t pool;
     {
         std::lock_guard<std::mutex> g(g_pool_mu);
@@ -90,13 +119,54 @@ int ws_alloc(void** p, size_t bytes, hipStream_t s) {
 }
 
 void ws_free(void* p, hipStream_t s) {
-    if (p) hipFreeAsync(p, s);
+    if (!p) return;
+    int dev = 0;
+    if (hipGetDevice(&dev) == hipSuccess && dev >= 0 && dev < 64) {
+        std::lock_guard<std::mutex> g(g_arena_mu);
+        char* c = static_cast<char*>(p);
+        if (g_arena[dev] && c >= g_arena[dev] && c < g_arena[dev] + g_arena_bytes[dev]) {
+            auto& marks = g_arena_marks[dev];
+            const size_t off = (size_t)(c - g_arena[dev]);
+            for (size_t i = marks.size(); i-- > 0;)
+                if (marks[i].first == off) {
+                    marks.erase(marks.begin() + (std::ptrdiff_t)i);
+                    break;
+                }
+            // space above the newest live block is free again (a hole below it waits for it)
+            g_arena_used[dev] = marks.empty() ? 0 : marks.back().second;
+            return;
+        }
+    }
+    hipFreeAsync(p, s);
+}
+
+int workspace_reserve(size_t bytes, size_t* reserved) {
+    int dev = 0;
+    ARMSP_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= 64) return ARMSP_ERROR_NOT_SUPPORTED;
+    std::lock_guard<std::mutex> g(g_arena_mu);
+    // room for two workspaces at once, each as large as any seen so far
+    size_t want = bytes > 2 * g_ws_high[dev] + 256 ? bytes : 2 * g_ws_high[dev] + 256;
+    want = (want + 255) & ~(size_t)255;
+    if (want > g_arena_bytes[dev]) {
+        if (!g_arena_marks[dev].empty()) return ARMSP_ERROR_NOT_SUPPORTED;  // a capture holds the arena
+        void* a = nullptr;
+        ARMSP_HIP(hipMalloc(&a, want));
+        if (g_arena[dev]) g_retired.push_back(g_arena[dev]);
+        g_arena[dev] = static_cast<char*>(a);
+        g_arena_bytes[dev] = want;
+        g_arena_used[dev] = 0;
+    }
+    if (reserved) *reserved = g_arena_bytes[dev];
+    return ARMSP_SUCCESS;
 }
 
 bool timing_on() { return g_timing; }
 
 TimedLaunch::TimedLaunch(const char* family, hipStream_t s) {
     if (!g_timing) return;
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;  // no timing events inside a captured graph
+    if (hipStreamIsCapturing(s, &cs) != hipSuccess || cs != hipStreamCaptureStatusNone) return;
     TimingRecord* r = new TimingRecord{family, nullptr, nullptr};
     if (hipEventCreate(&r->start) != hipSuccess) {
         delete r;
@@ -156,6 +226,8 @@ int armsp_device_available(void) {
     if (hipGetDeviceProperties(&prop, dev) != hipSuccess) return 0;
     return std::string(prop.gcnArchName).rfind("gfx950", 0) == 0 ? 1 : 0;
 }
+
+int armsp_workspace_reserve(size_t bytes, size_t* reserved) { return armsp::workspace_reserve(bytes, reserved); }
 
 int armsp_timing_enable(int on) {
     std::lock_guard<std::mutex> g(armsp::g_timing_mu);

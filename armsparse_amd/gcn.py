@@ -15,23 +15,24 @@ import numpy as np
 import torch
 import torch.nn.functional as F
 
-from . import data
+from . import data, kernels
 from .op import GCNConv
 
 
 class Net(torch.nn.Module):
     """conv1 → ReLU → dropout → conv2 → log_softmax (gcn_custom.py:72-90)."""
 
-    def __init__(self, in_feats: int, n_hidden: int, n_classes: int, precision: str = "fp32"):
+    def __init__(self, in_feats: int, n_hidden: int, n_classes: int, precision: str = "fp32", dropout: float = 0.5):
         super().__init__()
         self.conv1 = GCNConv(in_feats, n_hidden, cached=True, normalize=True, precision=precision)
         self.conv2 = GCNConv(n_hidden, n_classes, cached=True, normalize=True, precision=precision)
         self.reg_params = self.conv1.parameters()
         self.non_reg_params = self.conv2.parameters()
+        self.p = dropout
 
     def forward(self, x, g):
         x = F.relu(self.conv1(x, *g))
-        x = F.dropout(x, training=self.training)
+        x = F.dropout(x, p=self.p, training=self.training)
         x = self.conv2(x, *g)
         return F.log_softmax(x, dim=1)
 
@@ -46,11 +47,23 @@ class GCNTask:
     train_mask: torch.Tensor
     precision: str
     nnz: int
+    train_idx: torch.Tensor = None  # positions of train_mask (index_select needs no host sync)
+    y_train: torch.Tensor = None
+
+    def __post_init__(self):
+        if self.train_idx is None:
+            self.train_idx = self.train_mask.nonzero().squeeze(1)
+            self.y_train = self.y.index_select(0, self.train_idx)
+
+
+def _adam(model: Net, capturable: bool):
+    return torch.optim.Adam([dict(params=model.reg_params, weight_decay=5e-4),
+                             dict(params=model.non_reg_params, weight_decay=0)], lr=0.01, capturable=capturable)
 
 
 def make_task(precision: str = "fp32", seed: int = data.CFG_SEED + 3, n: int = 19_717, pairs: int = 44_324,
               features: int = 500, hidden: int = 64, classes: int = 3, feat_density: float = 0.01,
-              device: str = "cuda") -> GCNTask:
+              device: str = "cuda", dropout: float = 0.5, capturable: bool = False) -> GCNTask:
     g = data.gcn_graph(n, pairs, seed)
     r = np.random.default_rng(seed)
     x = np.zeros((n, features), np.float32)
@@ -69,31 +82,69 @@ def make_task(precision: str = "fp32", seed: int = data.CFG_SEED + 3, n: int = 1
     if precision == "bf16":
         xt = xt.to(torch.bfloat16)
     graph = (t(g.rowptr), t(g.colind), t(g.colptr), t(g.rowind), t(g.value_csr), t(g.value_csc))
-    model = Net(features, hidden, classes, precision).to(dev)
-    opt = torch.optim.Adam([dict(params=model.reg_params, weight_decay=5e-4),
-                            dict(params=model.non_reg_params, weight_decay=0)], lr=0.01)
-    return GCNTask(model, opt, xt, graph, t(y), t(train), precision, int(g.colind.size))
+    model = Net(features, hidden, classes, precision, dropout).to(dev)
+    return GCNTask(model, _adam(model, capturable), xt, graph, t(y), t(train), precision, int(g.colind.size))
 
 
 SCALED = dict(n=2_400_000, pairs=62_000_000, features=100, hidden=256, feat_density=0.1)
 
 
-def retarget(task: GCNTask, precision: str) -> GCNTask:
+def retarget(task: GCNTask, precision: str, capturable: bool = False) -> GCNTask:
     """A fresh model and optimizer of `precision` on the same graph, features and labels."""
     x = task.x.float() if precision == "fp32" else task.x.to(torch.bfloat16)
     dev = x.device
-    model = Net(x.shape[1], task.model.conv1.out_channels, task.model.conv2.out_channels, precision).to(dev)
-    opt = torch.optim.Adam([dict(params=model.reg_params, weight_decay=5e-4),
-                            dict(params=model.non_reg_params, weight_decay=0)], lr=0.01)
-    return GCNTask(model, opt, x, task.graph, task.y, task.train_mask, precision, task.nnz)
+    model = Net(x.shape[1], task.model.conv1.out_channels, task.model.conv2.out_channels, precision,
+                task.model.p).to(dev)
+    return GCNTask(model, _adam(model, capturable), x, task.graph, task.y, task.train_mask, precision, task.nnz,
+                   task.train_idx, task.y_train)
+
+
+def _step(task: GCNTask) -> torch.Tensor:
+    task.model.train()
+    task.optimizer.zero_grad(set_to_none=True)
+    out = task.model(task.x, task.graph)
+    loss = F.nll_loss(out.index_select(0, task.train_idx), task.y_train)
+    loss.backward()
+    task.optimizer.step()
+    return loss.detach()
 
 
 def train_step(task: GCNTask, sync: bool = True):
-    """forward + nll loss + backward + Adam step (gcn_custom.py:122-126)."""
-    task.model.train()
-    task.optimizer.zero_grad()
-    out = task.model(task.x, task.graph)
-    loss = F.nll_loss(out[task.train_mask], task.y[task.train_mask])
-    loss.backward()
-    task.optimizer.step()
-    return float(loss) if sync else loss.detach()
+    """forward + nll loss over the training nodes + backward + Adam step
+    (gcn_custom.py:122-126)."""
+    loss = _step(task)
+    return float(loss) if sync else loss
+
+
+class GraphedStep:
+    """The training step captured once in a HIP graph and replayed: one launch
+    per epoch instead of one per kernel, for graphs small enough that an epoch
+    is launch-bound (PubMed size).  The task's optimizer must be capturable
+    (make_task(..., capturable=True)).  Warm-up steps run eagerly on a side
+    stream (they fill the GCN layers' caches and the library's workspace
+    high-water mark), the library's capture arena is reserved, then one step
+    is captured; replay() runs it again, loss() reads its last loss."""
+
+    def __init__(self, task: GCNTask, warmup: int = 3):
+        if not task.optimizer.defaults.get("capturable"):
+            raise ValueError("GraphedStep needs a capturable optimizer (make_task(..., capturable=True))")
+        self.task = task
+        dev = task.x.device
+        side = torch.cuda.Stream(device=dev)
+        side.wait_stream(torch.cuda.current_stream(dev))
+        with torch.cuda.stream(side):
+            for _ in range(warmup):
+                _step(task)
+        torch.cuda.current_stream(dev).wait_stream(side)
+        torch.cuda.synchronize(dev)
+        kernels.workspace_reserve(0, dev)
+        task.optimizer.zero_grad(set_to_none=True)
+        self.graph = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(self.graph):
+            self._loss = _step(task)
+
+    def replay(self) -> None:
+        self.graph.replay()
+
+    def loss(self) -> float:
+        return float(self._loss)

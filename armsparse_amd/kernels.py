@@ -362,6 +362,24 @@ def gemm_bf16(a: torch.Tensor, b: torch.Tensor, *, out: Optional[torch.Tensor] =
     return out
 
 
+def colsum(x: torch.Tensor) -> torch.Tensor:
+    """x.sum(0) for a 2-D float32 device tensor, deterministic (armsp_colsum_f32)."""
+    _require(x.is_cuda and x.dtype == torch.float32 and x.dim() == 2, "colsum: needs a 2-D float32 device tensor")
+    x = x.contiguous()
+    out = torch.empty(x.shape[1], device=x.device)
+    call("armsp_colsum_f32", x.shape[0], x.shape[1], _ptr(x), _ptr(out), _stream(x.device))
+    return out
+
+
+def workspace_reserve(nbytes: int = 0, device: Optional[torch.device] = None) -> int:
+    """Reserve the capture arena of `device` (the current one by default): at
+    least nbytes and at least the largest workspace used so far."""
+    r = ctypes.c_size_t(0)
+    with torch.cuda.device(device if device is not None else torch.cuda.current_device()):
+        call("armsp_workspace_reserve", ctypes.c_size_t(nbytes), ctypes.byref(r))
+    return r.value
+
+
 def to_bf16(x: torch.Tensor) -> torch.Tensor:
     """float32 → bfloat16 (round to nearest even) with the library kernel."""
     _require(x.is_cuda and x.dtype == torch.float32, "to_bf16: needs a float32 device tensor")

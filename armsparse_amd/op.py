@@ -111,7 +111,7 @@ class GCNAggregate(torch.autograd.Function):
         if hdt == torch.bfloat16:
             g = kernels.to_bf16(g.float())
         dh = kernels.spmm(colptr, rowind, coef_bwd, g, row_scale=out_norm)
-        dbias = gy.sum(0) if ctx.needs_input_grad[9] else None
+        dbias = kernels.colsum(gy.float()) if ctx.needs_input_grad[9] else None
         return dh.to(hdt), None, None, None, None, None, None, None, None, dbias
 
 

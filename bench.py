@@ -226,6 +226,13 @@ def main():
             extras[f"gcn_{prec}_ms_per_epoch"] = round(el_g / 30 * 1e3, 3)
             del task
         extras["gcn_bf16_speedup"] = round(extras["gcn_fp32_ms_per_epoch"] / extras["gcn_bf16_ms_per_epoch"], 3)
+        # the bf16 step captured once in a HIP graph and replayed (one launch per epoch)
+        torch.manual_seed(0)
+        task = gcn.make_task(precision="bf16", capturable=True)
+        step = gcn.GraphedStep(task)
+        el_g, _ = timed(step.replay, 30, 5, [])
+        extras["gcn_bf16_graph_ms_per_epoch"] = round(el_g / 30 * 1e3, 3)
+        del step, task
     if "gcn_scaled" in only:
         from armsparse_amd import gcn
 

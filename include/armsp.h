@@ -17,6 +17,9 @@
  * own stream-ordered memory pool on the caller's stream
  * (hipMallocFromPoolAsync / hipFreeAsync), so concurrent calls on different
  * streams never share it and the device's default pool is left untouched.
+ * While the stream is being captured into a HIP graph they take it from the
+ * arena reserved by armsp_workspace_reserve instead (nothing may allocate
+ * during capture).
  */
 #ifndef ARMSP_H
 #define ARMSP_H
@@ -72,6 +75,17 @@ const char* armsp_status_string(int status);
 int armsp_last_hip_error(void);
 /* 1 if a gfx950 device is visible, else 0.  Never fails. */
 int armsp_device_available(void);
+/* Reserves the device's capture arena: the workspace entry points use while
+ * their stream is being captured into a HIP graph.  Its size is at least
+ * `bytes` and at least the largest workspace any call on this device has
+ * asked for so far, so reserve after a warm-up of the work to be captured.
+ * The arena is handed to one call at a time: the captured work must run on
+ * one stream, and graphs that share it must not replay concurrently.  An arena
+ * lives until the process ends (a captured graph keeps its address); a larger
+ * reservation retires it.  *reserved (may be NULL) receives the size.  No
+ * counterpart in the reference (it has no device memory). */
+int armsp_workspace_reserve(size_t bytes, size_t* reserved);
+
 /* Kernel timing: when enabled, every main kernel launch is bracketed by HIP
  * events on its own stream; totals are read per kernel family. */
 int armsp_timing_enable(int on);
@@ -279,7 +293,11 @@ int armsp_softmax_xent(int rows, int classes, const float* logits, const int* la
                        float* dlogits, float* loss, void* stream);
 /* grad[i] = 0 where act[i] <= 0. */
 int armsp_relu_backward(int64_t n, const float* act, float* grad, void* stream);
-/* out[n] = sum over m of X[m][n] (rows summed in order). */
+/* out[n] = sum over m of X[m][n]: rows summed in order up to 4096 rows; taller
+ * matrices in at most 1024 row blocks (each in four interleaved row phases)
+ * whose sums are added in block order.  Deterministic; also the bias gradient
+ * of the GCN layer (a torch reduction there could record memset nodes into a
+ * captured graph). */
 int armsp_colsum_f32(int M, int N, const float* X, float* out, void* stream);
 /* One Adam step (L2 weight decay added to the gradient); step counts from 1. */
 int armsp_adam_step(int64_t n, float* param, const float* grad, float* m, float* v, float lr, float beta1,

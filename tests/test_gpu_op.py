@@ -7,7 +7,7 @@ import pytest
 import torch
 
 import oracle
-from armsparse_amd import data, gcn, spmm
+from armsparse_amd import data, gcn, kernels, spmm
 from armsparse_amd.op import GCNConv, SPMMFunction
 
 pytestmark = pytest.mark.gpu
@@ -148,3 +148,33 @@ def test_fused_gcn_layer_matches_unfused(graph512):
     assert np.max(oracle.row_rel_err(outs[1], outs[0])) <= 1e-5
     for a, b in zip(grads[0], grads[1]):
         assert np.max(np.abs(a - b)) <= 1e-5 * (np.max(np.abs(a)) + 1e-12)
+
+
+def test_graphed_gcn_step_matches_eager():
+    """The bf16 training step captured in a HIP graph: five replays, each
+    followed by an eager step of an identically initialised model and a
+    synchronisation, must give the eager step's loss and parameters every time
+    (dropout 0: the graph's and the eager step's random streams differ)."""
+    print("[case] graphed GCN step vs eager", flush=True)
+    torch.manual_seed(0)
+    a = gcn.make_task("bf16", n=4000, pairs=9000, features=120, hidden=32, dropout=0.0, capturable=True)
+    b = gcn.retarget(a, "bf16", capturable=True)
+    b.model.load_state_dict(a.model.state_dict())
+    step = gcn.GraphedStep(a, warmup=3)  # three eager steps of `a`, then the capture
+    for _ in range(3):
+        gcn.train_step(b)
+    for i in range(5):
+        step.replay()
+        eager_loss = gcn.train_step(b)
+        torch.cuda.synchronize()
+        assert abs(step.loss() - eager_loss) <= 1e-6 * max(1.0, abs(eager_loss)), i
+        for (na, pa), (nb, pb) in zip(a.model.named_parameters(), b.model.named_parameters()):
+            d = (pa - pb).abs().max().item()
+            assert d <= 1e-6 * max(1.0, pb.abs().max().item()), (i, na, d)
+
+
+def test_workspace_reserve_keeps_growing():
+    r1 = kernels.workspace_reserve(1 << 20)
+    r2 = kernels.workspace_reserve(0)
+    r3 = kernels.workspace_reserve(3 << 20)
+    assert r1 >= 1 << 20 and r2 == r1 and r3 >= 3 << 20
