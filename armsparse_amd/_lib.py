@@ -102,6 +102,7 @@ _SIGS = {
     "armsp_csr_transpose": (c_int, [_I, _I, _I, _P, _P, _P, _P, _P, _P, _P]),
     "armsp_coo_to_csr": (c_int, [_I, _I, _I, _P, _P, _P, _I, _P, _P, _P, _P, _P]),
     "armsp_spmm_max_f32": (c_int, [_I, _I, _I, _I, _P, _P, _P, _P, _I, c_float, _P, _I, _P, _P]),
+    "armsp_spmm_datareuse_f32": (c_int, [_I, _I, _I, _I, _P, _P, _P, _P, _P, _I, _P, _I, _I, _P]),
     "armsp_spmm_max_backward_f32": (c_int, [_I, _I, _I, _I, _P, _P, _P, _I, _P, _P, _I, _P]),
     "armsp_gemm_bf16": (c_int, [_I, _I, _I, _P, c_int64, c_int64, _P, c_int64, c_int64, _P, c_int64, _I, _P]),
     "armsp_gemm_bf16_ex": (c_int, [_I, _I, _I, _P, c_int64, c_int64, _P, c_int64, c_int64, _P, _I, c_int64, _I, _P]),
@@ -131,6 +132,7 @@ _SIGS = {
     "armsp_gen_corpus_csr": (c_int64, [_I, _I, c_double, c_double, c_uint64, c_int64, _P, _P, _P]),
     "armsp_csr_to_vcsr": (c_int64, [_I, _I, _P, _P, _P, c_float, _P, _P, _P]),
     "armsp_csr_check": (c_int, [_I, _I, c_int64, _P, _P]),
+    "armsp_csr_to_datareuse": (c_int, [_I, _I, _P, _P, _P, _P, _P, _P, _P]),
     "armsp_read_mtx": (c_int, [c_char_p, _I, POINTER(c_int), POINTER(c_int), POINTER(c_int64), _P, _P, _P]),
 }
 

@@ -195,3 +195,38 @@ void csr_spmm_kernel(int A_nrows, int B_ncols, int* A_csrRowPtr, int* A_csrColIn
     spmm_shim(fn, A_csrRowPtr, A_csrColInd, A_csrVal, B_dnVal, C_dnVal, A_nrows, K, B_ncols, B_ncols,
               ARMSP_ACCUMULATE);
 }
+
+void DataReuseSpmm(int rowPtr[], int colInd[], float val[], float B[], float C[], int M, int K, int N, int,
+                   bool* flag) {
+    const char* fn = "DataReuseSpmm";
+    if (M < 0 || K < 0 || N < 0 || !rowPtr) die(fn, "bad dimensions or row pointer");
+    if (M == 0 || N == 0) return;
+    const int nnz = rowPtr[M];
+    if (rowPtr[0] != 0) die(fn, "rowPtr[0] != 0");
+    for (int r = 0; r < M; ++r)
+        if (rowPtr[r + 1] < rowPtr[r]) die(fn, "rowPtr is decreasing");
+    if (nnz > 0 && (!colInd || !val || !flag || !B)) die(fn, "null operand");
+    if (!C) die(fn, "null C");
+    for (int i = 0; i < nnz; ++i)
+        if (colInd[i] < 0 || colInd[i] >= K) die(fn, "column index out of range");
+    std::vector<uint8_t> fl((size_t)nnz);
+    for (int i = 0; i < nnz; ++i) fl[i] = flag[i] ? 1 : 0;
+    hipStream_t s;
+    hip_or_die(fn, hipStreamCreate(&s));
+    Dev drp(fn, (size_t)(M + 1) * 4), dci(fn, (size_t)nnz * 4), dva(fn, (size_t)nnz * 4), dfl(fn, (size_t)nnz);
+    Dev dB(fn, (size_t)K * N * 4), dC(fn, (size_t)M * N * 4);
+    hip_or_die(fn, hipMemcpyAsync(drp.p, rowPtr, (size_t)(M + 1) * 4, hipMemcpyHostToDevice, s));
+    if (nnz) {
+        hip_or_die(fn, hipMemcpyAsync(dci.p, colInd, (size_t)nnz * 4, hipMemcpyHostToDevice, s));
+        hip_or_die(fn, hipMemcpyAsync(dva.p, val, (size_t)nnz * 4, hipMemcpyHostToDevice, s));
+        hip_or_die(fn, hipMemcpyAsync(dfl.p, fl.data(), (size_t)nnz, hipMemcpyHostToDevice, s));
+        hip_or_die(fn, hipMemcpyAsync(dB.p, B, (size_t)K * N * 4, hipMemcpyHostToDevice, s));
+    }
+    hip_or_die(fn, hipMemcpyAsync(dC.p, C, (size_t)M * N * 4, hipMemcpyHostToDevice, s));
+    status_or_die(fn, armsp_spmm_datareuse_f32(M, K, N, nnz, drp.as<int>(), dci.as<int>(), dva.as<float>(),
+                                               dfl.as<uint8_t>(), dB.as<float>(), N, dC.as<float>(), N,
+                                               ARMSP_ACCUMULATE, s));
+    hip_or_die(fn, hipMemcpyAsync(C, dC.p, (size_t)M * N * 4, hipMemcpyDeviceToHost, s));
+    hip_or_die(fn, hipStreamSynchronize(s));
+    hipStreamDestroy(s);
+}

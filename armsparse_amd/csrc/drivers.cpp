@@ -10,7 +10,7 @@
 //   balanceNUMAAwareSPDM M K N      (numaAware/balanceNUMAAwareSPDM.cpp)
 //   numaAwareSpMV M K threads       (numaAware/numaAwareSpMV.cpp:126-)
 //   numaAwareSpMM M K N threads     (numaAware/numaAwareSpMM.cpp)
-//   suitesparse_sdmm threads [file] (suitesparse_sdmm.cpp:63-147: rowBaseSpmm, N = 1024)
+//   suitesparse_sdmm threads [file] (suitesparse_sdmm.cpp:63-147: rowBaseSpmm and DataReuseSpmm, N = 1024)
 //   suitesparse_spmm threads [file] (suitesparse_spmm.cpp:90-172: sparse_matrix_mul3, A·A)
 //   suitesparse_spmv threads [file] (suitesparse_spmv.cpp:73-132: blocked-CSR SpMV, x = 5)
 //
@@ -297,8 +297,24 @@ int run_suitesparse(const std::string& name, const char* path) {
             st_check(armsp_spmm_f32(a.M, a.K, N, a.nnz(), rp, ci, va, dB, N, dC, N, ARMSP_ACCUMULATE | ARMSP_TRUNC4,
                                     &row, nullptr), "spmm");
         }, ite);
+        // the DataReuse format the reference builds beside it (toDataReuseFormat) and its SpMM
+        std::vector<int> drp(a.M + 1), dci(a.nnz());
+        std::vector<float> dva(a.nnz());
+        std::vector<uint8_t> dfl(a.nnz());
+        if (armsp_csr_to_datareuse(a.M, a.K, a.rp.data(), a.ci.data(), a.va.data(), drp.data(), dci.data(),
+                                   dva.data(), dfl.data()))
+            die("DataReuse conversion failed");
+        int *xrp = upload(drp), *xci = upload(dci);
+        float* xva = upload(dva);
+        uint8_t* xfl = upload(dfl);
+        const double t2 = time_kernel([&] {
+            st_check(armsp_spmm_datareuse_f32(a.M, a.K, N, a.nnz(), xrp, xci, xva, xfl, dB, N, dC, N,
+                                              ARMSP_ACCUMULATE, nullptr), "datareuse");
+        }, ite);
         std::printf("m k n ratio:%d %d %d  nnz:%d\n", a.M, a.K, N, declared);
         std::printf("rowBaseSpmmTime %g\n", t);
+        std::printf("DataReuseSpmmTime %g\n", t2);
+        hipFree(xrp); hipFree(xci); hipFree(xva); hipFree(xfl);
         hipFree(dB);
         hipFree(dC);
     } else if (name == "suitesparse_spmm") {  // A·A into a dense C, then toCSR of C

@@ -449,6 +449,49 @@ int64_t armsp_csr_to_vcsr(int M, int K, const int* rowptr, const int* colind, co
     return size;
 }
 
+int armsp_csr_to_datareuse(int M, int K, const int* rowptr, const int* colind, const float* val, int* out_rowptr,
+                           int* out_colind, float* out_val, uint8_t* out_flag) {
+    if (M < 0 || K < 0 || !rowptr || !out_rowptr) return 1;
+    const int64_t nnz = rowptr[M];
+    if (nnz > 0 && (!colind || !out_colind || !out_flag)) return 1;
+    if (armsp_csr_check(M, K, nnz, rowptr, colind)) return 2;
+    // entries of row r+1 already placed behind their partner in row r
+    std::vector<uint8_t> taken;
+    int64_t size = 0;
+    for (int r = 0; r < M; ++r) {
+        out_rowptr[r] = (int)size;
+        const bool even = (r & 1) == 0;
+        if (even) {
+            const int q0 = r + 1 < M ? rowptr[r + 1] : 0, q1 = r + 1 < M ? rowptr[r + 2] : 0;
+            taken.assign((size_t)(q1 - q0), 0);
+            int q = q0;
+            for (int p = rowptr[r]; p < rowptr[r + 1]; ++p) {
+                const int c = colind[p];
+                out_colind[size] = c;
+                if (out_val) out_val[size] = val ? val[p] : 1.0f;
+                out_flag[size++] = 0;
+                while (q < q1 && colind[q] < c) ++q;  // both rows ascend: one merge pass
+                if (q < q1 && colind[q] == c) {
+                    out_colind[size] = c;
+                    if (out_val) out_val[size] = val ? val[q] : 1.0f;
+                    out_flag[size++] = 1;
+                    taken[(size_t)(q - q0)] = 1;
+                }
+            }
+        } else {
+            const int q0 = rowptr[r];
+            for (int p = rowptr[r]; p < rowptr[r + 1]; ++p) {
+                if (taken[(size_t)(p - q0)]) continue;
+                out_colind[size] = colind[p];
+                if (out_val) out_val[size] = val ? val[p] : 1.0f;
+                out_flag[size++] = 0;
+            }
+        }
+    }
+    out_rowptr[M] = (int)size;
+    return 0;
+}
+
 int armsp_csr_check(int M, int K, int64_t nnz, const int* rowptr, const int* colind) {
     if (!rowptr) return 1;
     if (rowptr[0] != 0) return 1;

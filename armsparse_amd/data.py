@@ -111,6 +111,20 @@ def csr_check(a: CSR) -> int:
 
 
 # ------------------------------------------------------------ workloads
+def csr_to_datareuse(a: CSR):
+    """(rowptr, colind, values, flag uint8): the 2-row DataReuse format of
+    toDataReuseFormat (armsp_csr_to_datareuse), same entry count as a."""
+    rp = np.empty(a.shape[0] + 1, np.int32)
+    ci = np.empty(a.nnz, np.int32)
+    va = np.empty(a.nnz, np.float32)
+    fl = np.empty(a.nnz, np.uint8)
+    st = lib.armsp_csr_to_datareuse(a.shape[0], a.shape[1], _p(a.rowptr), _p(a.colind), _p(a.values), _p(rp),
+                                    _p(ci), _p(va), _p(fl))
+    if st:
+        raise ValueError(f"armsp_csr_to_datareuse: error {st}")
+    return rp, ci, va, fl
+
+
 def read_matrix_market(path: str, mirror: bool = True) -> CSR:
     """Matrix Market coordinate file → canonical CSR (armsp_read_mtx): a later
     entry for a position overwrites an earlier one, zeros are dropped, pattern

@@ -278,6 +278,27 @@ def csr_transpose(rowptr: torch.Tensor, colind: torch.Tensor, values: Optional[t
     return tuple(None if x is None else x.cpu() for x in res) if host else res
 
 
+def spmm_datareuse(rowptr: torch.Tensor, colind: torch.Tensor, values: Optional[torch.Tensor], flag: torch.Tensor,
+                   dense: torch.Tensor, *, out: Optional[torch.Tensor] = None, accumulate: bool = False) -> torch.Tensor:
+    """A·dense on the 2-row DataReuse format (data.csr_to_datareuse): entry p of
+    row m goes to output row m + flag[p] (DataReuseSpmm)."""
+    fn = "spmm_datareuse"
+    _require(rowptr.dtype == torch.int32 and colind.dtype == torch.int32, f"{fn}: indices must be int32")
+    _require(flag.dtype == torch.uint8 and flag.numel() == colind.numel(), f"{fn}: flag must be uint8, one per entry")
+    _require(dense.is_cuda and dense.dim() == 2 and dense.dtype == torch.float32, f"{fn}: dense must be a 2-D float32 "
+             "device tensor")
+    m, nnz = rowptr.numel() - 1, colind.numel()
+    k, n = dense.shape
+    d = dense.contiguous()
+    if out is None:
+        _require(not accumulate, f"{fn}: accumulate needs an output tensor")
+        out = torch.empty((m, n), device=dense.device)
+    _require(out.shape == (m, n) and out.is_contiguous(), f"{fn}: out must be a contiguous [M, N] tensor")
+    call("armsp_spmm_datareuse_f32", m, k, n, nnz, _ptr(rowptr), _ptr(colind), _ptr(values), _ptr(flag), _ptr(d), n,
+         _ptr(out), n, _lib.ACCUMULATE if accumulate else 0, _stream(dense.device))
+    return out
+
+
 MAX_INIT = -10000.0  # max_init() of gcn/dgl-custom/binary_reduce_max.cu:22-24
 
 

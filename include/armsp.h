@@ -200,6 +200,17 @@ int armsp_spgemm_dense_f32(int M, int K, int N,
                            int nnzB, const int* rowptrB, const int* colindB, const float* valB,
                            float* C, int ldc, void* stream);
 
+/* ------------------------------------------------------- DataReuse SpMM */
+/* C (=|+= with ARMSP_ACCUMULATE) A·B on the 2-row DataReuse format
+ * (armsp_csr_to_datareuse): every entry of row m goes to C row m + flag[p].
+ * Replaces DataReuseSpmm (suitesparse_sdmm.cpp:25-43, datasets/suitesparse.cpp
+ * :23-50); a row pair shares the B row of a flagged entry and its partner, and
+ * each C row gets its additions in the reference's order (bit-identical to
+ * the sequential kernel).  Flags of odd rows are treated as 0. */
+int armsp_spmm_datareuse_f32(int M, int K, int N, int nnz,
+                             const int* rowptr, const int* colind, const float* val, const uint8_t* flag,
+                             const float* B, int ldb, float* C, int ldc, int flags, void* stream);
+
 /* ------------------------------------------------------- max aggregation */
 /* C[r][j] = max(init, max over the row's nonzeros p of val[p]·B[colind[p]][j]),
  * folded in CSR order with `acc > x ? acc : x`: the DGL max reducer of

@@ -84,6 +84,17 @@ int64_t armsp_gen_corpus_csr(int kind, int M, double avg_row, double param, uint
 int64_t armsp_csr_to_vcsr(int M, int K, const int* rowptr, const int* colind, const float* val,
                           float threshold, int* out_rowptr, int* out_colind, float* out_val);
 
+/* Canonical CSR → the 2-row DataReuse format of toDataReuseFormat
+ * (suitesparse_sdmm.cpp:150-179, datasets/suitesparse.cpp:229-263), built
+ * from CSR instead of a dense matrix: after each entry (r, c) of an even row
+ * r, the entry (r+1, c) of the next row, if any, follows with flag 1; an odd
+ * row keeps only the columns its even partner lacks.  Same entry count as the
+ * CSR (explicit zeros of the CSR stay entries; a dense matrix has none);
+ * outputs out_rowptr[M+1], out_colind/out_val/out_flag[nnz] (val may be NULL:
+ * values 1).  Returns 0, 1 bad arguments, 2 not canonical CSR. */
+int armsp_csr_to_datareuse(int M, int K, const int* rowptr, const int* colind, const float* val,
+                           int* out_rowptr, int* out_colind, float* out_val, uint8_t* out_flag);
+
 /* Validates canonical CSR metadata on the host: returns 0 if valid, else a
  * positive code (1 rowptr[0]!=0, 2 decreasing rowptr, 3 rowptr[M]!=nnz,
  * 4 column out of range, 5 columns not strictly increasing). */

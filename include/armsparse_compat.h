@@ -48,4 +48,9 @@ void sparse_dense_vector(float* B, float* C, int rowPtr[], int colInd[], float v
 void csr_spmm_kernel(int A_nrows, int B_ncols, int* A_csrRowPtr, int* A_csrColInd, float* A_csrVal, float* B_dnVal,
                      float* C_dnVal);
 
+/* C[m + flag] += val·B[colInd] for every entry of row m, on the 2-row
+ * DataReuse format of toDataReuseFormat (suitesparse_sdmm.cpp:25-43). */
+void DataReuseSpmm(int rowPtr[], int colInd[], float val[], float B[], float C[], int M, int K, int N, int threads,
+                   bool* flag);
+
 #endif /* ARMSPARSE_COMPAT_H */

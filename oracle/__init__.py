@@ -40,6 +40,8 @@ for _n, _a in {
     "orc_sparse_dense_vector": [_P, _P, _P, _P, _P, _I, _I, _I],
     "orc_toCSR": [_P, _I, _I, _P, _P, _P],
     "orc_toCSC": [_P, _I, _I, _P, _P, _P],
+    "orc_toDataReuseFormat": [_P, _I, _I, _P, _P, _P, _P],
+    "orc_DataReuseSpmm": [_P, _P, _P, _P, _P, _P, _I, _I, _I, _I],
     "orc_toVectorAwareCSR": [_P, _I, _I, _P, _P, _P, c_float],
     "orc_numa_sdmm": [_P, _P, _P, _P, _P, _P, _I, _I, _I, _I],
     "orc_spmm_f64": [_P, _P, _P, _P, _I, _P, _P, _I, _I, _I],
@@ -162,6 +164,29 @@ def toCSR(dense):
     ci, va = np.empty(nnz, dtype=np.int32), np.empty(nnz, dtype=np.float32)
     lib.orc_toCSR(_p(dense), rows, cols, _p(rp), _p(ci), _p(va))
     return rp, ci, va
+
+
+def toDataReuseFormat(dense):
+    """(rowptr, colind, val, flag uint8) of toDataReuseFormat on a copy of dense."""
+    d = _f32(dense).copy()
+    rows, cols = d.shape
+    n = int(np.count_nonzero(d))
+    rp = np.empty(rows + 1, dtype=np.int32)
+    ci, va = np.empty(n, dtype=np.int32), np.empty(n, dtype=np.float32)
+    fl = np.empty(n, dtype=np.uint8)
+    got = lib.orc_toDataReuseFormat(_p(d), rows, cols, _p(rp), _p(ci), _p(va), _p(fl))
+    assert got == n
+    return rp, ci, va, fl
+
+
+def DataReuseSpmm(rowptr, colind, val, flag, B, C, threads=None):
+    """C (in place, returned) += A·B through DataReuseSpmm's pair order."""
+    B = _f32(B)
+    M, N, K = len(rowptr) - 1, B.shape[1], B.shape[0]
+    assert C.dtype == np.float32 and C.flags.c_contiguous and C.shape == (M, N)
+    lib.orc_DataReuseSpmm(_p(_i32(rowptr)), _p(_i32(colind)), _p(_f32(val)), _p(np.ascontiguousarray(flag, np.uint8)),
+                          _p(B), _p(C), M, K, N, threads or THREADS)
+    return C
 
 
 def toCSC(dense):

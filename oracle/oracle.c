@@ -192,6 +192,61 @@ int orc_toCSR(const float* dense, int rows, int cols, int* rowPtr, int* colInd, 
     return size;
 }
 
+/* toDataReuseFormat (suitesparse_sdmm.cpp:150-179, datasets/suitesparse.cpp:
+ * 229-263): row-major scan; after each nonzero (row, col) of an even row, the
+ * nonzero (row+1, col) of the next row, if any, follows it with flag 1 and is
+ * erased from the matrix (max_tile = 2), so an odd row keeps only the columns
+ * its even partner lacks.  Works on (and, like the reference, erases entries
+ * of) `dense`; returns the entry count. */
+int orc_toDataReuseFormat(float* dense, int rows, int cols, int* rowPtr, int* colInd, float* val,
+                          unsigned char* flag) {
+    int size = 0;
+    const int max_tile = 2;
+    for (int row = 0; row < rows; row++) {
+        rowPtr[row] = size;
+        for (int col = 0; col < cols; col++) {
+            const float num = dense[(size_t)row * cols + col];
+            if (num != 0) {
+                val[size] = num;
+                colInd[size] = col;
+                flag[size] = 0;
+                size++;
+                for (int cur = row + 1; cur % max_tile != 0; cur++) {
+                    if (cur < rows && dense[(size_t)cur * cols + col] != 0) {
+                        val[size] = dense[(size_t)cur * cols + col];
+                        colInd[size] = col;
+                        flag[size] = 1;
+                        size++;
+                        dense[(size_t)cur * cols + col] = 0;
+                    } else {
+                        break;
+                    }
+                }
+            }
+        }
+    }
+    rowPtr[rows] = size;
+    return size;
+}
+
+/* DataReuseSpmm (suitesparse_sdmm.cpp:25-43): each entry of row m is a
+ * cblas_saxpy of its B row into C row m + flag, rows taken in pairs by one
+ * thread (schedule(static, 2)), so C row m+1 gets the flagged entries of row
+ * m first, then row m+1's own. */
+void orc_DataReuseSpmm(const int* rowPtr, const int* colInd, const float* val, const unsigned char* flag,
+                       const float* B, float* C, int M, int K, int N, int threads) {
+    (void)K;
+#pragma omp parallel for num_threads(threads) schedule(static, 2)
+    for (int m = 0; m < M; m++) {
+        for (int idx = rowPtr[m]; idx < rowPtr[m + 1]; idx++) {
+            const int k = colInd[idx];
+            float* c = C + (size_t)(m + (flag[idx] ? 1 : 0)) * N;
+            const float* b = B + (size_t)k * N;
+            for (int i = 0; i < N; i++) c[i] = fmaf(val[idx], b[i], c[i]);
+        }
+    }
+}
+
 /* toCSC (SpMM.cpp:80-95): column-major scan of a dense matrix. */
 int orc_toCSC(const float* dense, int rows, int cols, int* colPtr, int* rowInd, float* val) {
     int size = 0;

@@ -37,10 +37,10 @@ def test_every_declared_symbol_is_exported():
     compat = re.findall(r"^(?:void)\s+(\w+)\(", open(os.path.join(INCLUDE, "armsparse_compat.h")).read(), re.M)
     assert set(compat) == {"rowBaseSpmm", "rowBaseSpmm2", "rowBaseSpmm3", "rowWiseBlocking",
                            "rowWiseRegisterBlocking", "irregular", "origin", "sparse_matrix_mul3",
-                           "sparse_dense_vector", "csr_spmm_kernel"}
+                           "sparse_dense_vector", "csr_spmm_kernel", "DataReuseSpmm"}
     assert set(compat) <= exported
     # helpers the reference drivers define themselves must not collide
-    for helper in ("toCSR", "prepareSparse", "getHighResolutionTime", "num_vector_neon"):
+    for helper in ("toCSR", "prepareSparse", "getHighResolutionTime", "num_vector_neon", "toDataReuseFormat"):
         assert helper not in exported
 
 

@@ -44,7 +44,7 @@ GOLDEN = os.path.join(ROOT, "tests", "golden")
 
 
 @pytest.mark.parametrize("driver,keys", [
-    ("suitesparse_sdmm", ["m k n ratio:", "rowBaseSpmmTime"]),
+    ("suitesparse_sdmm", ["m k n ratio:", "rowBaseSpmmTime", "DataReuseSpmmTime"]),
     ("suitesparse_spmm", ["tocsrTime:", "m k n ratio:", "SpMMTime", "C nnz:"]),
     ("suitesparse_spmv", ["m k:", "time take:"]),
 ])

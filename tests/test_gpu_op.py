@@ -178,3 +178,51 @@ def test_workspace_reserve_keeps_growing():
     r2 = kernels.workspace_reserve(0)
     r3 = kernels.workspace_reserve(3 << 20)
     assert r1 >= 1 << 20 and r2 == r1 and r3 >= 3 << 20
+
+
+def test_library_calls_replay_in_a_hip_graph():
+    """Merge-path SpMM (partition + carries), a bf16-output reduction GEMM (two
+    workspaces held at once) and the column sum, captured in one HIP graph:
+    new inputs are copied in before each of four replays, with eager library work
+    and a synchronisation between replays, and every replay must equal the eager
+    calls on the same inputs bit for bit."""
+    print("[case] library calls in a HIP graph", flush=True)
+    from armsparse_amd import kernels as K
+    a = data.rmat_csr(13, 1 << 16, seed=3)
+    rp, ci, va = dev(a.rowptr, a.colind, a.values)
+    r = torch.Generator(device="cuda").manual_seed(1)
+    X = torch.empty((a.shape[1], 64), device="cuda")
+    GA = torch.empty((96, 20000), device="cuda")
+    GB = torch.empty((20000, 48), device="cuda")
+    T = torch.empty((9000, 40), device="cuda")
+
+    def fill():
+        for t in (X, GA, GB, T):
+            t.normal_(generator=r)
+
+    def work():
+        y = K.spmm(rp, ci, va, X, config=dict(schedule="merge"))
+        g = K.gemm_bf16(GA.to(torch.bfloat16), GB.to(torch.bfloat16), out_dtype=torch.bfloat16)
+        c = K.colsum(T)
+        return y, g, c
+
+    fill()
+    work()  # eager warm-up: the workspace high-water mark
+    torch.cuda.synchronize()
+    K.workspace_reserve(0)
+    side = torch.cuda.Stream()
+    side.wait_stream(torch.cuda.current_stream())
+    graph = torch.cuda.CUDAGraph()
+    with torch.cuda.stream(side):
+        with torch.cuda.graph(graph, stream=side):
+            static = work()
+    torch.cuda.current_stream().wait_stream(side)
+    for i in range(4):
+        fill()
+        graph.replay()
+        torch.cuda.synchronize()
+        eager = work()
+        K.spmm(rp, ci, va, X)  # unrelated eager library work between replays
+        torch.cuda.synchronize()
+        for s_, e_ in zip(static, eager):
+            assert torch.equal(s_, e_), i

@@ -416,3 +416,58 @@ def test_spmm_max_power_law_and_backward(rmat14):
     rows, cols = np.nonzero(ea >= 0)
     np.add.at(want, (ci[ea[rows, cols]], cols), g[rows, cols])
     assert np.array_equal(db, want)
+
+
+# --------------------------------------------------------- DataReuse SpMM
+@pytest.mark.parametrize("source,n", [("random", 24), ("random", 64), ("odd_rows", 33), ("gre_1107.mtx", 128),
+                                      ("1138_bus.mtx", 40), ("rmat14", 64)])
+def test_datareuse_spmm_bit_exact(source, n, rmat14):
+    """The row-pair kernel against the sequential DataReuseSpmm restatement, bit
+    for bit, overwriting and accumulating; and against FP64 A·B."""
+    case(f"datareuse {source} n={n}")
+    if source == "random":
+        rp, ci, va = random_csr(300, 250, 0.06, n)
+        shape = (300, 250)
+    elif source == "odd_rows":
+        rp, ci, va = random_csr(77, 40, 0.5, 3)  # odd M, dense rows: many flagged entries
+        shape = (77, 40)
+    elif source == "rmat14":
+        a, _ = rmat14
+        rp, ci, va, shape = a.rowptr, a.colind, a.values, a.shape
+    else:
+        rp, ci, va, shape = read_mtx(source)
+    m, k = shape
+    frp, fci, fva, ffl = data.csr_to_datareuse(data.CSR(rp, ci, va, shape))
+    r = np.random.default_rng(n)
+    B = r.standard_normal((k, n)).astype(np.float32)
+    C0 = r.standard_normal((m, n)).astype(np.float32)
+    want = oracle.DataReuseSpmm(frp, fci, fva, ffl, B, np.zeros((m, n), np.float32))
+    want_acc = oracle.DataReuseSpmm(frp, fci, fva, ffl, B, C0.copy())
+    d = dev(frp, fci, fva, ffl, B)
+    got = kernels.spmm_datareuse(*d).cpu().numpy()
+    assert np.array_equal(got.view(np.uint32), want.view(np.uint32))
+    out = torch.from_numpy(C0.copy()).cuda()
+    kernels.spmm_datareuse(*d, out=out, accumulate=True)
+    assert np.array_equal(out.cpu().numpy().view(np.uint32), want_acc.view(np.uint32))
+    ref = oracle.spmm_f64(rp, ci, va, B)
+    bound = oracle.spmm_f64(rp, ci, np.abs(va), np.abs(B))
+    assert np.all(np.abs(got - ref) <= 1e-5 * bound + 1e-30)
+
+
+def test_datareuse_shim_host_arrays():
+    """The reference-name shim DataReuseSpmm (C++ linkage, host arrays, C += A·B)
+    on the golden 8×7 fixture's DataReuse format: C gains the reference's row sums."""
+    import ctypes
+
+    from armsparse_amd import _lib
+    case("DataReuseSpmm shim")
+    from conftest import FIX8_COLIND, FIX8_ROWPTR, FIX8_ROWSUMS, FIX8_VAL
+    frp, fci, fva, ffl = oracle.toDataReuseFormat(oracle.densify(FIX8_ROWPTR, FIX8_COLIND, FIX8_VAL, 7))
+    fn = _lib.lib["_Z13DataReuseSpmmPiS_PfS0_S0_iiiiPb"]
+    fn.restype = None
+    B = np.ones((7, 6), np.float32)
+    C = np.full((8, 6), 2.0, np.float32)
+    fl = ffl.astype(np.bool_)
+    p = lambda a: a.ctypes.data_as(ctypes.c_void_p)  # noqa: E731
+    fn(p(frp), p(fci), p(fva), p(B), p(C), ctypes.c_int(8), ctypes.c_int(7), ctypes.c_int(6), ctypes.c_int(8), p(fl))
+    assert np.array_equal(C, 2.0 + np.repeat(FIX8_ROWSUMS[:, None], 6, axis=1))
