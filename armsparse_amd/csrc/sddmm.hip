@@ -173,17 +173,34 @@ __device__ __forceinline__ void gather_vec(__amdgpu_buffer_rsrc_t rs, uint32_t o
     }
 }
 
+// S = 2: the dot products are split into two column halves (Kd is the half),
+// and blocks are numbered so that the 8 XCDs (block b runs on XCD b % 8)
+// alternate halves: each XCD's L2 then caches half of every hot V row.  Half 0
+// goes to out, half 1 to out2; k_sddmm_combine adds them and scales.
 template <typename T, int G, int W, bool COO, bool FULL>
 __global__ __launch_bounds__(256) void k_sddmm_stream(int M, int Kd, int nnz, int chunk,
                                                       const int* __restrict__ rowidx, const int* __restrict__ colind,
                                                       const float* __restrict__ scale, const T* __restrict__ U,
                                                       int ldu, const T* __restrict__ V, int ldv,
-                                                      const int* __restrict__ part, float* __restrict__ out) {
+                                                      const int* __restrict__ part, float* __restrict__ out, int S,
+                                                      int ngroups, float* __restrict__ out2) {
     extern __shared__ __align__(16) unsigned char smem[];
     const int lane = threadIdx.x & 63;
     const int wid = threadIdx.x >> 6;
     const int g = lane / G, l = lane & (G - 1);
-    const int wave = uniform((int)(blockIdx.x * 4u + wid));
+    int grp = (int)blockIdx.x;
+    if (S == 2) {
+        const int x = (int)(blockIdx.x & 7u);
+        grp = (int)(blockIdx.x >> 3) * 4 + (x >> 1);
+        if (grp >= ngroups) return;
+        if (x & 1) {  // the upper half of every row of U and V
+            U += Kd;
+            V += Kd;
+            out = out2;
+        }
+        scale = nullptr;
+    }
+    const int wave = uniform(grp * 4 + wid);
     const int64_t total = COO ? (int64_t)nnz : (int64_t)M + nnz;
     const int64_t d0l = (int64_t)wave * chunk;
     if (d0l >= total) return;
@@ -291,6 +308,15 @@ __global__ __launch_bounds__(256) void k_sddmm_stream(int M, int Kd, int nnz, in
     }
 }
 
+// out[e] = scale[e] · (out[e] + out2[e]): the two column halves of the split kernel.
+__global__ __launch_bounds__(256) void k_sddmm_combine(int nnz, const float* __restrict__ scale,
+                                                       float* __restrict__ out, const float* __restrict__ out2) {
+    const int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (e >= nnz) return;
+    const float d = out[e] + out2[e];
+    out[e] = scale ? scale[e] * d : d;
+}
+
 inline size_t sddmm_lds_bytes(int chunk) {
     return 4 * (((size_t)chunk * 8 + (size_t)(chunk + 1) * 4 + 15) & ~(size_t)15);
 }
@@ -298,12 +324,12 @@ inline size_t sddmm_lds_bytes(int chunk) {
 template <typename T, int G, int W>
 void launch_sddmm_stream(bool coo, dim3 grid, hipStream_t s, int M, int Kd, int nnz, int chunk, const int* ri,
                          const int* ci, const float* sc, const T* U, int ldu, const T* V, int ldv, const int* part,
-                         float* out) {
+                         float* out, int S, int ngroups, float* out2) {
     const size_t lds = sddmm_lds_bytes(chunk);
     const bool full = Kd % (G * W) == 0;
 #define ARMSP_SDDMM_ST(CO, FU)                                                                                 \
     hipLaunchKernelGGL((k_sddmm_stream<T, G, W, CO, FU>), grid, dim3(256), lds, s, M, Kd, nnz, chunk, ri, ci, sc, U, \
-                       ldu, V, ldv, part, out)
+                       ldu, V, ldv, part, out, S, ngroups, out2)
     if (coo) {
         if (full) ARMSP_SDDMM_ST(true, true); else ARMSP_SDDMM_ST(true, false);
     } else {
@@ -361,14 +387,28 @@ int sddmm_impl(bool coo, int M, int K, int Kd, int nnz, const int* rowidx, const
     const int64_t items = (staged && !coo) ? (int64_t)M + nnz : (int64_t)nnz;
     const int64_t nwaves = (items + chunk - 1) / chunk;
     if (nwaves > INT_MAX / 4) return ARMSP_ERROR_NOT_SUPPORTED;
-    const dim3 grid((unsigned)((nwaves + 3) / 4));
+    // AUTO, fp32 rows of 512 bytes or more: split the dot products into column
+    // halves spread over the XCDs (each XCD's L2 then holds half of every hot V
+    // row), plus one pass that adds the halves
+    const int KdS = Kd / 2;
+    const bool split = c.schedule == ARMSP_SCHED_AUTO && c.group <= 0 && staged && !coo && sizeof(T) == 4 &&
+                       Kd * (int)sizeof(T) >= 512 && KdS % W == 0 && aligned(U + KdS, 16) && aligned(V + KdS, 16);
+    if (split) {
+        G = 4;
+        while (G < (W == 8 ? 4 : 8) && G * W < KdS) G <<= 1;
+    }
+    const int ngroups = (int)((nwaves + 3) / 4);
+    const dim3 grid(split ? (unsigned)(((int64_t)ngroups + 3) / 4 * 8) : (unsigned)ngroups);
     // the staged CSR kernel starts each wave from a precomputed merge-path partition
     void* ws = nullptr;
     int* part = nullptr;
+    float* out2 = nullptr;
     if (staged && !coo) {
-        int st = ws_alloc(&ws, (size_t)(nwaves + 1) * sizeof(int), s);
+        const size_t pb = ((size_t)(nwaves + 1) * sizeof(int) + 255) & ~(size_t)255;
+        int st = ws_alloc(&ws, pb + (split ? (size_t)nnz * sizeof(float) : 0), s);
         if (st) return st;
         part = static_cast<int*>(ws);
+        if (split) out2 = reinterpret_cast<float*>(static_cast<char*>(ws) + pb);
         TimedLaunch tp("merge_partition", s);
         st = merge_partition(M, nnz, chunk, nwaves, rowidx, part, s);
         if (st) {
@@ -381,10 +421,10 @@ int sddmm_impl(bool coo, int M, int K, int Kd, int nnz, const int* rowidx, const
 #define ARMSP_SDDMM_G(WW)                                                                                    \
     if (staged) {                                                                                            \
         switch (G) {                                                                                         \
-            case 4: launch_sddmm_stream<T, 4, WW>(coo, grid, s, M, Kd, nnz, chunk, rowidx, colind, scale, U, ldu, V, ldv, part, out); break; \
-            case 8: launch_sddmm_stream<T, 8, WW>(coo, grid, s, M, Kd, nnz, chunk, rowidx, colind, scale, U, ldu, V, ldv, part, out); break; \
-            case 16: launch_sddmm_stream<T, 16, WW>(coo, grid, s, M, Kd, nnz, chunk, rowidx, colind, scale, U, ldu, V, ldv, part, out); break; \
-            default: launch_sddmm_stream<T, 32, WW>(coo, grid, s, M, Kd, nnz, chunk, rowidx, colind, scale, U, ldu, V, ldv, part, out); break; \
+            case 4: launch_sddmm_stream<T, 4, WW>(coo, grid, s, M, split ? KdS : Kd, nnz, chunk, rowidx, colind, scale, U, ldu, V, ldv, part, out, split ? 2 : 1, ngroups, out2); break; \
+            case 8: launch_sddmm_stream<T, 8, WW>(coo, grid, s, M, split ? KdS : Kd, nnz, chunk, rowidx, colind, scale, U, ldu, V, ldv, part, out, split ? 2 : 1, ngroups, out2); break; \
+            case 16: launch_sddmm_stream<T, 16, WW>(coo, grid, s, M, split ? KdS : Kd, nnz, chunk, rowidx, colind, scale, U, ldu, V, ldv, part, out, split ? 2 : 1, ngroups, out2); break; \
+            default: launch_sddmm_stream<T, 32, WW>(coo, grid, s, M, split ? KdS : Kd, nnz, chunk, rowidx, colind, scale, U, ldu, V, ldv, part, out, split ? 2 : 1, ngroups, out2); break; \
         }                                                                                                    \
     } else {                                                                                                 \
         switch (G) {                                                                                         \
@@ -400,6 +440,9 @@ int sddmm_impl(bool coo, int M, int K, int Kd, int nnz, const int* rowidx, const
         ARMSP_SDDMM_G(WMAX)
     }
 #undef ARMSP_SDDMM_G
+    if (split)
+        hipLaunchKernelGGL(k_sddmm_combine, dim3((unsigned)(((int64_t)nnz + 255) / 256)), dim3(256), 0, s, nnz,
+                           scale, out, out2);
     const hipError_t e = hipGetLastError();
     if (ws) ws_free(ws, s);
     if (e != hipSuccess) return hip_fail(e);

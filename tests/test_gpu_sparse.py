@@ -79,7 +79,7 @@ def test_spmv_full_rmat_1m():
 # ----------------------------------------------------------------- SDDMM
 # schedule "merge" (and AUTO) runs the LDS-staged kernel over merge-path chunks,
 # "nnz" the unstaged nnz-split kernel
-@pytest.mark.parametrize("kd", [1, 3, 8, 32, 100, 128])
+@pytest.mark.parametrize("kd", [1, 3, 8, 32, 100, 128, 256])
 @pytest.mark.parametrize("group", [0, 4, 8, 16, 32])
 @pytest.mark.parametrize("sched", ["merge", "nnz"])
 def test_sddmm_csr(rmat14, kd, group, sched):
@@ -120,11 +120,14 @@ def test_sddmm_coo_and_bf16(rmat14):
         assert np.all(np.abs(out - refb) <= TOL * bound * 1.01 + 1e-30)
 
 
-def test_sddmm_full_rmat_1m_kd32():
-    case("sddmm full 1M R-MAT Kd=32")
+@pytest.mark.parametrize("kd", [32, 128])
+def test_sddmm_full_rmat_1m(kd):
+    """The 1M R-MAT at both widths of the workload; Kd=128 runs the AUTO split
+    (column halves over the XCDs, then the combine pass)."""
+    case(f"sddmm full 1M R-MAT Kd={kd}")
     a = rmat_1m_cached()
-    U = data.host_normal(a.shape[0] * 32, data.CFG_SEED + 1, data.STREAM_DENSE + 2).reshape(-1, 32)
-    V = data.host_normal(a.shape[1] * 32, data.CFG_SEED + 1, data.STREAM_DENSE + 3).reshape(-1, 32)
+    U = data.host_normal(a.shape[0] * kd, data.CFG_SEED + 1, data.STREAM_DENSE + 2).reshape(-1, kd)
+    V = data.host_normal(a.shape[1] * kd, data.CFG_SEED + 1, data.STREAM_DENSE + 3).reshape(-1, kd)
     d_rp, d_ci, d_va, d_U, d_V = dev(a.rowptr, a.colind, a.values, U, V)
     out = kernels.sddmm(d_rp, d_ci, d_U, d_V, scale=d_va).cpu().numpy()
     ref = oracle.sddmm_f64(a.rowptr, a.colind, U, V, scale=a.values)
