@@ -387,13 +387,17 @@ int sddmm_impl(bool coo, int M, int K, int Kd, int nnz, const int* rowidx, const
     const int64_t items = (staged && !coo) ? (int64_t)M + nnz : (int64_t)nnz;
     const int64_t nwaves = (items + chunk - 1) / chunk;
     if (nwaves > INT_MAX / 4) return ARMSP_ERROR_NOT_SUPPORTED;
-    // AUTO, fp32 rows of 512 bytes or more: split the dot products into column
-    // halves spread over the XCDs (each XCD's L2 then holds half of every hot V
-    // row), plus one pass that adds the halves
+    // unroll = 2: split the dot products into column halves spread over the XCDs
+    // (each XCD's L2 then holds half of every hot V row), plus one pass that adds
+    // the halves — a win on skewed columns whose V rows outgrow the L2 (the 1M
+    // R-MAT at Kd = 128: 13%), a loss where columns are uniform or local, so it is
+    // the selector's choice, not AUTO's.  Where it cannot apply (COO, the unstaged
+    // kernel, odd halves) the call runs unsplit.
+    if (c.unroll < 0 || c.unroll > 2) return ARMSP_ERROR_INVALID_VALUE;
     const int KdS = Kd / 2;
-    const bool split = c.schedule == ARMSP_SCHED_AUTO && c.group <= 0 && staged && !coo && sizeof(T) == 4 &&
-                       Kd * (int)sizeof(T) >= 512 && KdS % W == 0 && aligned(U + KdS, 16) && aligned(V + KdS, 16);
-    if (split) {
+    const bool split = c.unroll == 2 && staged && !coo && KdS > 0 && Kd % 2 == 0 && KdS % W == 0 &&
+                       aligned(U + KdS, W * (int)sizeof(T)) && aligned(V + KdS, W * (int)sizeof(T));
+    if (split && c.group <= 0) {
         G = 4;
         while (G < (W == 8 ? 4 : 8) && G * W < KdS) G <<= 1;
     }

@@ -53,7 +53,8 @@ SPMM_CONFIGS = [None,
                 dict(schedule="merge", group=32, vec=2, chunk=256, unroll=4),
                 dict(schedule="merge", group=32, vec=2, chunk=512, unroll=8)]
 SDDMM_CONFIGS = ([None] + [dict(schedule="merge", group=g, chunk=c) for g in (4, 8, 16) for c in (128, 256, 512)]
-                 + [dict(schedule="nnz", group=g, chunk=256) for g in (8, 16)])
+                 + [dict(schedule="nnz", group=g, chunk=256) for g in (8, 16)]
+                 + [dict(schedule="merge", group=8, chunk=c, unroll=2) for c in (128, 256)])  # column halves
 CONFIGS = {"spmm": SPMM_CONFIGS, "sddmm": SDDMM_CONFIGS}
 
 

@@ -215,6 +215,11 @@ def main():
             sb = 4 * (M + 1) + 12 * nnz + 2 * 4 * M * kd
             extras[f"sddmm_kd{kd}_GBps"] = round(sb * steps_x / el_s / 1e9, 1)
             extras[f"sddmm_kd{kd}_us"] = round(el_s / steps_x * 1e6, 1)
+            if kd == 128:  # dot products split into column halves over the XCDs (a selector candidate)
+                split = _lib.Config.make("merge", unroll=2)
+                el_s, _ = timed(lambda: kernels.sddmm(rp, ci, U, V, scale=va, config=split), steps_x, 3, [])
+                extras["sddmm_kd128_split_GBps"] = round(sb * steps_x / el_s / 1e9, 1)
+                extras["sddmm_kd128_split_us"] = round(el_s / steps_x * 1e6, 1)
             del U, V
     if "gcn" in only:
         from armsparse_amd import gcn

@@ -54,13 +54,19 @@ typedef enum armsp_schedule {
 typedef struct armsp_config {
     int32_t schedule; /* armsp_schedule */
     int32_t group;    /* lanes per row (ROW) or per nonzero (SDDMM): 1,2,4,...,64.
-                         SpMM MERGE: 0/64 a whole wave per nonzero; 32 half a wave per
-                         nonzero over 128-byte column slices of B; 1 forces the
-                         64-bit-address kernel (otherwise used only when a dense-row
-                         offset needs more than 32 bits) */
+                         SpMM MERGE: 0 automatic (lane walks for N <= 8, half-wave
+                         slices with 8-byte lanes for B rows of 512+ bytes, else a
+                         whole wave per nonzero); 64 a whole wave per nonzero; 32 half
+                         a wave per nonzero over column slices of 32·vec elements
+                         spread over the XCDs; 1 forces the 64-bit-address kernel
+                         (otherwise used only when a dense-row offset needs more than
+                         32 bits) */
     int32_t vec;      /* dense elements per lane: 1, 2, 4 (8 for bf16/fp8 B) */
     int32_t chunk;    /* items per wave for NNZ / MERGE (multiple of 64) */
-    int32_t unroll;   /* nonzeros whose dense rows are fetched together: 1,2,4,8,16 */
+    int32_t unroll;   /* SpMM: nonzeros whose dense rows are fetched together: 4, 8, 16.
+                         SDDMM (staged CSR kernel): 2 splits each dot product into
+                         column halves spread over the XCDs plus a combine pass (pays
+                         on skewed columns whose V rows outgrow the L2); 0/1 none */
 } armsp_config;
 
 /* flags for the SpMM / SpMV entry points */

@@ -81,7 +81,7 @@ def test_spmv_full_rmat_1m():
 # "nnz" the unstaged nnz-split kernel
 @pytest.mark.parametrize("kd", [1, 3, 8, 32, 100, 128, 256])
 @pytest.mark.parametrize("group", [0, 4, 8, 16, 32])
-@pytest.mark.parametrize("sched", ["merge", "nnz"])
+@pytest.mark.parametrize("sched", ["merge", "nnz", "merge-split"])
 def test_sddmm_csr(rmat14, kd, group, sched):
     case(f"sddmm kd={kd} G={group} {sched}")
     a, (d_rp, d_ci, d_va) = rmat14
@@ -89,7 +89,9 @@ def test_sddmm_csr(rmat14, kd, group, sched):
     U = r.standard_normal((a.shape[0], kd)).astype(np.float32)
     V = r.standard_normal((a.shape[1], kd)).astype(np.float32)
     d_U, d_V = dev(U, V)
-    if group == 0:
+    if sched == "merge-split":  # column halves over the XCDs + combine (odd Kd runs unsplit)
+        cfg = dict(schedule="merge", group=group, unroll=2)
+    elif group == 0:
         cfg = None if sched == "merge" else dict(schedule="nnz")
     else:
         cfg = dict(schedule=sched, group=group, chunk=512 if sched == "merge" else 0)
@@ -120,16 +122,17 @@ def test_sddmm_coo_and_bf16(rmat14):
         assert np.all(np.abs(out - refb) <= TOL * bound * 1.01 + 1e-30)
 
 
-@pytest.mark.parametrize("kd", [32, 128])
-def test_sddmm_full_rmat_1m(kd):
-    """The 1M R-MAT at both widths of the workload; Kd=128 runs the AUTO split
-    (column halves over the XCDs, then the combine pass)."""
-    case(f"sddmm full 1M R-MAT Kd={kd}")
+@pytest.mark.parametrize("kd,split", [(32, False), (128, False), (128, True)])
+def test_sddmm_full_rmat_1m(kd, split):
+    """The 1M R-MAT at both widths of the workload, and Kd=128 split into column
+    halves over the XCDs (then the combine pass)."""
+    case(f"sddmm full 1M R-MAT Kd={kd} split={split}")
     a = rmat_1m_cached()
     U = data.host_normal(a.shape[0] * kd, data.CFG_SEED + 1, data.STREAM_DENSE + 2).reshape(-1, kd)
     V = data.host_normal(a.shape[1] * kd, data.CFG_SEED + 1, data.STREAM_DENSE + 3).reshape(-1, kd)
     d_rp, d_ci, d_va, d_U, d_V = dev(a.rowptr, a.colind, a.values, U, V)
-    out = kernels.sddmm(d_rp, d_ci, d_U, d_V, scale=d_va).cpu().numpy()
+    cfg = dict(schedule="merge", unroll=2) if split else None
+    out = kernels.sddmm(d_rp, d_ci, d_U, d_V, scale=d_va, config=cfg).cpu().numpy()
     ref = oracle.sddmm_f64(a.rowptr, a.colind, U, V, scale=a.values)
     bound = oracle.sddmm_f64(a.rowptr, a.colind, np.abs(U), np.abs(V), scale=np.abs(a.values))
     assert np.all(np.abs(out - ref) <= TOL * bound + 1e-30)
