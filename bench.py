@@ -181,10 +181,24 @@ def main():
                                  (As.rowptr, As.colind, As.values, Bs, xs))
         sC = torch.empty((4096, 64), device=dev)
         sy = torch.empty(4096, device=dev)
-        el_a, _ = timed(lambda: kernels.spmm(srp, sci, sva, sB, out=sC), 200, 10, [])
-        el_b, _ = timed(lambda: kernels.spmv(srp, sci, sva, sx, out=sy), 200, 10, [])
-        extras["cfg0_spmm_us"] = round(el_a / 200 * 1e6, 2)
+        call_a = lambda: kernels.spmm(srp, sci, sva, sB, out=sC)  # noqa: E731
+        call_b = lambda: kernels.spmv(srp, sci, sva, sx, out=sy)  # noqa: E731
+        el_a, _ = timed(call_a, 200, 10, [])
+        el_b, _ = timed(call_b, 200, 10, [])
+        extras["cfg0_spmm_us"] = round(el_a / 200 * 1e6, 2)  # per Python call (host-bound at this size)
         extras["cfg0_spmv_us"] = round(el_b / 200 * 1e6, 2)
+        # the same call captured once in a HIP graph and replayed: the device-side cost
+        kernels.workspace_reserve(0)
+        for nm, fn in (("spmm", call_a), ("spmv", call_b)):
+            side = torch.cuda.Stream()
+            side.wait_stream(torch.cuda.current_stream())
+            g = torch.cuda.CUDAGraph()
+            with torch.cuda.stream(side):
+                with torch.cuda.graph(g, stream=side):
+                    fn()
+            torch.cuda.current_stream().wait_stream(side)
+            el_g, _ = timed(g.replay, 200, 10, [])
+            extras[f"cfg0_{nm}_graph_us"] = round(el_g / 200 * 1e6, 2)
         A6 = data.uniform_65k()
         r6, c6, v6 = (torch.from_numpy(t).to(dev) for t in (A6.rowptr, A6.colind, A6.values))
         x6 = torch.empty(65536, device=dev)
@@ -215,6 +229,13 @@ def main():
             sb = 4 * (M + 1) + 12 * nnz + 2 * 4 * M * kd
             extras[f"sddmm_kd{kd}_GBps"] = round(sb * steps_x / el_s / 1e9, 1)
             extras[f"sddmm_kd{kd}_us"] = round(el_s / steps_x * 1e6, 1)
+            # bf16 operands (the GCN's dA: features in bf16), fp32 accumulation
+            Ub, Vb = U.to(torch.bfloat16), V.to(torch.bfloat16)
+            el_b, _ = timed(lambda: kernels.sddmm(rp, ci, Ub, Vb, scale=va), steps_x, 3, [])
+            sbb = 4 * (M + 1) + 12 * nnz + 2 * 2 * M * kd
+            extras[f"sddmm_kd{kd}_bf16_GBps"] = round(sbb * steps_x / el_b / 1e9, 1)
+            extras[f"sddmm_kd{kd}_bf16_us"] = round(el_b / steps_x * 1e6, 1)
+            del Ub, Vb
             if kd == 128:  # dot products split into column halves over the XCDs (a selector candidate)
                 split = _lib.Config.make("merge", unroll=2)
                 el_s, _ = timed(lambda: kernels.sddmm(rp, ci, U, V, scale=va, config=split), steps_x, 3, [])

@@ -477,3 +477,24 @@ def test_datareuse_shim_host_arrays():
     p = lambda a: a.ctypes.data_as(ctypes.c_void_p)  # noqa: E731
     fn(p(frp), p(fci), p(fva), p(B), p(C), ctypes.c_int(8), ctypes.c_int(7), ctypes.c_int(6), ctypes.c_int(8), p(fl))
     assert np.array_equal(C, 2.0 + np.repeat(FIX8_ROWSUMS[:, None], 6, axis=1))
+
+
+def test_sddmm_operands_beyond_4gib():
+    """U and V with a row stride of 1100 floats over 2^20 rows (4.4 GB each): the
+    staged kernel's 32-bit offsets cannot reach them, so the call takes the
+    unstaged kernel with 64-bit addresses.  Against FP64."""
+    case("sddmm U, V beyond 4 GiB")
+    n, ld, kd = 1 << 20, 1100, 8
+    a = data.uniform_csr(n, n, 1 << 20, seed=43)
+    Uf = torch.empty((n, ld), device="cuda")
+    Vf = torch.empty((n, ld), device="cuda")
+    kernels.fill_normal_(Uf, 8, 16)
+    kernels.fill_normal_(Vf, 9, 16)
+    U, V = Uf[:, :kd], Vf[:, :kd]
+    d_rp, d_ci, d_va = dev(a.rowptr, a.colind, a.values)
+    out = kernels.sddmm(d_rp, d_ci, U, V, scale=d_va).cpu().numpy()
+    Uh, Vh = U.contiguous().cpu().numpy(), V.contiguous().cpu().numpy()
+    del Uf, Vf, U, V
+    ref = oracle.sddmm_f64(a.rowptr, a.colind, Uh, Vh, scale=a.values)
+    bound = oracle.sddmm_f64(a.rowptr, a.colind, np.abs(Uh), np.abs(Vh), scale=np.abs(a.values))
+    assert np.all(np.abs(out - ref) <= TOL * bound + 1e-30)

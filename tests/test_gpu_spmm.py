@@ -314,3 +314,25 @@ def test_epilogue_fusion(rmat14, sched, accumulate):
     scale = rs[:, None] * oracle.spmm_f64(a.rowptr, a.colind, np.abs(a.values * cs[a.colind]), np.abs(B)) + 2
     assert np.all(np.abs(out - ref) <= TOL * scale)
     assert (out >= 0).all()
+
+
+@pytest.mark.parametrize("sched", [None, "row", "nnz"])
+def test_dense_operand_beyond_4gib(sched):
+    """B with a row stride of 1100 floats over 2^20 rows (4.4 GB): byte offsets of
+    its last rows exceed 32 bits, so every schedule must address it with 64-bit
+    offsets by itself (AUTO switches to the 64-bit merge kernel); uniform columns
+    reach the far rows.  Against FP64."""
+    case(f"B beyond 4 GiB, schedule {sched}")
+    K, ld, N = 1 << 20, 1100, 64
+    a = data.uniform_csr(1 << 14, K, 1 << 20, seed=41)
+    full = torch.empty((K, ld), device="cuda")
+    kernels.fill_uniform_(full, 7, 16)
+    B = full[:, :N]
+    assert B.stride(0) * K * 4 > 1 << 32
+    d_rp, d_ci, d_va = dev(a.rowptr, a.colind, a.values)
+    out = kernels.spmm(d_rp, d_ci, d_va, B, config=sched).cpu().numpy()
+    Bh = B.contiguous().cpu().numpy()
+    del full, B
+    f64 = oracle.spmm_f64(a.rowptr, a.colind, a.values, Bh)
+    bound = oracle.spmm_f64(a.rowptr, a.colind, np.abs(a.values), np.abs(Bh))
+    assert np.all(np.abs(out - f64) <= 1e-5 * bound + 1e-30)
