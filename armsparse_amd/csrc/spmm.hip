@@ -747,7 +747,7 @@ __global__ __launch_bounds__(256) void k_spmm_slice(int M, int N, int nnz, int c
 }
 
 // ---------------------------------------------- MERGE split for narrow N
-// For N ≤ 8 a wave per nonzero would leave most lanes idle, and a row per
+// For N ≤ 4 a wave per nonzero would leave most lanes idle, and a row per
 // lane group stalls on hub rows.  Here the wave first stages the products
 // a·B[c, 0..N) of its tile's nonzeros in LDS (every lane issues the gathers of
 // IPT nonzeros at once), then each lane walks IPT consecutive items of the
@@ -1149,8 +1149,7 @@ int launch_narrow(int ipt, int64_t nwaves, hipStream_t s, int M, int N, int nnz,
                   const int* part, int* crow, float* cval) {
     if (N <= 1) return launch_narrow_nv<TB, 1>(ipt, nwaves, s, M, N, nnz, rp, ci, va, B, ldb, bs, C, ldc, flags, part, crow, cval);
     if (N <= 2) return launch_narrow_nv<TB, 2>(ipt, nwaves, s, M, N, nnz, rp, ci, va, B, ldb, bs, C, ldc, flags, part, crow, cval);
-    if (N <= 4) return launch_narrow_nv<TB, 4>(ipt, nwaves, s, M, N, nnz, rp, ci, va, B, ldb, bs, C, ldc, flags, part, crow, cval);
-    return launch_narrow_nv<TB, 8>(ipt, nwaves, s, M, N, nnz, rp, ci, va, B, ldb, bs, C, ldc, flags, part, crow, cval);
+    return launch_narrow_nv<TB, 4>(ipt, nwaves, s, M, N, nnz, rp, ci, va, B, ldb, bs, C, ldc, flags, part, crow, cval);
 }
 
 // MERGE gathers through 32-bit buffer offsets unless a dense-row offset could exceed them
@@ -1182,9 +1181,12 @@ int spmm_impl(int M, int K, int N, int nnz, const int* rowptr, const int* colind
 
     armsp_config c = cfg ? *cfg : armsp_config{0, 0, 0, 0, 0};
     int sched = c.schedule;
-    // AUTO: merge path except for 9..31 columns (a row per lane group); N ≤ 8 runs the
-    // per-lane item walk (k_spmm_narrow), wider N a whole wave per nonzero
-    if (sched == ARMSP_SCHED_AUTO) sched = (N >= 32 || N <= 8) ? ARMSP_SCHED_MERGE : ARMSP_SCHED_ROW;
+    // AUTO: the merge path at every width — N ≤ 4 runs the per-lane item walk
+    // (k_spmm_narrow), wider N a whole wave or half waves per nonzero.  (A row per
+    // lane group is often faster on regular rows, but a hub row stalls its group:
+    // 20x slower on the 1M R-MAT at N = 8; the schedule selector picks it where it
+    // pays.)
+    if (sched == ARMSP_SCHED_AUTO) sched = ARMSP_SCHED_MERGE;
     if (flags & ARMSP_TRUNC4) sched = ARMSP_SCHED_ROW;  // truncation is per row
     if (nnz == 0) sched = ARMSP_SCHED_ROW;               // writes (or keeps) every row
 
@@ -1227,11 +1229,13 @@ int spmm_impl(int M, int K, int N, int nnz, const int* rowptr, const int* colind
     if (chunk % 64) return ARMSP_ERROR_INVALID_VALUE;
     // dense-row byte offsets in 32 bits: (K-1)·ldb + N elements from B
     const bool wide = c.group == 1 || ((uint64_t)K * (uint64_t)ldb * sizeof(TB)) > 0xffffffffull;
-    // N ≤ 8 (group 0): lanes walk runs of 4, 8 or 16 merge items (k_spmm_narrow); the
-    // wave tile is 64 × that run, so the chunk is rounded to 256, 512 or 1024
-    const bool narrow = merge && !wide && N <= 8 && c.group == 0;
-    // AUTO half-wave slices (see below) run best with 256 items per wave and 4 gathers per batch
-    if (merge && !wide && !narrow && c.group == 0 && N * (int)sizeof(TB) >= 512) {
+    // N ≤ 4 (group 0): lanes walk runs of 4, 8 or 16 merge items (k_spmm_narrow); the
+    // wave tile is 64 × that run, so the chunk is rounded to 256, 512 or 1024.  From
+    // N = 5 up a whole wave per nonzero is faster (1M R-MAT, N = 8: 238 vs 347 us)
+    const bool narrow = merge && !wide && N <= 4 && c.group == 0;
+    // AUTO half-wave slices (see below), and a whole wave per nonzero on narrow rows of
+    // B (N < 32), run best with 256 items per wave and 4 gathers per batch
+    if (merge && !wide && !narrow && c.group == 0 && (N * (int)sizeof(TB) >= 512 || N < 32)) {
         if (c.chunk <= 0) chunk = 256;
         if (c.unroll <= 0) U = 4;
     }

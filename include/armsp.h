@@ -54,7 +54,7 @@ typedef enum armsp_schedule {
 typedef struct armsp_config {
     int32_t schedule; /* armsp_schedule */
     int32_t group;    /* lanes per row (ROW) or per nonzero (SDDMM): 1,2,4,...,64.
-                         SpMM MERGE: 0 automatic (lane walks for N <= 8, half-wave
+                         SpMM MERGE: 0 automatic (lane walks for N <= 4, half-wave
                          slices with 8-byte lanes for B rows of 512+ bytes, else a
                          whole wave per nonzero); 64 a whole wave per nonzero; 32 half
                          a wave per nonzero over column slices of 32·vec elements

@@ -161,8 +161,9 @@ def test_split_widths(rmat14, N, sched):
 @pytest.mark.parametrize("N", [1, 2, 3, 4, 5, 7, 8])
 @pytest.mark.parametrize("chunk", [256, 512, 1024])
 def test_narrow_merge(rmat14, N, chunk):
-    """N ≤ 8 with the merge schedule: lanes walk runs of merge items (hub rows of the
-    power-law matrix are cut across lanes and waves); overwrite and accumulate."""
+    """Narrow N with the merge schedule: for N ≤ 4 lanes walk runs of merge items
+    (hub rows of the power-law matrix are cut across lanes and waves), from N = 5
+    a whole wave per nonzero; overwrite and accumulate."""
     case(f"narrow N={N} chunk={chunk}")
     a, (d_rp, d_ci, d_va) = rmat14
     r = np.random.default_rng(100 + N)
