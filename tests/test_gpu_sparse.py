@@ -498,3 +498,47 @@ def test_sddmm_operands_beyond_4gib():
     ref = oracle.sddmm_f64(a.rowptr, a.colind, Uh, Vh, scale=a.values)
     bound = oracle.sddmm_f64(a.rowptr, a.colind, np.abs(Uh), np.abs(Vh), scale=np.abs(a.values))
     assert np.all(np.abs(out - ref) <= TOL * bound + 1e-30)
+
+
+def test_concurrent_calls_from_host_threads(rmat14):
+    """Four host threads, each with its own stream, call SpMM (merge path:
+    partition and carry workspaces from the shared pool), SpMV and SDDMM
+    repeatedly; every result equals the serial one bit for bit (the NUMA
+    drivers call the kernels from four pthreads at once)."""
+    import threading
+    case("concurrent host threads")
+    a, (d_rp, d_ci, d_va) = rmat14
+    r = np.random.default_rng(77)
+    Bs = [torch.from_numpy(r.standard_normal((a.shape[1], 64)).astype(np.float32)).cuda() for _ in range(4)]
+    xs = [torch.from_numpy(r.standard_normal(a.shape[1]).astype(np.float32)).cuda() for _ in range(4)]
+    U = torch.from_numpy(r.standard_normal((a.shape[0], 32)).astype(np.float32)).cuda()
+
+    def work(i):
+        return (kernels.spmm(d_rp, d_ci, d_va, Bs[i]), kernels.spmv(d_rp, d_ci, d_va, xs[i]),
+                kernels.sddmm(d_rp, d_ci, U, Bs[i][:, :32].contiguous(), scale=d_va))
+
+    serial = [work(i) for i in range(4)]
+    torch.cuda.synchronize()
+    results, errors = [[None] * 10 for _ in range(4)], []
+
+    def run(i):
+        try:
+            s = torch.cuda.Stream()
+            with torch.cuda.stream(s):
+                for k in range(10):
+                    results[i][k] = work(i)
+            s.synchronize()
+        except Exception as e:  # surfaced below
+            errors.append(e)
+
+    threads = [threading.Thread(target=run, args=(i,)) for i in range(4)]
+    for t in threads:
+        t.start()
+    for t in threads:
+        t.join()
+    torch.cuda.synchronize()
+    assert not errors, errors
+    for i in range(4):
+        for k in range(10):
+            for got, want in zip(results[i][k], serial[i]):
+                assert torch.equal(got, want), (i, k)
