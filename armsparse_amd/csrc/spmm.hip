@@ -1235,7 +1235,9 @@ int spmm_impl(int M, int K, int N, int nnz, const int* rowptr, const int* colind
     const bool narrow = merge && !wide && N <= 4 && c.group == 0;
     // AUTO half-wave slices (see below), and a whole wave per nonzero on narrow rows of
     // B (N < 32), run best with 256 items per wave and 4 gathers per batch
-    if (merge && !wide && !narrow && c.group == 0 && (N * (int)sizeof(TB) >= 512 || N < 32)) {
+    const int esz = (int)sizeof(TB);
+    const bool slice_rows = N * esz >= 512 || (esz == 2 && N * esz >= 128);
+    if (merge && !wide && !narrow && c.group == 0 && (slice_rows || N < 32)) {
         if (c.chunk <= 0) chunk = 256;
         if (c.unroll <= 0) U = 4;
     }
@@ -1277,11 +1279,14 @@ int spmm_impl(int M, int K, int N, int nnz, const int* rowptr, const int* colind
     // nonzero with 8-byte lanes, the slices spread over the XCDs — on power-law matrices
     // each XCD's L2 then holds half (or less) of every hot row (measured: 18–25% faster
     // on the 1M R-MAT at N=128/256 fp32, 8% at N=256 bf16, within 6% elsewhere)
-    const int esz = (int)sizeof(TB);
-    const bool auto_slice = merge && !wide && !narrow && c.group == 0 && chunk <= 2048 && N * esz >= 512 &&
-                            N % (8 / esz) == 0 &&
-                            ldb % (8 / esz) == 0 && ldc % (8 / esz) == 0 && aligned(B, 8) && aligned(C, 4 * (8 / esz));
-    const int SV = auto_slice ? 8 / esz : (c.vec > 0 ? c.vec : 4 / esz);
+    // bf16 rows of 128-511 bytes: half-wave slices with 4-byte lanes (1M R-MAT: N = 128
+    // 637 -> 574 us, N = 64 336 -> 315 us)
+    const int alb = N * esz >= 512 ? 8 : 4;  // AUTO's lane bytes
+    const int asv = alb / esz;
+    const bool auto_slice = merge && !wide && !narrow && c.group == 0 && chunk <= 2048 && slice_rows &&
+                            N % asv == 0 && ldb % asv == 0 && ldc % asv == 0 && aligned(B, alb) &&
+                            aligned(C, 4 * asv);
+    const int SV = auto_slice ? asv : (c.vec > 0 ? c.vec : 4 / esz);
     const int slb = SV * esz;
     const bool slice = auto_slice ||
                        (merge && !wide && c.group == 32 && chunk <= 2048 && (slb == 4 || slb == 8 || slb == 16) &&
