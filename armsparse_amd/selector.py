@@ -454,7 +454,7 @@ def main(argv=None):
     ap.add_argument("--matrices", type=int, default=2000)
     ap.add_argument("--seed", type=int, default=2024)
     ap.add_argument("--steps", type=int, default=3000)
-    ap.add_argument("--out", default="gpurun_out/selector")
+    ap.add_argument("--out", default="selector_out")
     ap.add_argument("--dataset", default="", help="reuse a saved dataset .npz instead of autotuning")
     a = ap.parse_args(argv)
     os.makedirs(a.out, exist_ok=True)

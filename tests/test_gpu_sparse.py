@@ -278,7 +278,7 @@ def test_transpose_empty_and_valueless():
 
 
 def test_transpose_full_rmat_1m():
-    """BASELINE size: exact equality with the stable host transpose, and twice = identity."""
+    """Full size (1M R-MAT): exact equality with the stable host transpose, and twice = identity."""
     case("transpose full 1M R-MAT")
     a = rmat_1m_cached()
     cp, ri, vt = _check_transpose(a.rowptr, a.colind, a.values, a.shape[1])

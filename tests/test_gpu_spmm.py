@@ -269,7 +269,7 @@ def test_fp8_dense(rmat14, sched):
 
 
 def test_full_size_rmat_1m_n128():
-    """BASELINE size: 1M×1M R-MAT, 2^24 nnz, N=128, every row: MERGE against the
+    """Full size: 1M×1M R-MAT, 2^24 nnz, N=128, every row: MERGE against the
     FP64 oracle and against the CPU-order ROW kernel (bit-exact with the
     restated reference by the tests above)."""
     case("full 1M R-MAT N=128")
