@@ -1,5 +1,6 @@
 // Library runtime: status strings, HIP error capture, stream-ordered
 // workspace, optional event timing of kernel launches.
+#include <atomic>
 #include <mutex>
 #include <string>
 #include <utility>
@@ -186,13 +187,16 @@ void TimedLaunch::stop() {
     rec = nullptr;
 }
 
+// compute units of the current device, looked up once per device
 int device_cus() {
-    static int cus = 0;
-    if (!cus) {
-        int dev = 0;
-        hipGetDevice(&dev);
-        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-        if (cus <= 0) cus = 256;
+    static std::atomic<int> cached[64];
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 256;
+    int cus = cached[dev].load(std::memory_order_relaxed);
+    if (cus <= 0) {
+        if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
+            cus = 256;
+        cached[dev].store(cus, std::memory_order_relaxed);
     }
     return cus;
 }

@@ -22,10 +22,11 @@ from .op import GCNConv
 class Net(torch.nn.Module):
     """conv1 → ReLU → dropout → conv2 → log_softmax (gcn_custom.py:72-90)."""
 
-    def __init__(self, in_feats: int, n_hidden: int, n_classes: int, precision: str = "fp32", dropout: float = 0.5):
+    def __init__(self, in_feats: int, n_hidden: int, n_classes: int, precision: str = "fp32", dropout: float = 0.5,
+                 order: str = "transform_first"):
         super().__init__()
-        self.conv1 = GCNConv(in_feats, n_hidden, cached=True, normalize=True, precision=precision)
-        self.conv2 = GCNConv(n_hidden, n_classes, cached=True, normalize=True, precision=precision)
+        self.conv1 = GCNConv(in_feats, n_hidden, cached=True, normalize=True, precision=precision, order=order)
+        self.conv2 = GCNConv(n_hidden, n_classes, cached=True, normalize=True, precision=precision, order=order)
         self.reg_params = self.conv1.parameters()
         self.non_reg_params = self.conv2.parameters()
         self.p = dropout
@@ -63,7 +64,8 @@ def _adam(model: Net, capturable: bool):
 
 def make_task(precision: str = "fp32", seed: int = data.CFG_SEED + 3, n: int = 19_717, pairs: int = 44_324,
               features: int = 500, hidden: int = 64, classes: int = 3, feat_density: float = 0.01,
-              device: str = "cuda", dropout: float = 0.5, capturable: bool = False) -> GCNTask:
+              device: str = "cuda", dropout: float = 0.5, capturable: bool = False,
+              order: str = "transform_first") -> GCNTask:
     g = data.gcn_graph(n, pairs, seed)
     r = np.random.default_rng(seed)
     x = np.zeros((n, features), np.float32)
@@ -82,19 +84,20 @@ def make_task(precision: str = "fp32", seed: int = data.CFG_SEED + 3, n: int = 1
     if precision == "bf16":
         xt = xt.to(torch.bfloat16)
     graph = (t(g.rowptr), t(g.colind), t(g.colptr), t(g.rowind), t(g.value_csr), t(g.value_csc))
-    model = Net(features, hidden, classes, precision, dropout).to(dev)
+    model = Net(features, hidden, classes, precision, dropout, order).to(dev)
     return GCNTask(model, _adam(model, capturable), xt, graph, t(y), t(train), precision, int(g.colind.size))
 
 
 SCALED = dict(n=2_400_000, pairs=62_000_000, features=100, hidden=256, feat_density=0.1)
 
 
-def retarget(task: GCNTask, precision: str, capturable: bool = False) -> GCNTask:
-    """A fresh model and optimizer of `precision` on the same graph, features and labels."""
+def retarget(task: GCNTask, precision: str, capturable: bool = False, order: str = "transform_first") -> GCNTask:
+    """A fresh model and optimizer of `precision` (and layer `order`, which needs
+    bf16's fused aggregation) on the same graph, features and labels."""
     x = task.x.float() if precision == "fp32" else task.x.to(torch.bfloat16)
     dev = x.device
     model = Net(x.shape[1], task.model.conv1.out_channels, task.model.conv2.out_channels, precision,
-                task.model.p).to(dev)
+                task.model.p, order).to(dev)
     return GCNTask(model, _adam(model, capturable), x, task.graph, task.y, task.train_mask, precision, task.nnz,
                    task.train_idx, task.y_train)
 

@@ -159,13 +159,22 @@ class GCNConv(torch.nn.Module):
     scale by 1/sqrt(in-degree) (counts from rowptr), add bias."""
 
     def __init__(self, in_channels, out_channels, improved=False, cached=False, bias=True, normalize=True,
-                 precision: str = "fp32", fused: Optional[bool] = None, **kwargs):
+                 precision: str = "fp32", fused: Optional[bool] = None, order: str = "transform_first", **kwargs):
         super().__init__()
         if precision not in ("fp32", "bf16"):
             raise ValueError("precision must be 'fp32' or 'bf16'")
+        if order not in ("transform_first", "aggregate_first", "auto"):
+            raise ValueError("order must be 'transform_first', 'aggregate_first' or 'auto'")
         # fused: normalisation, aggregation and bias in one SpMM launch (default for bf16;
         # the fp32 default keeps the reference's separate tensor ops)
         self.fused = (precision == "bf16") if fused is None else bool(fused)
+        # order (fused layers): Â·(x·W) as the reference computes it, or (Â·x)·W, which
+        # gathers in_channels instead of out_channels per edge and, when x needs no
+        # gradient, runs no SpMM in the backward pass; "auto" aggregates first when
+        # in_channels < out_channels
+        if order != "transform_first" and not self.fused:
+            raise ValueError("order other than 'transform_first' needs the fused aggregation")
+        self.order = order
         self.in_channels = in_channels
         self.out_channels = out_channels
         self.improved = improved
@@ -194,7 +203,46 @@ class GCNConv(torch.nn.Module):
     def out_deg_sqrt(indptr):
         return (1 / torch.sqrt((indptr[1:] - indptr[:-1]).float())).unsqueeze(dim=1)
 
+    def aggregates_first(self) -> bool:
+        return self.fused and (self.order == "aggregate_first" or
+                               (self.order == "auto" and self.in_channels < self.out_channels))
+
+    def _norms_and_coefs(self, rowptr, colind, colptr, rowind, edge_weight_csr, edge_weight_csc):
+        """Degree norms (1/sqrt of the counts, cached with ``cached``) and the fused
+        aggregation's per-edge coefficients."""
+        if not self.cached or self.cached_result is None:
+            if self.normalize:
+                self.cached_result = self.in_deg_sqrt(rowptr), self.out_deg_sqrt(colptr)
+            else:
+                self.cached_result = None, None
+        in_deg_norm, out_deg_norm = self.cached_result
+        in_n = in_deg_norm.reshape(-1).contiguous() if self.normalize else None
+        out_n = out_deg_norm.reshape(-1).contiguous() if self.normalize else None
+        coefs = self.cached_coef if self.cached else None
+        if coefs is None:
+            coefs = gcn_edge_coefficients(colind, rowind, edge_weight_csr, edge_weight_csc, in_n, out_n)
+            if self.cached:
+                self.cached_coef = coefs
+        return in_n, out_n, coefs
+
+    def _forward_aggregate_first(self, x, rowptr, colind, colptr, rowind, edge_weight_csr, edge_weight_csc):
+        """(diag(in_norm)·A·diag(out_norm)·x)·W + bias: the SpMM gathers in_channels per
+        edge; x·W's rounding moves after the aggregation (bf16: x and the aggregate are
+        rounded to bfloat16, the sums and products stay fp32)."""
+        in_n, out_n, coefs = self._norms_and_coefs(rowptr, colind, colptr, rowind, edge_weight_csr,
+                                                   edge_weight_csc)
+        h = x
+        if self.precision == "bf16" and h.dtype != torch.bfloat16:
+            h = h.to(torch.bfloat16) if h.requires_grad else kernels.to_bf16(h.contiguous())
+        z = GCNAggregate.apply(h.contiguous(), rowptr, colind, colptr, rowind, coefs[0], coefs[1], in_n, out_n,
+                               None)
+        y = _BF16Linear.apply(z, self.weight, False) if self.precision == "bf16" else torch.matmul(z, self.weight)
+        return y + self.bias if self.bias is not None else y
+
     def forward(self, x, rowptr, colind, colptr, rowind, edge_weight_csr=None, edge_weight_csc=None):
+        if self.aggregates_first():
+            return self._forward_aggregate_first(x, rowptr, colind, colptr, rowind, edge_weight_csr,
+                                                 edge_weight_csc)
         if self.precision == "bf16":
             # bf16 features for the aggregation straight from the GEMM (fused path with
             # more than 8 output channels); fp32 otherwise
@@ -202,6 +250,13 @@ class GCNConv(torch.nn.Module):
             x = _BF16Linear.apply(x.contiguous(), self.weight, out_bf16)
         else:
             x = torch.matmul(x, self.weight)
+        if self.fused:
+            # bf16 features halve the gather bytes; a handful of output channels gather
+            # next to nothing, so those stay fp32 (no conversions either way)
+            in_n, out_n, coefs = self._norms_and_coefs(rowptr, colind, colptr, rowind, edge_weight_csr,
+                                                       edge_weight_csc)
+            return GCNAggregate.apply(x.contiguous(), rowptr, colind, colptr, rowind, coefs[0], coefs[1], in_n,
+                                      out_n, self.bias)
         if not self.cached or self.cached_result is None:
             if self.normalize:
                 in_deg_norm = self.in_deg_sqrt(rowptr)
@@ -211,19 +266,6 @@ class GCNConv(torch.nn.Module):
                 out_deg_norm = torch.ones(colptr.shape[0] - 1, 1, dtype=x.dtype, device=x.device)
             self.cached_result = in_deg_norm, out_deg_norm
         in_deg_norm, out_deg_norm = self.cached_result
-        if self.fused:
-            in_n = in_deg_norm.reshape(-1).contiguous() if self.normalize else None
-            out_n = out_deg_norm.reshape(-1).contiguous() if self.normalize else None
-            coefs = self.cached_coef if self.cached else None
-            if coefs is None:
-                coefs = gcn_edge_coefficients(colind, rowind, edge_weight_csr, edge_weight_csc, in_n, out_n)
-                if self.cached:
-                    self.cached_coef = coefs
-            # bf16 features halve the gather bytes; a handful of output channels gather
-            # next to nothing, so those stay fp32 (no conversions either way)
-            h = x
-            return GCNAggregate.apply(h.contiguous(), rowptr, colind, colptr, rowind, coefs[0], coefs[1], in_n,
-                                      out_n, self.bias)
         if self.normalize:
             x = x * out_deg_norm
         if self.precision == "bf16":

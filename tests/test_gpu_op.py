@@ -150,6 +150,65 @@ def test_fused_gcn_layer_matches_unfused(graph512):
         assert np.max(np.abs(a - b)) <= 1e-5 * (np.max(np.abs(a)) + 1e-12)
 
 
+def _gcn_dense_fp64(g):
+    """Â = diag(1/sqrt(in-degree counts))·A·diag(1/sqrt(out-degree counts)), dense fp64."""
+    n = g.rowptr.size - 1
+    a = np.zeros((n, n))
+    rows = np.repeat(np.arange(n), np.diff(g.rowptr))
+    np.add.at(a, (rows, g.colind), g.value_csr.astype(np.float64))
+    in_n = 1 / np.sqrt(np.diff(g.rowptr).astype(np.float64))
+    out_n = 1 / np.sqrt(np.diff(g.colptr).astype(np.float64))
+    return in_n[:, None] * a * out_n[None, :]
+
+
+@pytest.mark.parametrize("precision,tol", [("fp32", 1e-5), ("bf16", 1e-2)])
+def test_aggregate_first_layer_matches_fp64(graph512, precision, tol):
+    """order="aggregate_first": y = (Â·x)·W + b, dW = (Â·x)ᵀ·G, db = Σ G, dx = Âᵀ·G·Wᵀ
+    against the dense fp64 closed forms; "auto" picks it only when in < out."""
+    g, *_ = graph512
+    gt = dev(g.rowptr, g.colind, g.colptr, g.rowind, g.value_csr, g.value_csc)
+    torch.manual_seed(2)
+    conv = GCNConv(40, 96, fused=True, precision=precision, order="aggregate_first").cuda()
+    with torch.no_grad():
+        conv.bias.uniform_(-0.2, 0.2)
+    assert conv.aggregates_first()
+    assert GCNConv(40, 96, precision="bf16", order="auto").aggregates_first()
+    assert not GCNConv(96, 40, precision="bf16", order="auto").aggregates_first()
+    with pytest.raises(ValueError):
+        GCNConv(40, 96, order="aggregate_first")  # needs the fused aggregation
+    x = torch.rand(512, 40, device="cuda", requires_grad=True)
+    G = torch.randn(512, 96, device="cuda")
+    y = conv(x, *gt[:4], *gt[4:])
+    y.backward(G)
+    ah = _gcn_dense_fp64(g)
+    xd, gd = x.detach().double().cpu().numpy(), G.double().cpu().numpy()
+    wd, bd = conv.weight.detach().double().cpu().numpy(), conv.bias.detach().double().cpu().numpy()
+    z = ah @ xd
+    ref_y = z @ wd + bd
+    assert y.dtype == torch.float32
+    assert np.max(oracle.row_rel_err(y.detach().cpu().numpy(), ref_y)) <= tol
+
+    def rel(a, b):
+        return np.linalg.norm(a - b) / np.linalg.norm(b)
+    assert rel(conv.weight.grad.cpu().numpy(), z.T @ gd) <= tol
+    assert rel(conv.bias.grad.cpu().numpy(), gd.sum(0)) <= tol
+    assert rel(x.grad.cpu().numpy(), ah.T @ gd @ wd.T) <= tol
+
+
+def test_aggregate_first_training_matches_transform_first():
+    """A bf16 model whose first layer aggregates first trains like the reference
+    order: same initial weights, five steps, losses within bf16 rounding."""
+    torch.manual_seed(0)
+    a = gcn.make_task("bf16", n=4000, pairs=9000, features=32, hidden=96, dropout=0.0)
+    b = gcn.retarget(a, "bf16", order="auto")
+    assert b.model.conv1.aggregates_first() and not b.model.conv2.aggregates_first()
+    b.model.load_state_dict(a.model.state_dict())
+    la = [gcn.train_step(a) for _ in range(5)]
+    lb = [gcn.train_step(b) for _ in range(5)]
+    assert lb[-1] < lb[0]
+    assert max(abs(p - q) for p, q in zip(la, lb)) <= 1e-2 * max(abs(q) for q in la)
+
+
 def test_graphed_gcn_step_matches_eager():
     """The bf16 training step captured in a HIP graph: five replays, each
     followed by an eager step of an identically initialised model and a
