@@ -387,7 +387,8 @@ __global__ void k_splitk_reduce(int M, int N, int splits, const float* __restric
     if (idx >= (int64_t)M * N) return;
     const int m = (int)(idx / N), n = (int)(idx % N);
     float s = 0.0f;
-    for (int z = 0; z < splits; ++z) s += part[(int64_t)z * M * N + idx];
+#pragma unroll 8
+    for (int z = 0; z < splits; ++z) s += part[(int64_t)z * M * N + idx];  // loads run ahead of the in-order adds
     float* p = C + (int64_t)m * ldc + n;
     *p = accumulate ? *p + s : s;
 }

@@ -134,13 +134,17 @@ __global__ void k_colsum(int M, int N, const float* __restrict__ X, float* __res
     const int n = blockIdx.x * blockDim.x + threadIdx.x;
     if (n >= N) return;
     float s = 0.0f;
-    for (int m = 0; m < M; ++m) s += X[(int64_t)m * N + n];
+#pragma unroll 8
+    for (int m = 0; m < M; ++m) s += X[(int64_t)m * N + n];  // loads run ahead of the in-order adds
     out[n] = s;
 }
 
 // Tall matrices: block b sums rows [b·R, (b+1)·R) — 64 columns by 4 row
 // phases, the phases joined in LDS in a fixed order — into part[b][·];
-// k_colsum_parts then adds the blocks' sums in block order.  Deterministic.
+// k_colsum_parts then adds the blocks' sums the same way (block phases, fixed
+// join).  Deterministic.  R is small (128 rows) so that a short matrix still
+// spreads over many blocks, and every loop keeps 8 loads in flight ahead of
+// its in-order adds.
 __global__ __launch_bounds__(256) void k_colsum_blocks(int M, int N, int R, const float* __restrict__ X,
                                                        float* __restrict__ part) {
     __shared__ float s[4][64];
@@ -149,19 +153,28 @@ __global__ __launch_bounds__(256) void k_colsum_blocks(int M, int N, int R, cons
     const int64_t r0 = (int64_t)blockIdx.x * R;
     const int64_t r1 = r0 + R < M ? r0 + R : M;
     float acc = 0.0f;
-    if (n < N)
+    if (n < N) {
+#pragma unroll 8
         for (int64_t m = r0 + ty; m < r1; m += 4) acc += X[m * N + n];
+    }
     s[ty][tx] = acc;
     __syncthreads();
     if (ty == 0 && n < N) part[(int64_t)blockIdx.x * N + n] = ((s[0][tx] + s[1][tx]) + s[2][tx]) + s[3][tx];
 }
 
-__global__ void k_colsum_parts(int nb, int N, const float* __restrict__ part, float* __restrict__ out) {
-    const int n = blockIdx.x * blockDim.x + threadIdx.x;
-    if (n >= N) return;
-    float s = 0.0f;
-    for (int b = 0; b < nb; ++b) s += part[(int64_t)b * N + n];
-    out[n] = s;
+__global__ __launch_bounds__(256) void k_colsum_parts(int nb, int N, const float* __restrict__ part,
+                                                      float* __restrict__ out) {
+    __shared__ float s[4][64];
+    const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+    const int n = blockIdx.x * 64 + tx;
+    float acc = 0.0f;
+    if (n < N) {
+#pragma unroll 8
+        for (int b = ty; b < nb; b += 4) acc += part[(int64_t)b * N + n];
+    }
+    s[ty][tx] = acc;
+    __syncthreads();
+    if (ty == 0 && n < N) out[n] = ((s[0][tx] + s[1][tx]) + s[2][tx]) + s[3][tx];
 }
 
 __global__ void k_adam(int64_t n, float* __restrict__ p, const float* __restrict__ g, float* __restrict__ m,
@@ -279,8 +292,8 @@ int armsp_colsum_f32(int M, int N, const float* X, float* out, void* stream) {
         ARMSP_LAUNCHED();
         return ARMSP_SUCCESS;
     }
-    const int64_t nb64 = ((int64_t)M + 1023) / 1024;
-    const int nb = (int)(nb64 < 1024 ? nb64 : 1024);
+    const int64_t nb64 = ((int64_t)M + 127) / 128;
+    const int nb = (int)(nb64 < 4096 ? nb64 : 4096);
     const int R = (int)(((int64_t)M + nb - 1) / nb);
     void* ws = nullptr;
     int st = ws_alloc(&ws, (size_t)nb * N * sizeof(float), s);
@@ -288,7 +301,7 @@ int armsp_colsum_f32(int M, int N, const float* X, float* out, void* stream) {
     float* part = static_cast<float*>(ws);
     hipLaunchKernelGGL(k_colsum_blocks, dim3((unsigned)nb, (unsigned)((N + 63) / 64)), dim3(256), 0, s, M, N, R, X,
                        part);
-    hipLaunchKernelGGL(k_colsum_parts, dim3(blocks_for(N)), dim3(256), 0, s, nb, N, part, out);
+    hipLaunchKernelGGL(k_colsum_parts, dim3((unsigned)((N + 63) / 64)), dim3(256), 0, s, nb, N, part, out);
     const hipError_t e = hipGetLastError();
     ws_free(ws, s);
     return e == hipSuccess ? ARMSP_SUCCESS : hip_fail(e);

@@ -1250,6 +1250,12 @@ int spmm_impl(int M, int K, int N, int nnz, const int* rowptr, const int* colind
         chunk = chunk <= 256 ? 256 : (chunk <= 512 ? 512 : 1024);
     }
     const int64_t total = merge ? (int64_t)M + nnz : (int64_t)nnz;
+    if (merge && !narrow && c.chunk <= 0) {
+        // small inputs: shorter chunks until 16 waves per CU are in flight (PubMed-size
+        // graphs otherwise leave half the CUs idle while each wave walks its chunk)
+        const int64_t want = 16 * (int64_t)device_cus();
+        while (chunk > 64 && (total + chunk - 1) / chunk < want) chunk >>= 1;
+    }
     const int64_t nwaves = (total + chunk - 1) / chunk;
     if (nwaves > INT_MAX / 4) return ARMSP_ERROR_NOT_SUPPORTED;
     const unsigned tiles = (unsigned)((N + 64 * V - 1) / (64 * V));
