@@ -313,11 +313,27 @@ int armsp_softmax_xent(int rows, int classes, const float* logits, const int* la
 /* grad[i] = 0 where act[i] <= 0. */
 int armsp_relu_backward(int64_t n, const float* act, float* grad, void* stream);
 /* out[n] = sum over m of X[m][n]: rows summed in order up to 4096 rows; taller
- * matrices in at most 1024 row blocks (each in four interleaved row phases)
- * whose sums are added in block order.  Deterministic; also the bias gradient
+ * matrices in blocks of 128 rows or more (at most 4096 blocks, each summed in
+ * four interleaved row phases) whose sums are added the same way in a fixed
+ * order.  Deterministic; also the bias gradient
  * of the GCN layer (a torch reduction there could record memset nodes into a
  * captured graph). */
 int armsp_colsum_f32(int M, int N, const float* X, float* out, void* stream);
+/* GCN hidden-layer epilogue in one pass (replaces the bias add of GCNConv,
+ * op.py:122-124, and the F.relu / F.dropout of Net.forward, gcn_custom.py:84-86):
+ * H[m,n] = bf16(keep·max(Y[m,n] + bias[n], 0)/(1-p)) with H contiguous [M×N]
+ * bf16, and mask[m·N/8 + n/8] bit n%8 = (Y + bias > 0 and kept).  keep draws a
+ * 24-bit uniform per element from a counter-based hash of (*seed, m·N + n);
+ * with p > 0 the call then advances *seed (device memory) by one on the same
+ * stream, so a captured graph draws a new mask per replay.  N % 8 == 0,
+ * ldy % 4 == 0, Y, H and bias 16-byte aligned; bias may be NULL; 0 <= p < 1. */
+int armsp_bias_relu_dropout_bf16(int M, int N, const float* Y, int64_t ldy, const float* bias, float p,
+                                 uint64_t* seed, uint16_t* H, uint8_t* mask, void* stream);
+/* Its backward: GY = bf16(mask ? G/(1-p) : 0) (G, GY contiguous [M×N] bf16)
+ * and, when dbias is not NULL, dbias[n] = Σ_m of the same fp32 values, summed
+ * in a fixed order (deterministic).  N % 8 == 0, N <= 2048. */
+int armsp_bias_relu_dropout_backward_bf16(int M, int N, const uint16_t* G, const uint8_t* mask, float p,
+                                          uint16_t* GY, float* dbias, void* stream);
 /* One Adam step (L2 weight decay added to the gradient); step counts from 1. */
 int armsp_adam_step(int64_t n, float* param, const float* grad, float* m, float* v, float lr, float beta1,
                     float beta2, float eps, float weight_decay, int step, void* stream);
