@@ -118,6 +118,8 @@ _SIGS = {
     "armsp_softmax_xent": (c_int, [_I, _I, _P, _P, _P, _P, _P, _P]),
     "armsp_relu_backward": (c_int, [c_int64, _P, _P, _P]),
     "armsp_colsum_f32": (c_int, [_I, _I, _P, _P, _P]),
+    "armsp_bias_relu_dropout_bf16": (c_int, [_I, _I, _P, c_int64, _P, c_float, _P, _P, _P, _P]),
+    "armsp_bias_relu_dropout_backward_bf16": (c_int, [_I, _I, _P, _P, c_float, _P, _P, _P]),
     "armsp_adam_step": (c_int, [c_int64, _P, _P, _P, _P, c_float, c_float, c_float, c_float, c_float, _I, _P]),
     "armsp_mlp_infer": (c_int, [_I, _I, _I, _I, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P]),
     # host-side generators and format helpers (armsp_gen.h)

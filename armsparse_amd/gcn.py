@@ -32,8 +32,11 @@ class Net(torch.nn.Module):
         self.p = dropout
 
     def forward(self, x, g):
-        x = F.relu(self.conv1(x, *g))
-        x = F.dropout(x, p=self.p, training=self.training)
+        if self.conv1.fuses_activation():  # bias, ReLU, dropout and the bf16 cast in one pass
+            x = self.conv1.forward_act(x, *g, p=self.p if self.training else 0.0)
+        else:
+            x = F.relu(self.conv1(x, *g))
+            x = F.dropout(x, p=self.p, training=self.training)
         x = self.conv2(x, *g)
         return F.log_softmax(x, dim=1)
 

@@ -383,6 +383,47 @@ def gemm_bf16(a: torch.Tensor, b: torch.Tensor, *, out: Optional[torch.Tensor] =
     return out
 
 
+def bias_relu_dropout(y: torch.Tensor, bias: Optional[torch.Tensor], p: float,
+                      seed: Optional[torch.Tensor]) -> tuple:
+    """(h, mask): h = bf16(dropout_p(relu(y + bias))) for a 2-D float32 device
+    tensor y [M×N] (N % 8 == 0, unit column stride), mask one bit per element
+    (uint8 [M, N/8]: ReLU passed and kept).  ``seed`` is a one-element int64
+    device tensor that the call advances when p > 0 (armsp_bias_relu_dropout_bf16)."""
+    fn = "bias_relu_dropout"
+    _require(y.is_cuda and y.dtype == torch.float32 and y.dim() == 2 and y.stride(1) == 1, f"{fn}: needs a 2-D "
+             "float32 device tensor with unit column stride")
+    m, n = y.shape
+    _require(n % 8 == 0, f"{fn}: the column count must be a multiple of 8")
+    _require(0.0 <= p < 1.0, f"{fn}: p must be in [0, 1)")
+    if bias is not None:
+        _require(bias.is_cuda and bias.dtype == torch.float32 and bias.numel() == n, f"{fn}: bad bias")
+        bias = bias.contiguous()
+    if p > 0.0:
+        _require(seed is not None and seed.is_cuda and seed.dtype == torch.int64 and seed.numel() == 1,
+                 f"{fn}: dropout needs a one-element int64 device seed tensor")
+    h = torch.empty((m, n), dtype=torch.bfloat16, device=y.device)
+    mask = torch.empty((m, n // 8), dtype=torch.uint8, device=y.device)
+    call("armsp_bias_relu_dropout_bf16", m, n, _ptr(y), y.stride(0), _ptr(bias), float(p),
+         _ptr(seed) if p > 0.0 else None, _ptr(h), _ptr(mask), _stream(y.device))
+    return h, mask
+
+
+def bias_relu_dropout_backward(g: torch.Tensor, mask: torch.Tensor, p: float, want_dbias: bool) -> tuple:
+    """(gy, dbias) of bias_relu_dropout for the bf16 gradient g of h: gy =
+    bf16(mask ? g/(1-p) : 0), dbias = the fp32 column sums of the same values
+    (deterministic), or None."""
+    fn = "bias_relu_dropout_backward"
+    g = g.contiguous()
+    _require(g.is_cuda and g.dtype == torch.bfloat16 and g.dim() == 2, f"{fn}: needs a 2-D bfloat16 device tensor")
+    m, n = g.shape
+    _require(mask.dtype == torch.uint8 and mask.shape == (m, n // 8) and mask.is_contiguous(), f"{fn}: bad mask")
+    gy = torch.empty_like(g)
+    dbias = torch.empty(n, device=g.device) if want_dbias else None
+    call("armsp_bias_relu_dropout_backward_bf16", m, n, _ptr(g), _ptr(mask), float(p), _ptr(gy), _ptr(dbias),
+         _stream(g.device))
+    return gy, dbias
+
+
 def colsum(x: torch.Tensor) -> torch.Tensor:
     """x.sum(0) for a 2-D float32 device tensor, deterministic (armsp_colsum_f32)."""
     _require(x.is_cuda and x.dtype == torch.float32 and x.dim() == 2, "colsum: needs a 2-D float32 device tensor")

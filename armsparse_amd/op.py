@@ -153,6 +153,37 @@ class _BF16Linear(torch.autograd.Function):
         return gx, gw, None
 
 
+class _LinearBiasReluDropout(torch.autograd.Function):
+    """h = bf16(dropout_p(relu(z·W + b))) — x·W on the MFMA GEMM (fp32 out), then
+    the bias, ReLU, dropout and bf16 cast in one pass (armsp_bias_relu_dropout_bf16);
+    backward: one pass gives the bf16 gradient of z·W + b and the bias gradient,
+    then dW = zᵀ·gy (and dz = gy·Wᵀ when z needs it) on the GEMM."""
+
+    @staticmethod
+    def forward(ctx, z, weight, bias, p, seed):
+        zb = z if z.dtype == torch.bfloat16 else kernels.to_bf16(z.contiguous())
+        wb = kernels.to_bf16(weight.detach())
+        y = kernels.gemm_bf16(zb, wb)
+        h, mask = kernels.bias_relu_dropout(y, None if bias is None else bias.detach(), p, seed)
+        ctx.save_for_backward(zb, wb, mask)
+        ctx.p = p
+        ctx.z_dtype = z.dtype
+        ctx.has_bias = bias is not None
+        return h
+
+    @staticmethod
+    def backward(ctx, gh):
+        zb, wb, mask = ctx.saved_tensors
+        gyb, dbias = kernels.bias_relu_dropout_backward(gh.to(torch.bfloat16), mask, ctx.p,
+                                                        ctx.has_bias and ctx.needs_input_grad[2])
+        gz = gw = None
+        if ctx.needs_input_grad[0]:
+            gz = kernels.gemm_bf16(gyb, wb.t(), out_dtype=ctx.z_dtype)
+        if ctx.needs_input_grad[1]:
+            gw = kernels.gemm_bf16(zb.t(), gyb)
+        return gz, gw, dbias, None, None
+
+
 class GCNConv(torch.nn.Module):
     """GCN layer of gcn/pytorch-custom/op.py:78-153: x·W, scale by
     1/sqrt(out-degree) (counts from colptr), aggregate with SPMMFunction,
@@ -202,6 +233,30 @@ class GCNConv(torch.nn.Module):
     @staticmethod
     def out_deg_sqrt(indptr):
         return (1 / torch.sqrt((indptr[1:] - indptr[:-1]).float())).unsqueeze(dim=1)
+
+    def fuses_activation(self) -> bool:
+        """forward_act available: bf16, aggregating first, a multiple of 8 outputs."""
+        return self.precision == "bf16" and self.aggregates_first() and self.out_channels % 8 == 0
+
+    def forward_act(self, x, rowptr, colind, colptr, rowind, edge_weight_csr=None, edge_weight_csc=None,
+                    p: float = 0.0):
+        """dropout_p(relu(self(x, ...))) as a bfloat16 tensor, with the bias, ReLU,
+        dropout and cast fused after x·W (a hidden layer of Net; fuses_activation())."""
+        if not self.fuses_activation():
+            raise RuntimeError("forward_act needs a bf16 layer that aggregates first with a multiple of 8 outputs")
+        in_n, out_n, coefs = self._norms_and_coefs(rowptr, colind, colptr, rowind, edge_weight_csr,
+                                                   edge_weight_csc)
+        h = x
+        if h.dtype != torch.bfloat16:
+            h = h.to(torch.bfloat16) if h.requires_grad else kernels.to_bf16(h.contiguous())
+        z = GCNAggregate.apply(h.contiguous(), rowptr, colind, colptr, rowind, coefs[0], coefs[1], in_n, out_n,
+                               None)
+        seed = None
+        if p > 0.0:
+            if getattr(self, "_seed", None) is None or self._seed.device != z.device:
+                self._seed = torch.randint(0, 2**62, (1,), dtype=torch.int64).to(z.device)
+            seed = self._seed
+        return _LinearBiasReluDropout.apply(z, self.weight, self.bias, float(p), seed)
 
     def aggregates_first(self) -> bool:
         return self.fused and (self.order == "aggregate_first" or

@@ -209,6 +209,121 @@ def test_aggregate_first_training_matches_transform_first():
     assert max(abs(p - q) for p, q in zip(la, lb)) <= 1e-2 * max(abs(q) for q in la)
 
 
+def _bits(mask, n):
+    """uint8 [M, n/8] mask → bool [M, n] (bit j of byte k is column 8k + j)."""
+    m = mask.cpu().numpy()
+    return np.unpackbits(m, axis=1, bitorder="little")[:, :n].astype(bool)
+
+
+@pytest.mark.parametrize("n", [256, 24])
+def test_bias_relu_dropout_kernels(n):
+    """h = bf16(keep·relu(y + b)/(1-p)) bit for bit against torch on the kernel's
+    own mask; the mask is ReLU's mask with about p of it dropped; the seed
+    advances; the backward gives bf16(mask·g/(1-p)) bit for bit and the bias
+    gradient within 1e-5 of FP64, identically on a second call."""
+    print(f"[case] bias_relu_dropout N={n}", flush=True)
+    torch.manual_seed(3)
+    m = 3001
+    y = torch.randn(m, n, device="cuda")
+    b = torch.randn(n, device="cuda") * 0.1
+    v = y + b
+    # p = 0: plain bias + ReLU
+    h0, mask0 = kernels.bias_relu_dropout(y, b, 0.0, None)
+    assert torch.equal(torch.from_numpy(_bits(mask0, n)), (v > 0).cpu())
+    assert torch.equal(h0, torch.relu(v).to(torch.bfloat16))
+    p, scale = 0.5, 2.0
+    seed = torch.tensor([12345], dtype=torch.int64, device="cuda")
+    h, mask = kernels.bias_relu_dropout(y, b, p, seed)
+    keep = torch.from_numpy(_bits(mask, n)).cuda()
+    assert not (keep & (v <= 0)).any()
+    frac = keep.sum().item() / (v > 0).sum().item()
+    assert abs(frac - (1 - p)) < 0.02, frac
+    assert torch.equal(h, torch.where(keep, v * scale, torch.zeros_like(v)).to(torch.bfloat16))
+    assert seed.item() == 12346
+    h2, mask2 = kernels.bias_relu_dropout(y, b, p, seed)
+    assert not torch.equal(mask2, mask) and seed.item() == 12347
+    # backward
+    g = torch.randn(m, n, device="cuda").to(torch.bfloat16)
+    gy, db = kernels.bias_relu_dropout_backward(g, mask, p, True)
+    ref = torch.where(keep, g.float() * scale, torch.zeros_like(v))
+    assert torch.equal(gy, ref.to(torch.bfloat16))
+    db64 = ref.double().sum(0)
+    assert ((db.double() - db64).abs().max() / db64.abs().max()).item() <= 1e-5
+    gy2, db2 = kernels.bias_relu_dropout_backward(g, mask, p, True)
+    assert torch.equal(gy2, gy) and torch.equal(db2, db)
+    # no rows: a zero bias gradient; N not a multiple of 8 is refused
+    _, dbz = kernels.bias_relu_dropout_backward(g[:0], mask[:0], p, True)
+    assert torch.equal(dbz, torch.zeros(n, device="cuda"))
+    with pytest.raises(RuntimeError):
+        kernels.bias_relu_dropout(torch.randn(4, 12, device="cuda"), None, 0.0, None)
+
+
+def test_fused_activation_layer_matches_fp64(graph512):
+    """GCNConv.forward_act with p = 0 equals relu(Â·x·W + b) within bf16 rounding
+    (1e-2 of FP64), and its weight and bias gradients match the FP64 closed
+    forms through the forward's ReLU mask."""
+    g, *_ = graph512
+    gt = dev(g.rowptr, g.colind, g.colptr, g.rowind, g.value_csr, g.value_csc)
+    torch.manual_seed(4)
+    conv = GCNConv(40, 96, precision="bf16", order="auto").cuda()
+    with torch.no_grad():
+        conv.bias.uniform_(-0.2, 0.2)
+    assert conv.fuses_activation()
+    x = torch.rand(512, 40, device="cuda")
+    h = conv.forward_act(x, *gt[:4], *gt[4:], p=0.0)
+    assert h.dtype == torch.bfloat16
+    G = torch.randn(512, 96, device="cuda").to(torch.bfloat16)
+    h.backward(G)
+    ah = _gcn_dense_fp64(g)
+    z = ah @ x.double().cpu().numpy()
+    wd, bd = conv.weight.detach().double().cpu().numpy(), conv.bias.detach().double().cpu().numpy()
+    pre = z @ wd + bd
+    ref = np.maximum(pre, 0)
+    assert np.max(oracle.row_rel_err(h.detach().float().cpu().numpy(), ref)) <= 1e-2
+    # the backward follows the forward's own ReLU mask (h > 0): where the fp64
+    # pre-activation is within bf16 rounding of 0 the two may disagree
+    live = h.detach().float().cpu().numpy() > 0
+    assert np.mean(live != (pre > 0)) < 0.01
+    gd = G.double().cpu().numpy() * live
+
+    def rel(a, b):
+        return np.linalg.norm(a - b) / np.linalg.norm(b)
+    assert rel(conv.weight.grad.cpu().numpy(), z.T @ gd) <= 1e-2
+    assert rel(conv.bias.grad.cpu().numpy(), gd.sum(0)) <= 1e-2
+
+
+def test_graphed_gcn_step_with_fused_activation():
+    """The bf16 step with a first layer that aggregates first and fuses its
+    epilogue, captured in a HIP graph: with dropout 0 every replay equals an
+    eager step of an identical model; with dropout 0.5 replays draw fresh masks
+    (the losses differ) and stay finite."""
+    print("[case] graphed GCN step, fused activation", flush=True)
+    torch.manual_seed(0)
+    a = gcn.make_task("bf16", n=4000, pairs=9000, features=32, hidden=96, dropout=0.0, capturable=True)
+    a = gcn.retarget(a, "bf16", capturable=True, order="auto")
+    assert a.model.conv1.fuses_activation()
+    b = gcn.retarget(a, "bf16", capturable=True, order="auto")
+    b.model.load_state_dict(a.model.state_dict())
+    step = gcn.GraphedStep(a, warmup=3)
+    for _ in range(3):
+        gcn.train_step(b)
+    for i in range(4):
+        step.replay()
+        eager_loss = gcn.train_step(b)
+        torch.cuda.synchronize()
+        assert abs(step.loss() - eager_loss) <= 1e-6 * max(1.0, abs(eager_loss)), i
+    c = gcn.retarget(a, "bf16", capturable=True, order="auto")
+    c.model.p = 0.5
+    step_c = gcn.GraphedStep(c, warmup=2)
+    s0 = c.model.conv1._seed.item()
+    losses = []
+    for _ in range(4):
+        step_c.replay()
+        losses.append(step_c.loss())
+    assert c.model.conv1._seed.item() == s0 + 4  # one fresh mask per replay
+    assert all(np.isfinite(losses)) and len(set(losses)) == 4, losses
+
+
 def test_graphed_gcn_step_matches_eager():
     """The bf16 training step captured in a HIP graph: five replays, each
     followed by an eager step of an identically initialised model and a
