@@ -797,17 +797,44 @@ __global__ __launch_bounds__(256) void k_spmm_narrow(int M, int N, int nnz, cons
             }
         }
         float bv[IPT][NV];
+        if (sizeof(TB) == 4 && NV >= 2 && N >= 2 && N >= NV - 1) {
+            // fp32 rows of 2, 3 or 4 columns: one dword-aligned 8-, 12- or 16-byte load per
+            // nonzero instead of one load per column
 #pragma unroll
-        for (int q = 0; q < IPT; ++q)
+            for (int q = 0; q < IPT; ++q) {
+                const uint32_t off = (uint32_t)cs[q] * rowbytes;
 #pragma unroll
-            for (int n = 0; n < NV; ++n) {
-                bv[q][n] = 0.0f;
-                if (n < N) {
-                    float b[1];
-                    gather_b<TB, 1>(rs, (uint32_t)cs[q] * rowbytes + (uint32_t)(n * (int)sizeof(TB)), b);
-                    bv[q][n] = b[0];
+                for (int n = 0; n < NV; ++n) bv[q][n] = 0.0f;
+                if (NV == 2) {
+                    const auto t = __builtin_amdgcn_raw_buffer_load_b64(rs, off, 0, 0);
+                    bv[q][0] = __uint_as_float(t[0]);
+                    bv[q][1 % NV] = __uint_as_float(t[1]);
+                } else if (N == 3) {
+                    const auto t = __builtin_amdgcn_raw_buffer_load_b96(rs, off, 0, 0);
+                    bv[q][0] = __uint_as_float(t[0]);
+                    bv[q][1 % NV] = __uint_as_float(t[1]);
+                    bv[q][2 % NV] = __uint_as_float(t[2]);
+                } else {
+                    const auto t = __builtin_amdgcn_raw_buffer_load_b128(rs, off, 0, 0);
+                    bv[q][0] = __uint_as_float(t[0]);
+                    bv[q][1 % NV] = __uint_as_float(t[1]);
+                    bv[q][2 % NV] = __uint_as_float(t[2]);
+                    bv[q][3 % NV] = __uint_as_float(t[3]);
                 }
             }
+        } else {
+#pragma unroll
+            for (int q = 0; q < IPT; ++q)
+#pragma unroll
+                for (int n = 0; n < NV; ++n) {
+                    bv[q][n] = 0.0f;
+                    if (n < N) {
+                        float b[1];
+                        gather_b<TB, 1>(rs, (uint32_t)cs[q] * rowbytes + (uint32_t)(n * (int)sizeof(TB)), b);
+                        bv[q][n] = b[0];
+                    }
+                }
+        }
 #pragma unroll
         for (int q = 0; q < IPT; ++q) {
             const int k = lane + 64 * q;
