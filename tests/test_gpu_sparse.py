@@ -194,7 +194,8 @@ def test_gemm_bf16_mfma_layout(m, n, k):
 @pytest.mark.parametrize("m,n,k,layout", [
     (20000, 256, 100, "nn"), (20011, 250, 99, "nn"), (20000, 3, 256, "nn"), (20000, 64, 37, "nn"),
     (20000, 256, 3, "nt"), (12000, 100, 128, "tn"),              # tall: B resident, A streamed once
-    (100, 256, 70000, "tn"), (256, 3, 70001, "tn"), (37, 70, 66000, "nn")])  # reduction over a long K
+    (100, 256, 70000, "tn"), (256, 3, 70001, "tn"), (37, 70, 66000, "nn"),   # reduction over a long K
+    (256, 256, 66000, "tn")])                                                 # ... with the largest slabs
 def test_gemm_bf16_tall_and_reduction(m, n, k, layout):
     """The GCN layer's shapes: x·W (tall), the gradient back to x (tall, Wᵀ view) and
     xᵀ·dY (reduction, xᵀ view); against an FP64 GEMM of the same bf16 inputs."""
