@@ -184,6 +184,35 @@ class _LinearBiasReluDropout(torch.autograd.Function):
         return gz, gw, dbias, None, None
 
 
+class _AggregateBiasReluDropout(torch.autograd.Function):
+    """h = bf16(dropout_p(relu(diag(in_norm)·A·diag(out_norm)·x + bias))) for bf16
+    x: the fused SpMM (GCNAggregate's forward without its bias), then bias, ReLU,
+    dropout and the cast in one pass; backward: one pass gives the bf16 gradient
+    of the pre-activation and the bias gradient, which feed the Aᵀ SpMM directly
+    (no fp32 round trip of the gradient)."""
+
+    @staticmethod
+    def forward(ctx, h, rowptr, colind, colptr, rowind, coef_fwd, coef_bwd, in_norm, out_norm, bias, p, seed):
+        y = kernels.spmm(rowptr, colind, coef_fwd, h, row_scale=in_norm)
+        out, mask = kernels.bias_relu_dropout(y, None if bias is None else bias.detach(), p, seed)
+        ctx.save_for_backward(mask)
+        ctx.graph = (colptr, rowind, coef_bwd, out_norm, h.dtype)
+        ctx.p = p
+        ctx.has_bias = bias is not None
+        return out
+
+    @staticmethod
+    def backward(ctx, gout):
+        (mask,) = ctx.saved_tensors
+        colptr, rowind, coef_bwd, out_norm, hdt = ctx.graph
+        gyb, dbias = kernels.bias_relu_dropout_backward(gout.to(torch.bfloat16), mask, ctx.p,
+                                                        ctx.has_bias and ctx.needs_input_grad[9])
+        dh = None
+        if ctx.needs_input_grad[0]:
+            dh = kernels.spmm(colptr, rowind, coef_bwd, gyb, row_scale=out_norm).to(hdt)
+        return dh, None, None, None, None, None, None, None, None, dbias, None, None
+
+
 class GCNConv(torch.nn.Module):
     """GCN layer of gcn/pytorch-custom/op.py:78-153: x·W, scale by
     1/sqrt(out-degree) (counts from colptr), aggregate with SPMMFunction,
@@ -235,27 +264,32 @@ class GCNConv(torch.nn.Module):
         return (1 / torch.sqrt((indptr[1:] - indptr[:-1]).float())).unsqueeze(dim=1)
 
     def fuses_activation(self) -> bool:
-        """forward_act available: bf16, aggregating first, a multiple of 8 outputs."""
-        return self.precision == "bf16" and self.aggregates_first() and self.out_channels % 8 == 0
+        """forward_act available: a fused bf16 layer with a multiple of 8 outputs."""
+        return self.precision == "bf16" and self.fused and self.out_channels % 8 == 0
 
     def forward_act(self, x, rowptr, colind, colptr, rowind, edge_weight_csr=None, edge_weight_csc=None,
                     p: float = 0.0):
         """dropout_p(relu(self(x, ...))) as a bfloat16 tensor, with the bias, ReLU,
-        dropout and cast fused after x·W (a hidden layer of Net; fuses_activation())."""
+        dropout and cast fused into one pass after the layer's last product (a
+        hidden layer of Net; fuses_activation())."""
         if not self.fuses_activation():
-            raise RuntimeError("forward_act needs a bf16 layer that aggregates first with a multiple of 8 outputs")
+            raise RuntimeError("forward_act needs a fused bf16 layer with a multiple of 8 outputs")
         in_n, out_n, coefs = self._norms_and_coefs(rowptr, colind, colptr, rowind, edge_weight_csr,
                                                    edge_weight_csc)
+        seed = None
+        if p > 0.0:
+            if getattr(self, "_seed", None) is None or self._seed.device != x.device:
+                self._seed = torch.randint(0, 2**62, (1,), dtype=torch.int64).to(x.device)
+            seed = self._seed
+        if not self.aggregates_first():
+            h = _BF16Linear.apply(x.contiguous(), self.weight, True)  # bf16 features for the gathers
+            return _AggregateBiasReluDropout.apply(h, rowptr, colind, colptr, rowind, coefs[0], coefs[1], in_n,
+                                                   out_n, self.bias, float(p), seed)
         h = x
         if h.dtype != torch.bfloat16:
             h = h.to(torch.bfloat16) if h.requires_grad else kernels.to_bf16(h.contiguous())
         z = GCNAggregate.apply(h.contiguous(), rowptr, colind, colptr, rowind, coefs[0], coefs[1], in_n, out_n,
                                None)
-        seed = None
-        if p > 0.0:
-            if getattr(self, "_seed", None) is None or self._seed.device != z.device:
-                self._seed = torch.randint(0, 2**62, (1,), dtype=torch.int64).to(z.device)
-            seed = self._seed
         return _LinearBiasReluDropout.apply(z, self.weight, self.bias, float(p), seed)
 
     def aggregates_first(self) -> bool:

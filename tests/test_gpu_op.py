@@ -258,21 +258,23 @@ def test_bias_relu_dropout_kernels(n):
         kernels.bias_relu_dropout(torch.randn(4, 12, device="cuda"), None, 0.0, None)
 
 
-def test_fused_activation_layer_matches_fp64(graph512):
-    """GCNConv.forward_act with p = 0 equals relu(Â·x·W + b) within bf16 rounding
-    (1e-2 of FP64), and its weight and bias gradients match the FP64 closed
-    forms through the forward's ReLU mask."""
+@pytest.mark.parametrize("fin,fout", [(40, 96), (96, 40)])
+def test_fused_activation_layer_matches_fp64(graph512, fin, fout):
+    """GCNConv.forward_act with p = 0 — aggregating first (40 → 96) or after x·W
+    (96 → 40) — equals relu(Â·x·W + b) within bf16 rounding (1e-2 of FP64), and
+    its weight and bias gradients match the FP64 closed forms through the
+    forward's ReLU mask."""
     g, *_ = graph512
     gt = dev(g.rowptr, g.colind, g.colptr, g.rowind, g.value_csr, g.value_csc)
     torch.manual_seed(4)
-    conv = GCNConv(40, 96, precision="bf16", order="auto").cuda()
+    conv = GCNConv(fin, fout, precision="bf16", order="auto").cuda()
     with torch.no_grad():
         conv.bias.uniform_(-0.2, 0.2)
-    assert conv.fuses_activation()
-    x = torch.rand(512, 40, device="cuda")
+    assert conv.fuses_activation() and conv.aggregates_first() == (fin < fout)
+    x = torch.rand(512, fin, device="cuda")
     h = conv.forward_act(x, *gt[:4], *gt[4:], p=0.0)
     assert h.dtype == torch.bfloat16
-    G = torch.randn(512, 96, device="cuda").to(torch.bfloat16)
+    G = torch.randn(512, fout, device="cuda").to(torch.bfloat16)
     h.backward(G)
     ah = _gcn_dense_fp64(g)
     z = ah @ x.double().cpu().numpy()
