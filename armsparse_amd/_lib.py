@@ -43,6 +43,7 @@ SCHEDULES = {"auto": SCHED_AUTO, "row": SCHED_ROW, "nnz": SCHED_NNZ, "merge": SC
 ACCUMULATE = 0x1
 TRUNC4 = 0x2
 FUSED_MLA = 0x4
+VCSR_BALANCED = 0x8
 
 
 class Config(ctypes.Structure):

@@ -184,6 +184,14 @@ __global__ __launch_bounds__(256) void k_spmv_fixup(int nwaves, const int* __res
     if (r >= 0 && rprev != r) y[r] += (r == r63 ? s + ext : s);
 }
 
+// a·b rounded to fp32 by itself: the empty asm makes the product opaque, so the
+// compiler cannot contract it with the following add into a fused multiply-add
+__device__ __forceinline__ float unfused_product(float a, float b) {
+    float p = a * b;
+    asm volatile("" : "+v"(p));
+    return p;
+}
+
 __global__ __launch_bounds__(256) void k_spmv_vcsr(int M, const int* __restrict__ rowptr,
                                                    const int* __restrict__ colind, const float* __restrict__ val,
                                                    const float* __restrict__ x, float* __restrict__ y, int fused) {
@@ -192,15 +200,52 @@ __global__ __launch_bounds__(256) void k_spmv_vcsr(int M, const int* __restrict_
     if (row >= M) return;
     const int beg = rowptr[row], end = rowptr[row + 1];
     float s = 0.0f;
-    if (fused) {
-        for (int idx = beg; idx < end; idx += 4) s = fmaf(val[idx + l], x[colind[idx] + l], s);
-    } else {
-        // vmlaq_f32 is a multiply then an add: forbid contraction into an FMA here
-#pragma clang fp contract(off)
-        for (int idx = beg; idx < end; idx += 4) {
-            const float p = val[idx + l] * x[colind[idx] + l];
-            s = s + p;
+    // the lane's sum runs block after block in the reference's order; blocks come
+    // in batches of 16 whose loads are issued while the previous batch is added (a
+    // hub row of an R-MAT graph is tens of thousands of blocks walked by one
+    // 4-lane group, so its latency must not be paid per block)
+    constexpr int BB = 16;
+    auto load = [&](int base, float (&v)[BB], float (&xv)[BB]) {
+#pragma unroll
+        for (int u = 0; u < BB; ++u) {
+            v[u] = val[base + 4 * u + l];
+            xv[u] = x[colind[base + 4 * u] + l];
         }
+    };
+    auto add = [&](const float (&v)[BB], const float (&xv)[BB]) {
+        if (fused) {
+#pragma unroll
+            for (int u = 0; u < BB; ++u) s = fmaf(v[u], xv[u], s);
+        } else {  // vmlaq_f32 is a multiply then an add: the product is rounded on its own
+#pragma unroll
+            for (int u = 0; u < BB; ++u) s += unfused_product(v[u], xv[u]);
+        }
+    };
+    int idx = beg;
+    if (idx + 4 * BB <= end) {
+        float v0[BB], x0[BB], v1[BB], x1[BB];
+        load(idx, v0, x0);
+        while (true) {
+            idx += 4 * BB;
+            if (idx + 4 * BB > end) {
+                add(v0, x0);
+                break;
+            }
+            load(idx, v1, x1);
+            add(v0, x0);
+            idx += 4 * BB;
+            if (idx + 4 * BB > end) {
+                add(v1, x1);
+                break;
+            }
+            load(idx, v0, x0);
+            add(v1, x1);
+        }
+    }
+    if (fused) {
+        for (; idx < end; idx += 4) s = fmaf(val[idx + l], x[colind[idx] + l], s);
+    } else {
+        for (; idx < end; idx += 4) s += unfused_product(val[idx + l], x[colind[idx] + l]);
     }
     const float t = __fadd_rn(s, __shfl_xor(s, 2, 4));   // lanes 0/2: s0+s2, lanes 1/3: s1+s3
     const float tot = __fadd_rn(t, __shfl_xor(t, 1, 4));  // (s0+s2)+(s1+s3)
@@ -314,8 +359,14 @@ int armsp_spmv_vcsr_f32(int M, int nnz_padded, const int* rowptr, const int* col
     if (M < 0 || nnz_padded < 0 || (nnz_padded & 3)) return ARMSP_ERROR_INVALID_VALUE;
     if (M == 0) return ARMSP_SUCCESS;
     if (!rowptr || !y || (nnz_padded > 0 && (!colind || !val || !x))) return ARMSP_ERROR_INVALID_VALUE;
-    if (flags & ~(ARMSP_ACCUMULATE | ARMSP_FUSED_MLA)) return ARMSP_ERROR_INVALID_VALUE;
+    if (flags & ~(ARMSP_ACCUMULATE | ARMSP_FUSED_MLA | ARMSP_VCSR_BALANCED)) return ARMSP_ERROR_INVALID_VALUE;
     hipStream_t s = as_stream(stream);
+    if (flags & ARMSP_VCSR_BALANCED) {
+        // the padded blocks are a CSR matrix of their own: merge-path SpMV, y += A·x
+        if (flags & ARMSP_FUSED_MLA) return ARMSP_ERROR_INVALID_VALUE;
+        if (nnz_padded == 0) return ARMSP_SUCCESS;
+        return launch_merge<8>(M, nnz_padded, rowptr, colind, val, x, y, ARMSP_ACCUMULATE, s);
+    }
     TimedLaunch t("spmv_vcsr", s);
     hipLaunchKernelGGL(k_spmv_vcsr, dim3((unsigned)(((int64_t)M * 4 + 255) / 256)), dim3(256), 0, s, M, rowptr,
                        colind, val, x, y, (flags & ARMSP_FUSED_MLA) ? 1 : 0);

@@ -185,8 +185,11 @@ def spmv(rowptr: torch.Tensor, colind: torch.Tensor, values: Optional[torch.Tens
 
 
 def vcsr_spmv(rowptr: torch.Tensor, colind: torch.Tensor, values: torch.Tensor, x: torch.Tensor, y: torch.Tensor, *,
-              fused: bool = False) -> torch.Tensor:
-    """y += A·x on the zero-padded 1×4-blocked CSR (sparse_dense_vector, SpMV.cpp:155-179). In place."""
+              fused: bool = False, balanced: bool = False) -> torch.Tensor:
+    """y += A·x on the zero-padded 1×4-blocked CSR (sparse_dense_vector, SpMV.cpp:155-179). In place.
+    Bit-exact with the reference's order by default; ``balanced`` runs the merge-path
+    schedule over the same arrays (skewed rows, within 1e-5 per row)."""
+    _require(not (fused and balanced), "vcsr_spmv: balanced runs the unfused products only")
     fn = "vcsr_spmv"
     _check_csr(fn, rowptr, colind, values)
     _require(values is not None, f"{fn}: values required")
@@ -198,7 +201,7 @@ def vcsr_spmv(rowptr: torch.Tensor, colind: torch.Tensor, values: torch.Tensor, 
     dev = _device() if host else y.device
     rp, ci, va, xd, yd = (_on_device(t, dev) for t in (rowptr, colind, values, x, y))
     call("armsp_spmv_vcsr_f32", m, nnz, _ptr(rp), _ptr(ci), _ptr(va), _ptr(xd), _ptr(yd),
-         _lib.FUSED_MLA if fused else 0, _stream(dev))
+         (_lib.FUSED_MLA if fused else 0) | (_lib.VCSR_BALANCED if balanced else 0), _stream(dev))
     if host:
         y.copy_(yd.cpu())
     return y

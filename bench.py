@@ -199,6 +199,18 @@ def main():
             torch.cuda.current_stream().wait_stream(side)
             el_g, _ = timed(g.replay, 200, 10, [])
             extras[f"cfg0_{nm}_graph_us"] = round(el_g / 200 * 1e6, 2)
+        # the literal `./SpMM 4096 4096 4096 32` product: dense C (+)= A·B, A and B
+        # 4096² at 1% in CSR (sparse_matrix_mul3)
+        Bs2 = data.uniform_csr(4096, 4096, 167_772, data.CFG_SEED + 5)
+        ga = tuple(torch.from_numpy(t).to(dev) for t in (As.rowptr, As.colind, As.values))
+        gb = tuple(torch.from_numpy(t).to(dev) for t in (Bs2.rowptr, Bs2.colind, Bs2.values))
+        gc = torch.zeros((4096, 4096), device=dev)
+        el_g2, fam_g2 = timed(lambda: kernels.spgemm_dense(ga, gb, gc), 50, 5, ["spgemm"])
+        kg = fam_g2["spgemm"][0] / max(fam_g2["spgemm"][1], 1) * 1e-3
+        gbytes = 2 * (4 * 4097 + 8 * 167_772) + 8 * 4096 * 4096
+        extras["spgemm_4096_kernel_us"] = round(kg * 1e6, 1)
+        extras["spgemm_4096_GBps"] = round(gbytes / kg / 1e9, 1)
+        del gc
         A6 = data.uniform_65k()
         r6, c6, v6 = (torch.from_numpy(t).to(dev) for t in (A6.rowptr, A6.colind, A6.values))
         x6 = torch.empty(65536, device=dev)
@@ -219,6 +231,23 @@ def main():
         vb = 4 * (M + 1) + 8 * nnz + 4 * K + 4 * M
         extras["spmv_GBps"] = round(vb * steps_x * 4 / el_v / 1e9, 1)
         extras["spmv_us"] = round(el_v / (steps_x * 4) * 1e6, 1)
+        extras["spmv_kernel_us"] = round(fam_v["spmv"][0] / max(fam_v["spmv"][1], 1) * 1e3, 1)
+        # the reference's vector-aware (zero-padded 1×4 blocks) CSR SpMV, y += A·x
+        Av = data.csr_to_vcsr(A)
+        rv, cv, vv = (torch.from_numpy(t).to(dev) for t in (Av.rowptr, Av.colind, Av.values))
+        xv = torch.zeros(Av.shape[1], device=dev)
+        xv[:K] = x
+        el_w, fam_w = timed(lambda: kernels.vcsr_spmv(rv, cv, vv, xv, y), steps_x * 2, 5, ["spmv_vcsr"])
+        pv = int(cv.numel())
+        wb = 4 * (M + 1) + 8 * pv + 4 * K + 8 * M
+        kw = fam_w["spmv_vcsr"][0] / max(fam_w["spmv_vcsr"][1], 1) * 1e-3
+        extras["vcsr_spmv_padded_nnz"] = pv
+        extras["vcsr_spmv_kernel_us"] = round(kw * 1e6, 1)  # bit-exact: one 4-lane group per row
+        extras["vcsr_spmv_GBps"] = round(wb / kw / 1e9, 1)
+        el_wb, _ = timed(lambda: kernels.vcsr_spmv(rv, cv, vv, xv, y, balanced=True), steps_x * 2, 5, [])
+        extras["vcsr_spmv_balanced_us"] = round(el_wb / (steps_x * 2) * 1e6, 1)  # merge path, wall per call
+        extras["vcsr_spmv_balanced_GBps"] = round(wb * steps_x * 2 / el_wb / 1e9, 1)
+        del rv, cv, vv, xv, Av
     if "sddmm" in only:
         for kd in (32, 128):
             U = torch.empty((M, kd), device=dev)

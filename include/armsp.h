@@ -75,6 +75,8 @@ typedef struct armsp_config {
 #define ARMSP_ACCUMULATE 0x1   /* C += A·B (reference rowBaseSpmm2/3, csr_spmm_kernel); else C = A·B (origin) */
 #define ARMSP_TRUNC4     0x2   /* rowBaseSpmm semantics: only the first 4*floor(nnz_row/4) nonzeros of each row */
 #define ARMSP_FUSED_MLA  0x4   /* VCSR SpMV: fused multiply-add instead of the unfused vmlaq_f32 reading */
+#define ARMSP_VCSR_BALANCED 0x8 /* VCSR SpMV: merge-path schedule over the padded blocks (load-balanced on
+                                   skewed rows; within 1e-5 per row, not bit-exact) */
 
 /* ---------------------------------------------------------------- runtime */
 int armsp_version(void);
@@ -172,7 +174,10 @@ int armsp_spmv_f32(int M, int K, int nnz,
  * colind[idx] of a block start is a multiple of 4.  Replaces
  * sparse_dense_vector (SpMV.cpp:155-179): per row four partial sums
  * s_l += val*x (unfused, or fused with ARMSP_FUSED_MLA) reduced as
- * (s0+s2)+(s1+s3), bit-exact with the reference's order.  x must hold
+ * (s0+s2)+(s1+s3), bit-exact with the reference's order — one 4-lane group
+ * per row, so a row of many blocks bounds the call.  ARMSP_VCSR_BALANCED runs
+ * the merge-path SpMV over the same arrays instead (rows of any length spread
+ * over the device; the merge path's order of additions).  x must hold
  * K rounded up to a multiple of 4 entries. */
 int armsp_spmv_vcsr_f32(int M, int nnz_padded,
                         const int* rowptr, const int* colind, const float* val,

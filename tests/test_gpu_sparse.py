@@ -66,6 +66,39 @@ def test_vcsr_bitexact(fused):
     assert np.array_equal(d_y.cpu().numpy().view(np.uint32), expect.view(np.uint32))
 
 
+@pytest.mark.parametrize("fused", [False, True])
+def test_vcsr_bitexact_skewed_rows(fused):
+    """Rows of thousands of blocks (an R-MAT hub) take the kernel's batched loads and
+    its tail: still bit-exact with the restated sparse_dense_vector."""
+    case(f"vcsr skewed fused={fused}")
+    a = data.rmat_csr(14, 1 << 18, seed=11)
+    v = data.csr_to_vcsr(a)
+    assert np.diff(v.rowptr).max() > 4 * 32
+    x = data.host_uniform(v.shape[1], 5, 1)
+    y0 = np.random.default_rng(4).uniform(-1, 1, v.shape[0]).astype(np.float32)
+    d_rp, d_ci, d_va, d_x, d_y = dev(v.rowptr, v.colind, v.values, x, y0)
+    kernels.vcsr_spmv(d_rp, d_ci, d_va, d_x, d_y, fused=fused)
+    expect = oracle.sparse_dense_vector(x, y0.copy(), v.rowptr, v.colind, v.values, fused=fused)
+    assert np.array_equal(d_y.cpu().numpy().view(np.uint32), expect.view(np.uint32))
+
+
+def test_vcsr_balanced_schedule():
+    """ARMSP_VCSR_BALANCED: the merge path over the padded blocks, y += A·x within
+    the SpMV tolerance of FP64 (the reference's order is not kept)."""
+    case("vcsr balanced")
+    a = data.rmat_csr(14, 1 << 18, seed=11)
+    v = data.csr_to_vcsr(a)
+    x = data.host_uniform(v.shape[1], 5, 1)
+    y0 = np.random.default_rng(4).uniform(-1, 1, v.shape[0]).astype(np.float32)
+    d_rp, d_ci, d_va, d_x, d_y = dev(v.rowptr, v.colind, v.values, x, y0)
+    kernels.vcsr_spmv(d_rp, d_ci, d_va, d_x, d_y, balanced=True)
+    ref = oracle.spmv_f64(v.rowptr, v.colind, v.values, x, y0)
+    bound = oracle.spmv_f64(v.rowptr, v.colind, np.abs(v.values), np.abs(x))
+    assert np.all(np.abs(d_y.cpu().numpy() - ref) <= TOL * (bound + np.abs(y0)))
+    with pytest.raises(RuntimeError):
+        kernels.vcsr_spmv(d_rp, d_ci, d_va, d_x, d_y, fused=True, balanced=True)
+
+
 def test_spmv_full_rmat_1m():
     case("spmv full 1M R-MAT")
     a = rmat_1m_cached()
