@@ -1260,14 +1260,18 @@ int spmm_impl(int M, int K, int N, int nnz, const int* rowptr, const int* colind
     // wave tile is 64 × that run, so the chunk is rounded to 256, 512 or 1024.  From
     // N = 5 up a whole wave per nonzero is faster (1M R-MAT, N = 8: 238 vs 347 us)
     const bool narrow = merge && !wide && N <= 4 && c.group == 0;
-    // AUTO half-wave slices (see below), and a whole wave per nonzero on narrow rows of
-    // B (N < 32), run best with 256 items per wave and 4 gathers per batch
-    // slices start on 128-byte lines only when every row of B does: otherwise the line
-    // a slice boundary cuts is fetched by two XCDs (measured on the 1M R-MAT: bf16
-    // N=100 688 vs 490 us, fp32 N=200 1840 vs 1707 us against the stream kernel)
+    // AUTO column slices (see below) only where they pay: slices start on 128-byte
+    // lines only when every row of B does — otherwise the line a slice boundary cuts
+    // is fetched by two XCDs (1M R-MAT: bf16 N=100 688 vs 490 us, fp32 N=200 1840 vs
+    // 1707 us against the stream kernel) — and the row must fill its slices (bf16
+    // N=100 in 256-byte rows: 4.9 ms with a 72-byte last slice, 4.6 ms with the
+    // stream kernel on a 126M-edge graph)
     const int esz = (int)sizeof(TB);
     const bool line_rows = ((int64_t)ldb * esz) % 128 == 0 && aligned(B, 128);
-    const bool slice_rows = line_rows && (N * esz >= 512 || (esz == 2 && N * esz >= 128));
+    const bool slice_rows = line_rows && ((N * esz >= 512 && (N * esz) % 256 == 0) ||
+                                          (esz == 2 && N * esz >= 128 && N * esz < 512 && (N * esz) % 128 == 0));
+    // half-wave slices, and a whole wave per nonzero on narrow rows of B (N < 32), run
+    // best with 256 items per wave and 4 gathers per batch
     if (merge && !wide && !narrow && c.group == 0 && (slice_rows || N < 32)) {
         if (c.chunk <= 0) chunk = 256;
         if (c.unroll <= 0) U = 4;

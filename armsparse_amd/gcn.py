@@ -69,6 +69,20 @@ def _adam(model: Net, capturable: bool):
                             fused=fused)
 
 
+def line_pitched(x: torch.Tensor) -> torch.Tensor:
+    """x [n×f] as a view whose rows start on 128-byte lines (pitch rounded up to 128
+    bytes, the padding zero), so the aggregation's gathers of a row touch no more
+    lines than the row needs."""
+    n, f = x.shape
+    per_line = 128 // x.element_size()
+    fp = (f + per_line - 1) // per_line * per_line
+    if fp == f and x.is_contiguous():
+        return x
+    buf = torch.zeros((n, fp), dtype=x.dtype, device=x.device)
+    buf[:, :f] = x
+    return buf[:, :f]
+
+
 def make_task(precision: str = "fp32", seed: int = data.CFG_SEED + 3, n: int = 19_717, pairs: int = 44_324,
               features: int = 500, hidden: int = 64, classes: int = 3, feat_density: float = 0.01,
               device: str = "cuda", dropout: float = 0.5, capturable: bool = False,
@@ -89,7 +103,7 @@ def make_task(precision: str = "fp32", seed: int = data.CFG_SEED + 3, n: int = 1
     t = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev)  # noqa: E731
     xt = t(x)
     if precision == "bf16":
-        xt = xt.to(torch.bfloat16)
+        xt = line_pitched(xt.to(torch.bfloat16))
     graph = (t(g.rowptr), t(g.colind), t(g.colptr), t(g.rowind), t(g.value_csr), t(g.value_csc))
     model = Net(features, hidden, classes, precision, dropout, order).to(dev)
     return GCNTask(model, _adam(model, capturable), xt, graph, t(y), t(train), precision, int(g.colind.size))
@@ -101,7 +115,7 @@ SCALED = dict(n=2_400_000, pairs=62_000_000, features=100, hidden=256, feat_dens
 def retarget(task: GCNTask, precision: str, capturable: bool = False, order: str = "transform_first") -> GCNTask:
     """A fresh model and optimizer of `precision` (and layer `order`, which needs
     bf16's fused aggregation) on the same graph, features and labels."""
-    x = task.x.float() if precision == "fp32" else task.x.to(torch.bfloat16)
+    x = task.x.float() if precision == "fp32" else line_pitched(task.x.to(torch.bfloat16))
     dev = x.device
     model = Net(x.shape[1], task.model.conv1.out_channels, task.model.conv2.out_channels, precision,
                 task.model.p, order).to(dev)

@@ -90,6 +90,12 @@ class SPMMMaxFunction(torch.autograd.Function):
         return None, None, gf, None, None
 
 
+def _rows(x: torch.Tensor) -> torch.Tensor:
+    """x with unit column stride; a row pitch wider than the row (feature rows kept on
+    128-byte lines) is kept, since every kernel here takes the pitch as an argument."""
+    return x if x.dim() == 2 and x.stride(1) == 1 else x.contiguous()
+
+
 class GCNAggregate(torch.autograd.Function):
     """y = diag(in_norm)·A·diag(out_norm)·h + bias in one fused SpMM (armsp_spmm_ex);
     backward dh = diag(out_norm)·Aᵀ·diag(in_norm)·dy in one SpMM on (colptr, rowind).
@@ -282,14 +288,13 @@ class GCNConv(torch.nn.Module):
                 self._seed = torch.randint(0, 2**62, (1,), dtype=torch.int64).to(x.device)
             seed = self._seed
         if not self.aggregates_first():
-            h = _BF16Linear.apply(x.contiguous(), self.weight, True)  # bf16 features for the gathers
+            h = _BF16Linear.apply(_rows(x), self.weight, True)  # bf16 features for the gathers
             return _AggregateBiasReluDropout.apply(h, rowptr, colind, colptr, rowind, coefs[0], coefs[1], in_n,
                                                    out_n, self.bias, float(p), seed)
         h = x
         if h.dtype != torch.bfloat16:
             h = h.to(torch.bfloat16) if h.requires_grad else kernels.to_bf16(h.contiguous())
-        z = GCNAggregate.apply(h.contiguous(), rowptr, colind, colptr, rowind, coefs[0], coefs[1], in_n, out_n,
-                               None)
+        z = GCNAggregate.apply(_rows(h), rowptr, colind, colptr, rowind, coefs[0], coefs[1], in_n, out_n, None)
         return _LinearBiasReluDropout.apply(z, self.weight, self.bias, float(p), seed)
 
     def aggregates_first(self) -> bool:
@@ -323,8 +328,7 @@ class GCNConv(torch.nn.Module):
         h = x
         if self.precision == "bf16" and h.dtype != torch.bfloat16:
             h = h.to(torch.bfloat16) if h.requires_grad else kernels.to_bf16(h.contiguous())
-        z = GCNAggregate.apply(h.contiguous(), rowptr, colind, colptr, rowind, coefs[0], coefs[1], in_n, out_n,
-                               None)
+        z = GCNAggregate.apply(_rows(h), rowptr, colind, colptr, rowind, coefs[0], coefs[1], in_n, out_n, None)
         y = _BF16Linear.apply(z, self.weight, False) if self.precision == "bf16" else torch.matmul(z, self.weight)
         return y + self.bias if self.bias is not None else y
 
@@ -336,7 +340,7 @@ class GCNConv(torch.nn.Module):
             # bf16 features for the aggregation straight from the GEMM (fused path with
             # more than 8 output channels); fp32 otherwise
             out_bf16 = self.fused and self.out_channels > 8
-            x = _BF16Linear.apply(x.contiguous(), self.weight, out_bf16)
+            x = _BF16Linear.apply(_rows(x), self.weight, out_bf16)
         else:
             x = torch.matmul(x, self.weight)
         if self.fused:

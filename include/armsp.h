@@ -57,8 +57,8 @@ typedef struct armsp_config {
                          SpMM MERGE: 0 automatic (lane walks for N <= 4, half-wave
                          slices with 8-byte lanes for B rows of 512+ bytes and with
                          4-byte lanes for bf16 rows of 128-511 bytes, when ldb
-                         spans a multiple of 128 bytes; else a whole wave per
-                         nonzero); 64 a whole wave per nonzero; 32 half
+                         spans a multiple of 128 bytes and the row fills whole
+                         slices; else a whole wave per nonzero); 64 a whole wave per nonzero; 32 half
                          a wave per nonzero over column slices of 32·vec elements
                          spread over the XCDs; 1 forces the 64-bit-address kernel
                          (otherwise used only when a dense-row offset needs more than
