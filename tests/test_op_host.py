@@ -15,3 +15,21 @@ def test_gcnconv_order_rules():
         GCNConv(100, 256, order="auto")  # fp32 without the fused aggregation
     with pytest.raises(ValueError, match="order"):
         GCNConv(100, 256, precision="bf16", order="sideways")
+
+
+def test_fused_activation_rules():
+    assert GCNConv(500, 64, precision="bf16").fuses_activation()  # transform-first hidden layer
+    assert GCNConv(100, 256, precision="bf16", order="auto").fuses_activation()  # aggregate-first
+    assert not GCNConv(64, 3, precision="bf16").fuses_activation()  # 3 outputs: no 8-column groups
+    assert not GCNConv(500, 64).fuses_activation()  # fp32 keeps the reference's separate ops
+
+
+def test_line_pitched_features():
+    import torch
+    from armsparse_amd.gcn import line_pitched
+    x = torch.arange(6 * 100, dtype=torch.float32).reshape(6, 100).to(torch.bfloat16)
+    v = line_pitched(x)
+    assert v.shape == (6, 100) and v.stride() == (128, 1) and torch.equal(v, x)
+    assert v.untyped_storage().nbytes() == 6 * 128 * 2  # rows on 128-byte lines, zero padding
+    y = torch.zeros(3, 64, dtype=torch.bfloat16)
+    assert line_pitched(y) is y  # already 128 bytes a row
