@@ -186,19 +186,22 @@ __global__ __launch_bounds__(256) void k_spmv_fixup(int nwaves, const int* __res
 
 // a·b rounded to fp32 by itself: the empty asm makes the product opaque, so the
 // compiler cannot contract it with the following add into a fused multiply-add
+// (not volatile: the scheduler may still move it, and the loads around it)
 __device__ __forceinline__ float unfused_product(float a, float b) {
     float p = a * b;
-    asm volatile("" : "+v"(p));
+    asm("" : "+v"(p));
     return p;
 }
 
 __global__ __launch_bounds__(256) void k_spmv_vcsr(int M, const int* __restrict__ rowptr,
                                                    const int* __restrict__ colind, const float* __restrict__ val,
-                                                   const float* __restrict__ x, float* __restrict__ y, int fused) {
+                                                   const float* __restrict__ x, float* __restrict__ y, int fused,
+                                                   int long_blocks) {
     const int l = threadIdx.x & 3;
     const int row = (int)((blockIdx.x * 256u + threadIdx.x) >> 2);
     if (row >= M) return;
     const int beg = rowptr[row], end = rowptr[row + 1];
+    if (long_blocks > 0 && end - beg > 4 * long_blocks) return;  // k_spmv_vcsr_long's row
     float s = 0.0f;
     // the lane's sum runs block after block in the reference's order; blocks come
     // in batches of 16 whose loads are issued while the previous batch is added (a
@@ -250,6 +253,103 @@ __global__ __launch_bounds__(256) void k_spmv_vcsr(int M, const int* __restrict_
     const float t = __fadd_rn(s, __shfl_xor(s, 2, 4));   // lanes 0/2: s0+s2, lanes 1/3: s1+s3
     const float tot = __fadd_rn(t, __shfl_xor(t, 1, 4));  // (s0+s2)+(s1+s3)
     if (l == 0) y[row] = __fadd_rn(y[row], tot);
+}
+
+// Rows of more than VCSR_LONG blocks: k_vcsr_find_long lists them, and a wave
+// per listed row walks it in batches of 256 blocks.  All 64 lanes load a batch
+// (16-byte loads of values and x) into LDS while lanes 0-3 add the previous
+// batch in block order — each lane l keeps the reference's partial sum s_l, so
+// the result stays bit-exact — and the global loads of the next batch are in
+// flight during those adds.
+constexpr int VCSR_LONG = 512;
+constexpr int VCSR_BATCH = 256;
+
+__global__ void k_vcsr_find_long(int M, const int* __restrict__ rowptr, int* __restrict__ list,
+                                 int* __restrict__ count) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= M) return;
+    if (rowptr[r + 1] - rowptr[r] > 4 * VCSR_LONG) list[atomicAdd(count, 1)] = r;
+}
+
+__global__ void k_zero_count(int* count) { *count = 0; }
+
+__global__ __launch_bounds__(256) void k_spmv_vcsr_long(const int* __restrict__ list, const int* __restrict__ count,
+                                                        const int* __restrict__ rowptr,
+                                                        const int* __restrict__ colind,
+                                                        const float* __restrict__ val, const float* __restrict__ x,
+                                                        float* __restrict__ y, int fused) {
+    __shared__ float4 s_v[4][2][VCSR_BATCH];
+    __shared__ float4 s_x[4][2][VCSR_BATCH];
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    const int nlong = *count;
+    const int nwaves = (int)gridDim.x * 4;
+    for (int i = uniform((int)(blockIdx.x * 4u + wid)); i < nlong; i += nwaves) {
+        const int r = uniform(list[i]);
+        const int beg = uniform(rowptr[r]);
+        const int nb = uniform((rowptr[r + 1] - beg) >> 2);
+        const int nbat = (nb + VCSR_BATCH - 1) / VCSR_BATCH;
+        float4 rv[VCSR_BATCH / 64], rx[VCSR_BATCH / 64];
+        auto fetch = [&](int k) {
+#pragma unroll
+            for (int q = 0; q < VCSR_BATCH / 64; ++q) {
+                const int b = k * VCSR_BATCH + lane + 64 * q;
+                rv[q] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                rx[q] = rv[q];
+                if (b < nb) {
+                    rv[q] = *reinterpret_cast<const float4*>(val + beg + 4 * b);
+                    rx[q] = *reinterpret_cast<const float4*>(x + colind[beg + 4 * b]);
+                }
+            }
+        };
+        float s = 0.0f;
+        fetch(0);
+        for (int k = 0; k < nbat; ++k) {
+            const int buf = k & 1;
+#pragma unroll
+            for (int q = 0; q < VCSR_BATCH / 64; ++q) {
+                s_v[wid][buf][lane + 64 * q] = rv[q];
+                s_x[wid][buf][lane + 64 * q] = rx[q];
+            }
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            if (k + 1 < nbat) fetch(k + 1);  // in flight during the adds below
+            const int cnt = min(VCSR_BATCH, nb - k * VCSR_BATCH);
+            if (lane < 4) {
+                const float* pv = reinterpret_cast<const float*>(&s_v[wid][buf][0]) + lane;
+                const float* px = reinterpret_cast<const float*>(&s_x[wid][buf][0]) + lane;
+                // 16 blocks' operands read from LDS ahead of their in-order adds
+                int b = 0;
+                for (; b + 16 <= cnt; b += 16) {
+                    float a[16], c[16];
+#pragma unroll
+                    for (int u = 0; u < 16; ++u) {
+                        a[u] = pv[4 * (b + u)];
+                        c[u] = px[4 * (b + u)];
+                    }
+                    if (fused) {
+#pragma unroll
+                        for (int u = 0; u < 16; ++u) s = fmaf(a[u], c[u], s);
+                    } else {
+#pragma unroll
+                        for (int u = 0; u < 16; ++u) s += unfused_product(a[u], c[u]);
+                    }
+                }
+                if (fused) {
+                    for (; b < cnt; ++b) s = fmaf(pv[4 * b], px[4 * b], s);
+                } else {
+                    for (; b < cnt; ++b) s += unfused_product(pv[4 * b], px[4 * b]);
+                }
+            }
+        }
+        const float t = __fadd_rn(s, __shfl_xor(s, 2, 4));   // lanes 0/2: s0+s2, lanes 1/3: s1+s3
+        const float tot = __fadd_rn(t, __shfl_xor(t, 1, 4));  // (s0+s2)+(s1+s3)
+        if (lane == 0) y[r] = __fadd_rn(y[r], tot);
+        // the next row's first batch reuses this row's LDS: every lane is past the adds
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    }
 }
 
 template <int IPT>
@@ -367,10 +467,31 @@ int armsp_spmv_vcsr_f32(int M, int nnz_padded, const int* rowptr, const int* col
         if (nnz_padded == 0) return ARMSP_SUCCESS;
         return launch_merge<8>(M, nnz_padded, rowptr, colind, val, x, y, ARMSP_ACCUMULATE, s);
     }
+    const int fused = (flags & ARMSP_FUSED_MLA) ? 1 : 0;
+    // rows of more than VCSR_LONG blocks (possible only above that many blocks in
+    // all) go to the wave-per-row kernel
+    const bool split_long = nnz_padded / 4 > VCSR_LONG;
+    void* ws = nullptr;
+    int* list = nullptr;
+    int* count = nullptr;
+    if (split_long) {
+        const int st = ws_alloc(&ws, ((size_t)M + 64) * sizeof(int), s);
+        if (st) return st;
+        count = static_cast<int*>(ws);
+        list = count + 64;
+        hipLaunchKernelGGL(k_zero_count, dim3(1), dim3(1), 0, s, count);
+        hipLaunchKernelGGL(k_vcsr_find_long, dim3((unsigned)((M + 255) / 256)), dim3(256), 0, s, M, rowptr, list,
+                           count);
+    }
     TimedLaunch t("spmv_vcsr", s);
     hipLaunchKernelGGL(k_spmv_vcsr, dim3((unsigned)(((int64_t)M * 4 + 255) / 256)), dim3(256), 0, s, M, rowptr,
-                       colind, val, x, y, (flags & ARMSP_FUSED_MLA) ? 1 : 0);
-    ARMSP_LAUNCHED();
+                       colind, val, x, y, fused, split_long ? VCSR_LONG : 0);
+    if (split_long)
+        hipLaunchKernelGGL(k_spmv_vcsr_long, dim3((unsigned)(2 * device_cus())), dim3(256), 0, s, list, count, rowptr,
+                           colind, val, x, y, fused);
+    const hipError_t e = hipGetLastError();
+    if (ws) ws_free(ws, s);
+    if (e != hipSuccess) return hip_fail(e);
     t.stop();
     return ARMSP_SUCCESS;
 }
