@@ -108,17 +108,33 @@ class GCNAggregate(torch.autograd.Function):
     def forward(ctx, h, rowptr, colind, colptr, rowind, coef_fwd, coef_bwd, in_norm, out_norm, bias):
         y = kernels.spmm(rowptr, colind, coef_fwd, h, row_scale=in_norm, bias=bias)
         ctx.graph = (colptr, rowind, coef_bwd, out_norm, h.dtype)
+        # learnable edge weights reach coef_fwd (= w_csr·out_norm[colind]); its gradient
+        # needs h and the forward pattern
+        ctx.edge_grad = (h, rowptr, colind, in_norm) if ctx.needs_input_grad[5] else None
         return y
 
     @staticmethod
     def backward(ctx, gy):
         colptr, rowind, coef_bwd, out_norm, hdt = ctx.graph
         g = gy.contiguous()
-        if hdt == torch.bfloat16:
-            g = kernels.to_bf16(g.float())
-        dh = kernels.spmm(colptr, rowind, coef_bwd, g, row_scale=out_norm)
+        dh = dcoef = None
+        if ctx.needs_input_grad[0]:
+            gb = kernels.to_bf16(g.float()) if hdt == torch.bfloat16 else g
+            dh = kernels.spmm(colptr, rowind, coef_bwd, gb, row_scale=out_norm).to(hdt)
+        if ctx.edge_grad is not None:
+            dcoef = _edge_coef_grad(g, *ctx.edge_grad)
         dbias = kernels.colsum(gy.float()) if ctx.needs_input_grad[9] else None
-        return dh.to(hdt), None, None, None, None, None, None, None, None, dbias
+        return dh, None, None, None, None, dcoef, None, None, None, dbias
+
+
+def _edge_coef_grad(gy, h, rowptr, colind, in_norm):
+    """d loss / d coef[e] of y = diag(in_norm)·A(coef)·h: in_norm[r]·<gy[r], h[c]> over the
+    pattern of A (an SDDMM, fp32).  Autograd carries it back to the edge weights through
+    the coefficients' products (the SDDMM the unfused SPMMFunction path runs for dA)."""
+    g = gy.float()
+    if in_norm is not None:
+        g = g * in_norm.reshape(-1, 1)
+    return kernels.sddmm(rowptr, colind, g.contiguous(), h.float().contiguous())
 
 
 def gcn_edge_coefficients(colind, rowind, w_csr, w_csc, in_norm, out_norm):
@@ -203,6 +219,7 @@ class _AggregateBiasReluDropout(torch.autograd.Function):
         out, mask = kernels.bias_relu_dropout(y, None if bias is None else bias.detach(), p, seed)
         ctx.save_for_backward(mask)
         ctx.graph = (colptr, rowind, coef_bwd, out_norm, h.dtype)
+        ctx.edge_grad = (h, rowptr, colind, in_norm) if ctx.needs_input_grad[5] else None
         ctx.p = p
         ctx.has_bias = bias is not None
         return out
@@ -213,10 +230,12 @@ class _AggregateBiasReluDropout(torch.autograd.Function):
         colptr, rowind, coef_bwd, out_norm, hdt = ctx.graph
         gyb, dbias = kernels.bias_relu_dropout_backward(gout.to(torch.bfloat16), mask, ctx.p,
                                                         ctx.has_bias and ctx.needs_input_grad[9])
-        dh = None
+        dh = dcoef = None
         if ctx.needs_input_grad[0]:
             dh = kernels.spmm(colptr, rowind, coef_bwd, gyb, row_scale=out_norm).to(hdt)
-        return dh, None, None, None, None, None, None, None, None, dbias, None, None
+        if ctx.edge_grad is not None:
+            dcoef = _edge_coef_grad(gyb, *ctx.edge_grad)
+        return dh, None, None, None, None, dcoef, None, None, None, dbias, None, None
 
 
 class GCNConv(torch.nn.Module):
@@ -271,7 +290,8 @@ class GCNConv(torch.nn.Module):
 
     def fuses_activation(self) -> bool:
         """forward_act available: a fused bf16 layer with a multiple of 8 outputs."""
-        return self.precision == "bf16" and self.fused and self.out_channels % 8 == 0
+        # (the backward pass's column sums of the bias gradient hold at most 2048 columns)
+        return self.precision == "bf16" and self.fused and self.out_channels % 8 == 0 and self.out_channels <= 2048
 
     def forward_act(self, x, rowptr, colind, colptr, rowind, edge_weight_csr=None, edge_weight_csc=None,
                     p: float = 0.0):
@@ -279,7 +299,7 @@ class GCNConv(torch.nn.Module):
         dropout and cast fused into one pass after the layer's last product (a
         hidden layer of Net; fuses_activation())."""
         if not self.fuses_activation():
-            raise RuntimeError("forward_act needs a fused bf16 layer with a multiple of 8 outputs")
+            raise RuntimeError("forward_act needs a fused bf16 layer with a multiple of 8 outputs, at most 2048")
         in_n, out_n, coefs = self._norms_and_coefs(rowptr, colind, colptr, rowind, edge_weight_csr,
                                                    edge_weight_csc)
         seed = None

@@ -115,6 +115,58 @@ def test_gcnconv_forward(graph512, precision):
     assert err <= (1e-5 if precision == "fp32" else 1e-2)
 
 
+@pytest.mark.parametrize("order", ["transform_first", "aggregate_first"])
+@pytest.mark.parametrize("precision", ["fp32", "bf16"])
+def test_fused_gcnconv_edge_weight_gradient(graph512, precision, order):
+    """Learnable CSR edge weights get the same gradient through the fused aggregation
+    (coefficients folded once per graph) as through the unfused SPMMFunction path, whose
+    dA is the SDDMM closed form; bf16 features within 1e-2."""
+    g, w, cp, ri, wt = graph512
+    rp, ci, d_cp, d_ri, d_wt = dev(g.rowptr, g.colind, cp, ri, wt)
+    torch.manual_seed(3)
+    x = torch.rand(512, 24, device="cuda")
+    R = torch.randn(512, 16, device="cuda")
+    grads = {}
+    for fused in (False, True):
+        torch.manual_seed(4)
+        prec = precision if fused else "fp32"
+        conv = GCNConv(24, 16, precision=prec, fused=fused, order=order if fused else "transform_first").cuda()
+        with torch.no_grad():
+            conv.bias.uniform_(-0.1, 0.1)
+        ew = torch.from_numpy(w).cuda().requires_grad_(True)
+        (conv(x, rp, ci, d_cp, d_ri, ew, d_wt) * R).sum().backward()
+        grads[fused] = ew.grad.detach().cpu().numpy().astype(np.float64)
+    a, b = grads[True], grads[False]
+    rel = np.linalg.norm(a - b) / np.linalg.norm(b)
+    print(f"edge-weight grad fused {precision}/{order} vs unfused fp32: rel err {rel:.2e}")
+    assert rel <= (1e-5 if precision == "fp32" else 1e-2)
+
+
+def test_fused_activation_layer_edge_weight_gradient(graph512):
+    """forward_act (fused bf16 hidden layer: aggregation + bias/ReLU/dropout epilogue) also
+    returns the edge-weight gradient: equal to the fp32 unfused layer's with dropout off."""
+    g, w, cp, ri, wt = graph512
+    rp, ci, d_cp, d_ri, d_wt = dev(g.rowptr, g.colind, cp, ri, wt)
+    torch.manual_seed(3)
+    x = torch.rand(512, 24, device="cuda")
+    R = torch.randn(512, 16, device="cuda")
+    torch.manual_seed(4)
+    ref = GCNConv(24, 16, precision="fp32", fused=False).cuda()
+    torch.manual_seed(4)
+    conv = GCNConv(24, 16, precision="bf16").cuda()
+    for c in (ref, conv):
+        with torch.no_grad():
+            c.bias.fill_(0.05)
+    ew0 = torch.from_numpy(w).cuda().requires_grad_(True)
+    (torch.relu(ref(x, rp, ci, d_cp, d_ri, ew0, d_wt)) * R).sum().backward()
+    ew1 = torch.from_numpy(w).cuda().requires_grad_(True)
+    (conv.forward_act(x, rp, ci, d_cp, d_ri, ew1, d_wt, p=0.0).float() * R).sum().backward()
+    a, b = ew1.grad.double().cpu().numpy(), ew0.grad.double().cpu().numpy()
+    rel = np.linalg.norm(a - b) / np.linalg.norm(b)
+    print(f"edge-weight grad of the fused activation layer vs fp32: rel err {rel:.2e}")
+    assert rel <= 2e-2
+
+
 @pytest.mark.parametrize("precision", ["fp32", "bf16"])
 def test_gcn_training_step(precision):
     """2-layer GCN on the PubMed-shaped synthetic graph: loss decreases, grads match FP32 run."""

@@ -22,6 +22,9 @@ def test_fused_activation_rules():
     assert GCNConv(100, 256, precision="bf16", order="auto").fuses_activation()  # aggregate-first
     assert not GCNConv(64, 3, precision="bf16").fuses_activation()  # 3 outputs: no 8-column groups
     assert not GCNConv(500, 64).fuses_activation()  # fp32 keeps the reference's separate ops
+    # the fused backward's bias-gradient column sums hold at most 2048 columns
+    assert GCNConv(16, 2048, precision="bf16").fuses_activation()
+    assert not GCNConv(16, 2056, precision="bf16").fuses_activation()
 
 
 def test_line_pitched_features():
