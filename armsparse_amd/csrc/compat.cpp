@@ -42,18 +42,29 @@ struct Dev {
     }
 };
 
-void check_csr(const char* fn, int M, int K, const int* rowPtr, const int* colInd) {
+// The reference kernels take any CSR whose row pointers start at 0 and never
+// decrease and whose column indices are in range: toCSR writes each row's
+// columns in increasing order, toCSR2 (SDMM.cpp:371-394) every odd row in
+// decreasing order, and the kernels walk a row's entries in storage order,
+// whatever it is.  `distinct` also rejects a column repeated within a row (the
+// sparse B of sparse_matrix_mul3, whose row entries 64 lanes scatter at once).
+void check_csr(const char* fn, int M, int K, const int* rowPtr, const int* colInd, bool distinct = false) {
     if (M < 0 || K < 0) die(fn, "negative dimension");
     if (!rowPtr) die(fn, "null rowPtr");
+    if (rowPtr[0] != 0) die(fn, "rowPtr[0] != 0");
+    for (int r = 0; r < M; ++r)
+        if (rowPtr[r + 1] < rowPtr[r]) die(fn, "rowPtr is decreasing");
     const int nnz = rowPtr[M];
     if (nnz > 0 && !colInd) die(fn, "null colInd");
-    switch (armsp_csr_check(M, K, nnz, rowPtr, colInd)) {
-        case 0: return;
-        case 1: die(fn, "rowPtr[0] != 0");
-        case 2: die(fn, "rowPtr is decreasing");
-        case 3: die(fn, "rowPtr[M] != nnz");
-        case 4: die(fn, "column index out of range");
-        default: die(fn, "column indices not strictly increasing within a row");
+    for (int p = 0; p < nnz; ++p)
+        if (colInd[p] < 0 || colInd[p] >= K) die(fn, "column index out of range");
+    if (distinct && nnz > 0) {
+        std::vector<int> seen((size_t)K, -1);  // the last row each column appeared in
+        for (int r = 0; r < M; ++r)
+            for (int p = rowPtr[r]; p < rowPtr[r + 1]; ++p) {
+                if (seen[(size_t)colInd[p]] == r) die(fn, "column index repeated within a row");
+                seen[(size_t)colInd[p]] = r;
+            }
     }
 }
 
@@ -128,7 +139,7 @@ void sparse_matrix_mul3(int rowPtr1[], int colInd1[], float val1[], int rowPtr2[
     const char* fn = "sparse_matrix_mul3";
     if (N < 0) die(fn, "negative dimension");
     check_csr(fn, M, K, rowPtr1, colInd1);
-    check_csr(fn, K, N, rowPtr2, colInd2);
+    check_csr(fn, K, N, rowPtr2, colInd2, true);
     if (M == 0 || N == 0) return;
     if (!matrixC) die(fn, "null output");
     const int nnz1 = rowPtr1[M], nnz2 = rowPtr2[K];

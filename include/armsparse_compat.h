@@ -8,7 +8,9 @@
  * reference's per-element fmaf order, so results are bit-exact with the
  * reference's FMA arithmetic), and C is copied back.  `threads` is accepted
  * and ignored.  Preconditions the reference leaves undefined are checked
- * (canonical CSR, the blocking divisibility rules); on any failure the shim
+ * (rowPtr[0] = 0 and non-decreasing, column indices in range, the blocking
+ * divisibility rules; a row's columns may come in any order, as toCSR2,
+ * SDMM.cpp:371-394, writes them); on any failure the shim
  * prints "armsparse: <function>: <reason>" to stderr and exits with
  * EXIT_FAILURE, as the reference's own error paths do
  * (gcn/pytorch-custom/spmm_kernel.cu:5-19, numaAwareSDMM.cpp:353-357).
@@ -36,7 +38,8 @@ void irregular(int rowPtr[], int colInd[], float val[], float B[], float C[], in
 /* C = A·B (SDMM.cpp:422-448). */
 void origin(int rowPtr[], int colInd[], float val[], float B[], float C[], int M, int K, int N, int threads);
 
-/* Dense C[M×N] += A[M×K]·B[K×N], A and B CSR (SpMM.cpp:309-327). */
+/* Dense C[M×N] += A[M×K]·B[K×N], A and B CSR (SpMM.cpp:309-327); a row of B
+ * must not hold a column twice. */
 void sparse_matrix_mul3(int rowPtr1[], int colInd1[], float val1[], int rowPtr2[], int colInd2[], float val2[],
                         float* matrixC, int M, int K, int N, int threads);
 

@@ -95,24 +95,45 @@ _SHIM_MAIN = textwrap.dedent(r"""
         if (!std::strcmp(argv[1], "decreasing")) rowBaseSpmm3(rp, ci, v, B, C, 2, 2, 4, 1);
         int rp2[] = {0, 1, 2}, ci2[] = {0, 5};
         if (!std::strcmp(argv[1], "range")) rowBaseSpmm3(rp2, ci2, v, B, C, 2, 2, 4, 1);
+        // an odd row of toCSR2 (SDMM.cpp:371-394): columns in decreasing order
+        int rp3[] = {0, 1, 3}, ci3[] = {0, 1, 0};
+        float v3[] = {1, 2, 3};
+        if (!std::strcmp(argv[1], "descending")) rowBaseSpmm3(rp3, ci3, v3, B, C, 2, 2, 4, 1);
+        // sparse_matrix_mul3's B with column 1 twice in row 0
+        int rpb[] = {0, 2, 2}, cib[] = {1, 1};
+        if (!std::strcmp(argv[1], "repeated")) sparse_matrix_mul3(rp3, ci3, v3, rpb, cib, v, C, 2, 2, 2, 1);
         return 0;
     }
 """)
 
 
-@pytest.mark.parametrize("case,msg", [("blocking", "requires M % 8 == 0"), ("register", "requires N % 4 == 0"),
-                                      ("decreasing", "rowPtr is decreasing"),
-                                      ("range", "column index out of range")])
-def test_compat_shims_reject_bad_input(tmp_path, case, msg):
-    """The reference-name shims validate on the host and exit(EXIT_FAILURE) with a message."""
+def _shim_exe(tmp_path):
     src = tmp_path / "shim.cpp"
     src.write_text(_SHIM_MAIN)
     exe = tmp_path / "shim"
     subprocess.run(["g++", "-O1", f"-I{INCLUDE}", str(src), "-o", str(exe), LIB,
                     f"-Wl,-rpath,{os.path.dirname(LIB)}"], check=True)
-    r = subprocess.run([str(exe), case], capture_output=True, text=True, timeout=60)
+    return exe
+
+
+@pytest.mark.parametrize("case,msg", [("blocking", "requires M % 8 == 0"), ("register", "requires N % 4 == 0"),
+                                      ("decreasing", "rowPtr is decreasing"),
+                                      ("range", "column index out of range"),
+                                      ("repeated", "column index repeated within a row")])
+def test_compat_shims_reject_bad_input(tmp_path, case, msg):
+    """The reference-name shims validate on the host and exit(EXIT_FAILURE) with a message."""
+    r = subprocess.run([str(_shim_exe(tmp_path)), case], capture_output=True, text=True, timeout=60)
     assert r.returncode == 1
     assert msg in r.stderr and r.stderr.startswith("armsparse: ")
+
+
+def test_compat_shims_accept_descending_rows(tmp_path):
+    """A toCSR2 row (columns in decreasing order) passes validation, as every reference
+    kernel accepts it: without a device the call gets as far as the HIP runtime."""
+    if os.path.exists("/dev/kfd"):
+        pytest.skip("a device is present: the GPU test runs this input for real")
+    r = subprocess.run([str(_shim_exe(tmp_path)), "descending"], capture_output=True, text=True, timeout=60)
+    assert "column" not in r.stderr and "rowPtr" not in r.stderr
 
 
 def test_one_hip_runtime_whatever_the_import_order():

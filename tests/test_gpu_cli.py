@@ -23,16 +23,34 @@ def run(*args):
     (("numaAwareSDMM", "256", "256", "32"), ["Running parallelSum", "TimeTake", "sumUpTime", "Correct Result"]),
     (("numaAwareSPDM", "256", "256", "32"), ["TimeTake", "Correct Result"]),
     (("balanceNUMAAwareSPDM", "256", "256", "32"), ["TimeTake", "Correct Result"]),
-    (("SpMM", "512", "512", "512", "8"), ["M k N sparity:", "timeTake:"]),
-    (("numaAwareSpMM", "256", "256", "256", "8"), ["compute time", "sumup time", "total"]),
-    (("SpMV", "512", "512", "8"), ["m k ratio:", "time take:", "ToTCSRTime:"]),
-    (("numaAwareSpMV", "512", "512", "8"), ["TimeTake"]),
+    (("SpMM", "512", "512", "512", "8"), ["M k N sparity:", "timeTake:", "Correct Result"]),
+    (("numaAwareSpMM", "256", "256", "256", "8"), ["compute time", "sumup time", "total", "Correct Result"]),
+    (("SpMV", "512", "512", "8"), ["m k ratio:", "time take:", "ToTCSRTime:", "Correct Result"]),
+    (("numaAwareSpMV", "512", "512", "8"), ["TimeTake", "Correct Result"]),
 ])
 def test_driver(args, keys):
+    """Every driver checks its result against an FP64 product on the host: the
+    reference's |Δ| <= 0.1 and a per-row relative error of at most 1e-5."""
     r = run(*args)
     assert r.returncode == 0
     for k in keys:
         assert k in r.stdout
+    rel = [ln for ln in r.stdout.splitlines() if ln.startswith("max |C - FP64|")]
+    assert len(rel) == 1 and float(rel[0].rsplit(":", 1)[1]) <= 1e-5
+
+
+def test_numa_driver_phase_times():
+    """The NUMA lines carry measured phases: launch (time1), wait (time2) and the
+    carry fixup that stands in for sumUp."""
+    r = run("numaAwareSDMM", "2048", "2048", "64")
+    assert r.returncode == 0
+    vals = {}
+    for ln in r.stdout.splitlines():
+        if ln.startswith("time1 ") or ln.startswith("time2 "):
+            vals[ln.split()[0]] = float(ln.split()[1])
+        if ln.startswith("--memCopyTime"):
+            vals["sumUp"] = float(ln.split("sumUpTime:")[1].split()[0])
+    assert vals["time1"] > 0 and vals["time2"] >= 0 and vals["sumUp"] > 0
 
 
 def test_driver_bad_arguments():
