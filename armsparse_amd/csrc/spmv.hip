@@ -12,6 +12,7 @@
 //          four lanes per row keep the four partial sums s0..s3 and the row
 //          total is (s0+s2)+(s1+s3), as vadd/vpadd compute it.
 #include <climits>
+#include <cstdlib>
 
 #include "common.hpp"
 
@@ -193,15 +194,17 @@ __device__ __forceinline__ float unfused_product(float a, float b) {
     return p;
 }
 
-__global__ __launch_bounds__(256) void k_spmv_vcsr(int M, const int* __restrict__ rowptr,
-                                                   const int* __restrict__ colind, const float* __restrict__ val,
-                                                   const float* __restrict__ x, float* __restrict__ y, int fused,
-                                                   int long_blocks) {
+// Four lanes per row (rows of at most long_blocks blocks when long_blocks > 0),
+// 64 rows per 256-thread block `blk`.
+__device__ __forceinline__ void vcsr_short_rows(unsigned blk, int M, const int* __restrict__ rowptr,
+                                                const int* __restrict__ colind, const float* __restrict__ val,
+                                                const float* __restrict__ x, float* __restrict__ y, int fused,
+                                                int long_blocks) {
     const int l = threadIdx.x & 3;
-    const int row = (int)((blockIdx.x * 256u + threadIdx.x) >> 2);
+    const int row = (int)((blk * 256u + threadIdx.x) >> 2);
     if (row >= M) return;
     const int beg = rowptr[row], end = rowptr[row + 1];
-    if (long_blocks > 0 && end - beg > 4 * long_blocks) return;  // k_spmv_vcsr_long's row
+    if (long_blocks > 0 && end - beg > 4 * long_blocks) return;  // a listed long row
     float s = 0.0f;
     // the lane's sum runs block after block in the reference's order; blocks come
     // in batches of 16 whose loads are issued while the previous batch is added (a
@@ -255,101 +258,179 @@ __global__ __launch_bounds__(256) void k_spmv_vcsr(int M, const int* __restrict_
     if (l == 0) y[row] = __fadd_rn(y[row], tot);
 }
 
-// Rows of more than VCSR_LONG blocks: k_vcsr_find_long lists them, and a wave
-// per listed row walks it in batches of 256 blocks.  All 64 lanes load a batch
-// (16-byte loads of values and x) into LDS while lanes 0-3 add the previous
-// batch in block order — each lane l keeps the reference's partial sum s_l, so
-// the result stays bit-exact — and the global loads of the next batch are in
-// flight during those adds.
-constexpr int VCSR_LONG = 512;
-constexpr int VCSR_BATCH = 256;
+__global__ __launch_bounds__(256) void k_spmv_vcsr(int M, const int* __restrict__ rowptr,
+                                                   const int* __restrict__ colind, const float* __restrict__ val,
+                                                   const float* __restrict__ x, float* __restrict__ y, int fused) {
+    vcsr_short_rows(blockIdx.x, M, rowptr, colind, val, x, y, fused, 0);
+}
 
-__global__ void k_vcsr_find_long(int M, const int* __restrict__ rowptr, int* __restrict__ list,
+// Rows of more than `long_blocks` blocks: k_vcsr_find_long lists them and
+// workgroups of 256 lanes take listed rows from a shared counter, while the
+// other workgroups of the same launch walk the short rows (so the two overlap,
+// and the hub rows, listed and started first, are the critical path).  A
+// workgroup walks its row in batches of 256 blocks, one block per lane: each
+// lane keeps the loads of its next D batches in flight (values and x, 16 bytes
+// each; a block's column index one ring further ahead, so no x load waits for
+// its index), stages the block in LDS transposed — one array per position l of
+// the 4-block — and wave 0's lanes 0-3 keep the reference's four partial sums
+// s_l, adding each batch in block order four blocks per LDS read while the
+// next batch is staged into the other LDS buffer.  With the unfused reading
+// the lanes form the rounded products a·b, so lanes 0-3 only add; with the
+// fused one they stage values and x and lanes 0-3 run the fmaf chain.  Either
+// way each s_l sees the reference's sequence of operations: bit-exact.
+constexpr int VCSR_LONG_DEFAULT = 128;
+constexpr int VCSR_BATCH = 256;           // blocks per batch, one per lane
+constexpr int VCSR_PAD = VCSR_BATCH + 4;  // row pitch: lanes 0-3 read from distinct LDS banks
+constexpr int VCSR_AHEAD = 4;             // batches in flight per lane
+
+// Rows longer than this many blocks take the workgroup-per-row path
+// (ARMSP_VCSR_LONG_BLOCKS overrides it, read once per process).
+int vcsr_long_blocks() {
+    static const int v = [] {
+        const char* e = getenv("ARMSP_VCSR_LONG_BLOCKS");
+        const int n = e ? atoi(e) : 0;
+        return n > 0 ? n : VCSR_LONG_DEFAULT;
+    }();
+    return v;
+}
+
+__global__ void k_vcsr_find_long(int M, const int* __restrict__ rowptr, int long_blocks, int* __restrict__ list,
                                  int* __restrict__ count) {
     const int r = blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= M) return;
-    if (rowptr[r + 1] - rowptr[r] > 4 * VCSR_LONG) list[atomicAdd(count, 1)] = r;
+    if (rowptr[r + 1] - rowptr[r] > 4 * long_blocks) list[atomicAdd(count, 1)] = r;
 }
 
-__global__ void k_zero_count(int* count) { *count = 0; }
+// count[0] = listed rows, count[1] = the work counter
+__global__ void k_zero_count(int* count) {
+    count[0] = 0;
+    count[1] = 0;
+}
 
-__global__ __launch_bounds__(256) void k_spmv_vcsr_long(const int* __restrict__ list, const int* __restrict__ count,
-                                                        const int* __restrict__ rowptr,
-                                                        const int* __restrict__ colind,
-                                                        const float* __restrict__ val, const float* __restrict__ x,
-                                                        float* __restrict__ y, int fused) {
-    __shared__ float4 s_v[4][2][VCSR_BATCH];
-    __shared__ float4 s_x[4][2][VCSR_BATCH];
-    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-    const int nlong = *count;
-    const int nwaves = (int)gridDim.x * 4;
-    for (int i = uniform((int)(blockIdx.x * 4u + wid)); i < nlong; i += nwaves) {
-        const int r = uniform(list[i]);
-        const int beg = uniform(rowptr[r]);
-        const int nb = uniform((rowptr[r + 1] - beg) >> 2);
+template <bool FUSED>
+__device__ void vcsr_long_rows(const int* __restrict__ list, int* __restrict__ count,
+                               const int* __restrict__ rowptr, const int* __restrict__ colind,
+                               const float* __restrict__ val, const float* __restrict__ x, float* __restrict__ y,
+                               float (*s_a)[4][VCSR_PAD], float (*s_b)[4][VCSR_PAD], int* s_row) {
+    constexpr int D = VCSR_AHEAD;
+    const int t = threadIdx.x;
+    const int nlong = count[0];
+    while (true) {
+        if (t == 0) *s_row = atomicAdd(&count[1], 1);
+        __syncthreads();
+        const int i = *s_row;
+        __syncthreads();  // every lane has read the row before lane 0 takes the next
+        if (i >= nlong) break;  // the same for the whole workgroup once the list is drained
+        const int r = list[i];
+        const int beg = rowptr[r];
+        const int nb = (rowptr[r + 1] - beg) >> 2;
         const int nbat = (nb + VCSR_BATCH - 1) / VCSR_BATCH;
-        float4 rv[VCSR_BATCH / 64], rx[VCSR_BATCH / 64];
-        auto fetch = [&](int k) {
-#pragma unroll
-            for (int q = 0; q < VCSR_BATCH / 64; ++q) {
-                const int b = k * VCSR_BATCH + lane + 64 * q;
-                rv[q] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-                rx[q] = rv[q];
-                if (b < nb) {
-                    rv[q] = *reinterpret_cast<const float4*>(val + beg + 4 * b);
-                    rx[q] = *reinterpret_cast<const float4*>(x + colind[beg + 4 * b]);
-                }
+        // lane t owns block k·256 + t of batch k
+        auto col_of = [&](int k) {
+            const int b = k * VCSR_BATCH + t;
+            return b < nb ? colind[beg + 4 * b] : -1;
+        };
+        float4 rv[D], rx[D];
+        int rc[D];
+        auto fetch = [&](int k, int c, float4& v, float4& xv) {
+            const int b = k * VCSR_BATCH + t;
+            v = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            xv = v;
+            if (b < nb) {
+                v = *reinterpret_cast<const float4*>(val + beg + 4 * b);
+                xv = *reinterpret_cast<const float4*>(x + c);
             }
         };
+#pragma unroll
+        for (int u = 0; u < D; ++u) rc[u] = col_of(u);
+#pragma unroll
+        for (int u = 0; u < D; ++u) {
+            fetch(u, rc[u], rv[u], rx[u]);
+            rc[u] = col_of(u + D);
+        }
         float s = 0.0f;
-        fetch(0);
-        for (int k = 0; k < nbat; ++k) {
-            const int buf = k & 1;
+        for (int k0 = 0; k0 < nbat; k0 += D) {
 #pragma unroll
-            for (int q = 0; q < VCSR_BATCH / 64; ++q) {
-                s_v[wid][buf][lane + 64 * q] = rv[q];
-                s_x[wid][buf][lane + 64 * q] = rx[q];
-            }
-            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-            __builtin_amdgcn_wave_barrier();
-            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-            if (k + 1 < nbat) fetch(k + 1);  // in flight during the adds below
-            const int cnt = min(VCSR_BATCH, nb - k * VCSR_BATCH);
-            if (lane < 4) {
-                const float* pv = reinterpret_cast<const float*>(&s_v[wid][buf][0]) + lane;
-                const float* px = reinterpret_cast<const float*>(&s_x[wid][buf][0]) + lane;
-                // 16 blocks' operands read from LDS ahead of their in-order adds
-                int b = 0;
-                for (; b + 16 <= cnt; b += 16) {
-                    float a[16], c[16];
-#pragma unroll
-                    for (int u = 0; u < 16; ++u) {
-                        a[u] = pv[4 * (b + u)];
-                        c[u] = px[4 * (b + u)];
-                    }
-                    if (fused) {
-#pragma unroll
-                        for (int u = 0; u < 16; ++u) s = fmaf(a[u], c[u], s);
-                    } else {
-#pragma unroll
-                        for (int u = 0; u < 16; ++u) s += unfused_product(a[u], c[u]);
-                    }
+            for (int u = 0; u < D; ++u) {
+                const int k = k0 + u;
+                if (k >= nbat) break;  // uniform: nbat is the workgroup's
+                const int buf = k & 1;
+                if constexpr (FUSED) {
+                    s_a[buf][0][t] = rv[u].x;
+                    s_a[buf][1][t] = rv[u].y;
+                    s_a[buf][2][t] = rv[u].z;
+                    s_a[buf][3][t] = rv[u].w;
+                    s_b[buf][0][t] = rx[u].x;
+                    s_b[buf][1][t] = rx[u].y;
+                    s_b[buf][2][t] = rx[u].z;
+                    s_b[buf][3][t] = rx[u].w;
+                } else {  // vmlaq_f32 rounds the product on its own
+                    s_a[buf][0][t] = unfused_product(rv[u].x, rx[u].x);
+                    s_a[buf][1][t] = unfused_product(rv[u].y, rx[u].y);
+                    s_a[buf][2][t] = unfused_product(rv[u].z, rx[u].z);
+                    s_a[buf][3][t] = unfused_product(rv[u].w, rx[u].w);
                 }
-                if (fused) {
-                    for (; b < cnt; ++b) s = fmaf(pv[4 * b], px[4 * b], s);
-                } else {
-                    for (; b < cnt; ++b) s += unfused_product(pv[4 * b], px[4 * b]);
+                fetch(k + D, rc[u], rv[u], rx[u]);  // in flight for D batches
+                rc[u] = col_of(k + 2 * D);
+                // batch k staged; wave 0 has also finished adding batch k-1, whose
+                // buffer batch k+1 is about to overwrite
+                __syncthreads();
+                if (t < 4) {
+                    const int cnt = min(VCSR_BATCH, nb - k * VCSR_BATCH);
+                    const float* pa = &s_a[buf][t][0];
+                    const float* pb = &s_b[buf][FUSED ? t : 0][0];
+                    int b = 0;
+                    for (; b + 4 <= cnt; b += 4) {
+                        const float4 a4 = *reinterpret_cast<const float4*>(pa + b);
+                        if constexpr (FUSED) {
+                            const float4 x4 = *reinterpret_cast<const float4*>(pb + b);
+                            s = fmaf(a4.x, x4.x, s);
+                            s = fmaf(a4.y, x4.y, s);
+                            s = fmaf(a4.z, x4.z, s);
+                            s = fmaf(a4.w, x4.w, s);
+                        } else {
+                            s = __fadd_rn(s, a4.x);
+                            s = __fadd_rn(s, a4.y);
+                            s = __fadd_rn(s, a4.z);
+                            s = __fadd_rn(s, a4.w);
+                        }
+                    }
+                    for (; b < cnt; ++b) {
+                        if constexpr (FUSED)
+                            s = fmaf(pa[b], pb[b], s);
+                        else
+                            s = __fadd_rn(s, pa[b]);
+                    }
                 }
             }
         }
-        const float t = __fadd_rn(s, __shfl_xor(s, 2, 4));   // lanes 0/2: s0+s2, lanes 1/3: s1+s3
-        const float tot = __fadd_rn(t, __shfl_xor(t, 1, 4));  // (s0+s2)+(s1+s3)
-        if (lane == 0) y[r] = __fadd_rn(y[r], tot);
-        // the next row's first batch reuses this row's LDS: every lane is past the adds
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        if (t < 64) {  // wave 0: lanes 0-3 hold s0..s3
+            const float tt = __fadd_rn(s, __shfl_xor(s, 2, 4));  // lanes 0/2: s0+s2, lanes 1/3: s1+s3
+            const float tot = __fadd_rn(tt, __shfl_xor(tt, 1, 4));  // (s0+s2)+(s1+s3)
+            if (t == 0) y[r] = __fadd_rn(y[r], tot);
+        }
+        __syncthreads();  // the LDS buffers are free for the next row
     }
+}
+
+// One launch: workgroups [0, n_long_wg) walk the listed long rows, the rest the
+// short rows (64 per workgroup).
+template <bool FUSED>
+__global__ __launch_bounds__(256) void k_spmv_vcsr_split(int M, const int* __restrict__ rowptr,
+                                                         const int* __restrict__ colind,
+                                                         const float* __restrict__ val, const float* __restrict__ x,
+                                                         float* __restrict__ y, int long_blocks,
+                                                         const int* __restrict__ list, int* __restrict__ count,
+                                                         int n_long_wg) {
+    __shared__ __align__(16) float s_a[2][4][VCSR_PAD];
+    __shared__ __align__(16) float s_b[FUSED ? 2 : 1][4][FUSED ? VCSR_PAD : 4];
+    __shared__ int s_row;
+    if ((int)blockIdx.x < n_long_wg) {
+        vcsr_long_rows<FUSED>(list, count, rowptr, colind, val, x, y, s_a,
+                              reinterpret_cast<float (*)[4][VCSR_PAD]>(&s_b[0][0][0]), &s_row);
+        return;
+    }
+    vcsr_short_rows(blockIdx.x - (unsigned)n_long_wg, M, rowptr, colind, val, x, y, FUSED ? 1 : 0, long_blocks);
 }
 
 template <int IPT>
@@ -468,9 +549,12 @@ int armsp_spmv_vcsr_f32(int M, int nnz_padded, const int* rowptr, const int* col
         return launch_merge<8>(M, nnz_padded, rowptr, colind, val, x, y, ARMSP_ACCUMULATE, s);
     }
     const int fused = (flags & ARMSP_FUSED_MLA) ? 1 : 0;
-    // rows of more than VCSR_LONG blocks (possible only above that many blocks in
-    // all) go to the wave-per-row kernel
-    const bool split_long = nnz_padded / 4 > VCSR_LONG;
+    // rows of more than long_blocks blocks (possible only above that many blocks in all)
+    // go to the wave-per-row kernel; its 16-byte loads need 16-byte-aligned values and x
+    // (VCSR rows hold whole 4-blocks and block columns are multiples of 4, so rowptr
+    // entries and the gathered x offsets are multiples of 4 as well)
+    const int long_blocks = vcsr_long_blocks();
+    const bool split_long = nnz_padded / 4 > long_blocks && ((uintptr_t)val % 16) == 0 && ((uintptr_t)x % 16) == 0;
     void* ws = nullptr;
     int* list = nullptr;
     int* count = nullptr;
@@ -480,15 +564,23 @@ int armsp_spmv_vcsr_f32(int M, int nnz_padded, const int* rowptr, const int* col
         count = static_cast<int*>(ws);
         list = count + 64;
         hipLaunchKernelGGL(k_zero_count, dim3(1), dim3(1), 0, s, count);
-        hipLaunchKernelGGL(k_vcsr_find_long, dim3((unsigned)((M + 255) / 256)), dim3(256), 0, s, M, rowptr, list,
-                           count);
+        hipLaunchKernelGGL(k_vcsr_find_long, dim3((unsigned)((M + 255) / 256)), dim3(256), 0, s, M, rowptr,
+                           long_blocks, list, count);
     }
     TimedLaunch t("spmv_vcsr", s);
-    hipLaunchKernelGGL(k_spmv_vcsr, dim3((unsigned)(((int64_t)M * 4 + 255) / 256)), dim3(256), 0, s, M, rowptr,
-                       colind, val, x, y, fused, split_long ? VCSR_LONG : 0);
-    if (split_long)
-        hipLaunchKernelGGL(k_spmv_vcsr_long, dim3((unsigned)(2 * device_cus())), dim3(256), 0, s, list, count, rowptr,
-                           colind, val, x, y, fused);
+    const unsigned short_wg = (unsigned)(((int64_t)M * 4 + 255) / 256);
+    if (split_long) {  // long-row workgroups first: the hub rows are the critical path
+        const int n_long_wg = 2 * device_cus();
+        const dim3 grid((unsigned)n_long_wg + short_wg);
+        if (fused)
+            hipLaunchKernelGGL(k_spmv_vcsr_split<true>, grid, dim3(256), 0, s, M, rowptr, colind, val, x, y,
+                               long_blocks, list, count, n_long_wg);
+        else
+            hipLaunchKernelGGL(k_spmv_vcsr_split<false>, grid, dim3(256), 0, s, M, rowptr, colind, val, x, y,
+                               long_blocks, list, count, n_long_wg);
+    } else {
+        hipLaunchKernelGGL(k_spmv_vcsr, dim3(short_wg), dim3(256), 0, s, M, rowptr, colind, val, x, y, fused);
+    }
     const hipError_t e = hipGetLastError();
     if (ws) ws_free(ws, s);
     if (e != hipSuccess) return hip_fail(e);

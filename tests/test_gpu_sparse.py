@@ -73,12 +73,51 @@ def test_vcsr_bitexact_skewed_rows(fused):
     case(f"vcsr skewed fused={fused}")
     a = data.rmat_csr(14, 1 << 18, seed=11)
     v = data.csr_to_vcsr(a)
-    assert np.diff(v.rowptr).max() > 4 * 32
+    assert np.diff(v.rowptr).max() > 4 * 512  # rows for the wave-per-row kernel
     x = data.host_uniform(v.shape[1], 5, 1)
     y0 = np.random.default_rng(4).uniform(-1, 1, v.shape[0]).astype(np.float32)
     d_rp, d_ci, d_va, d_x, d_y = dev(v.rowptr, v.colind, v.values, x, y0)
     kernels.vcsr_spmv(d_rp, d_ci, d_va, d_x, d_y, fused=fused)
     expect = oracle.sparse_dense_vector(x, y0.copy(), v.rowptr, v.colind, v.values, fused=fused)
+    assert np.array_equal(d_y.cpu().numpy().view(np.uint32), expect.view(np.uint32))
+
+
+@pytest.mark.parametrize("fused", [False, True])
+def test_vcsr_bitexact_more_long_rows_than_waves(fused):
+    """More rows of over 512 blocks than the long-row kernel has waves (2 blocks of 4
+    waves per CU): waves take several rows from the shared counter and restage their
+    LDS for each; still bit-exact.  Half the rows are empty (short-row kernel)."""
+    case(f"vcsr many long rows fused={fused}")
+    M, K = 4400, 16384
+    rp, ci, va = random_csr(M, K, 0.042, 21, empty_rows=0.5)
+    v = data.csr_to_vcsr(data.CSR(rp, ci, va, (M, K)))
+    nb = np.diff(v.rowptr) // 4
+    cus = torch.cuda.get_device_properties(0).multi_processor_count
+    assert (nb > 512).sum() > 2 * cus * 4 and (nb == 0).sum() > 0
+    x = data.host_uniform(K, 6, 1)
+    y0 = np.random.default_rng(5).uniform(-1, 1, M).astype(np.float32)
+    d_rp, d_ci, d_va, d_x, d_y = dev(v.rowptr, v.colind, v.values, x, y0)
+    kernels.vcsr_spmv(d_rp, d_ci, d_va, d_x, d_y, fused=fused)
+    expect = oracle.sparse_dense_vector(x, y0.copy(), v.rowptr, v.colind, v.values, fused=fused)
+    assert np.array_equal(d_y.cpu().numpy().view(np.uint32), expect.view(np.uint32))
+
+
+def test_vcsr_unaligned_operands_bitexact():
+    """Values or x off 16-byte alignment (a sliced view) take the 4-lane kernel for
+    every row: same bits as the aligned call."""
+    case("vcsr unaligned")
+    a = data.rmat_csr(14, 1 << 18, seed=11)
+    v = data.csr_to_vcsr(a)
+    x = data.host_uniform(v.shape[1], 5, 1)
+    y0 = np.random.default_rng(4).uniform(-1, 1, v.shape[0]).astype(np.float32)
+    d_rp, d_ci = dev(v.rowptr, v.colind)
+    vbuf = torch.zeros(v.nnz + 1, device="cuda")
+    vbuf[1:] = torch.from_numpy(v.values).cuda()
+    xbuf = torch.zeros(x.size + 1, device="cuda")
+    xbuf[1:] = torch.from_numpy(x).cuda()
+    d_y = torch.from_numpy(y0.copy()).cuda()
+    kernels.vcsr_spmv(d_rp, d_ci, vbuf[1:], xbuf[1:], d_y)
+    expect = oracle.sparse_dense_vector(x, y0.copy(), v.rowptr, v.colind, v.values)
     assert np.array_equal(d_y.cpu().numpy().view(np.uint32), expect.view(np.uint32))
 
 
