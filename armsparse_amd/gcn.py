@@ -86,8 +86,11 @@ def line_pitched(x: torch.Tensor) -> torch.Tensor:
 def make_task(precision: str = "fp32", seed: int = data.CFG_SEED + 3, n: int = 19_717, pairs: int = 44_324,
               features: int = 500, hidden: int = 64, classes: int = 3, feat_density: float = 0.01,
               device: str = "cuda", dropout: float = 0.5, capturable: bool = False,
-              order: str = "transform_first") -> GCNTask:
-    g = data.gcn_graph(n, pairs, seed)
+              order: str = "transform_first", graph: "data.GCNGraph" = None) -> GCNTask:
+    """Synthetic features, labels and training nodes on the seeded power-law graph of
+    n nodes and `pairs` undirected edges (or on `graph`, e.g. a Matrix Market file's)."""
+    g = graph if graph is not None else data.gcn_graph(n, pairs, seed)
+    n = g.n
     r = np.random.default_rng(seed)
     x = np.zeros((n, features), np.float32)
     for lo in range(0, n, 1 << 18):  # in row blocks to bound host temporaries
