@@ -28,6 +28,112 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+ROOT = os.path.dirname(os.path.abspath(__file__))
+# Sources of the headline SpMM kernel: a committed counter summary is used only while
+# they are unchanged (its stamp is their hash).
+SPMM_SOURCES = ["armsparse_amd/csrc/spmm.hip", "armsparse_amd/csrc/partition.hip", "armsparse_amd/csrc/common.hpp",
+                "include/armsp.h"]
+GEMM_SOURCES = ["armsparse_amd/csrc/gemm.hip", "armsparse_amd/csrc/common.hpp", "include/armsp.h"]
+
+
+def source_stamp(paths):
+    import hashlib
+    h = hashlib.sha256()
+    for p in paths:
+        with open(os.path.join(ROOT, p), "rb") as f:
+            h.update(f.read())
+    return h.hexdigest()[:16]
+
+
+def stamped_profile(name, sources):
+    """A committed counter summary (profiles/<name>) if its stamp matches the current
+    sources, else None (stale figures are not reported)."""
+    path = os.path.join(ROOT, "profiles", name)
+    if not os.path.exists(path):
+        return None
+    rec = json.load(open(path))
+    return rec if rec.get("source_stamp") == source_stamp(sources) else None
+
+
+def sample_rows(rowptr, n_random=4096, n_longest=64, seed=1):
+    """Sorted row ids: the longest rows and seeded random ones (FP64 checks on a sample)."""
+    import numpy as np
+    M = len(rowptr) - 1
+    longest = np.argsort(np.diff(rowptr))[-min(n_longest, M):]
+    rnd = np.random.default_rng(seed).choice(M, min(n_random, M), replace=False)
+    return np.unique(np.concatenate([longest, rnd]))
+
+
+def sub_csr(rowptr, colind, val, rows):
+    import numpy as np
+    lens = (rowptr[rows + 1] - rowptr[rows]).astype(np.int64)
+    rp = np.zeros(len(rows) + 1, np.int32)
+    rp[1:] = np.cumsum(lens)
+    idx = np.concatenate([np.arange(rowptr[r], rowptr[r + 1]) for r in rows])
+    return rp, colind[idx], val[idx]
+
+
+def spmm_rows_err(rowptr, colind, val, Bh, got_rows, rows):
+    """Max norm-wise relative error per row of got_rows (= C[rows]) against the FP64
+    product of those rows of A with Bh (scipy in float64; independent of the kernels)."""
+    import numpy as np
+    import scipy.sparse as sp
+    srp, sci, sva = sub_csr(rowptr, colind, val, rows)
+    uc, inv = np.unique(sci, return_inverse=True)
+    sub = sp.csr_matrix((sva.astype(np.float64), inv.astype(np.int64), srp.astype(np.int64)), shape=(len(rows), len(uc)))
+    ref = sub @ Bh[uc].astype(np.float64)
+    num = np.linalg.norm(got_rows.astype(np.float64) - ref, axis=1)
+    den = np.linalg.norm(ref, axis=1)
+    return float(np.max(np.where(den > 0, num / np.where(den > 0, den, 1), num)))
+
+
+def spmv_rows_err(rowptr, colind, val, xh, got_rows, rows):
+    """Max |y - y64| / Σ|a·x| over the sampled rows (the dot product's own scale)."""
+    import numpy as np
+    import scipy.sparse as sp
+    srp, sci, sva = sub_csr(rowptr, colind, val, rows)
+    sub = sp.csr_matrix((sva.astype(np.float64), sci.astype(np.int64), srp.astype(np.int64)), shape=(len(rows), len(xh)))
+    ref = sub @ xh.astype(np.float64)
+    scale = abs(sub) @ np.abs(xh.astype(np.float64))
+    return float(np.max(np.abs(got_rows - ref) / np.maximum(scale, 1e-300)))
+
+
+def sddmm_rows_err(rowptr, colind, val, Uh, Vh, got, rows):
+    """Max |out - out64| / Σ|a·u·v| over the nonzeros of the sampled rows."""
+    import numpy as np
+    worst = 0.0
+    for r in rows:
+        lo, hi = int(rowptr[r]), int(rowptr[r + 1])
+        if lo == hi:
+            continue
+        terms = val[lo:hi, None].astype(np.float64) * Uh[r].astype(np.float64) * Vh[colind[lo:hi]].astype(np.float64)
+        ref, scale = terms.sum(1), np.abs(terms).sum(1)
+        worst = max(worst, float(np.max(np.abs(got[lo:hi] - ref) / np.maximum(scale, 1e-300))))
+    return worst
+
+
+def host_info():
+    """Cores this process may use and what the host has (physical cores from sysfs)."""
+    cores = set()
+    base = "/sys/devices/system/cpu"
+    for d in os.listdir(base):
+        if d.startswith("cpu") and d[3:].isdigit():
+            try:
+                pkg = open(f"{base}/{d}/topology/physical_package_id").read().strip()
+                core = open(f"{base}/{d}/topology/core_id").read().strip()
+                cores.add((pkg, core))
+            except OSError:
+                pass
+    quota = None
+    try:
+        q, per = open("/sys/fs/cgroup/cpu.max").read().split()
+        quota = None if q == "max" else round(int(q) / int(per), 2)
+    except (OSError, ValueError):
+        pass
+    return {"physical_cores": len(cores) or None, "logical_cpus": os.cpu_count(),
+            "affinity_cpus": len(os.sched_getaffinity(0)), "cgroup_cpu_limit": quota}
+
+
 def parse():
     p = argparse.ArgumentParser()
     p.add_argument("--gpus", type=int, default=1)
@@ -44,12 +150,140 @@ def parse():
     return p.parse_args()
 
 
+def cpu_baselines(args, A, B, headline_bytes):
+    """The oracle's restatements of the reference CPU kernels (OpenMP, C) timed on this
+    host: rowBaseSpmm3 on the headline workload (the reported value), plus
+    sparse_dense_vector on the same matrix's blocked CSR, sparse_matrix_mul3 on the
+    `./SpMM 4096 4096 4096` case and a PubMed-size GCN training epoch through the
+    restated CPU op.  Threads are bound spread over cores; every call is timed, and
+    the mean and median reported."""
+    import numpy as np
+
+    import oracle
+    from armsparse_amd import data
+
+    threads = int(os.environ.get("OMP_NUM_THREADS", len(os.sched_getaffinity(0))))
+    budget = args.cpu_seconds
+
+    def calls(fn, min_calls=3, max_calls=30, seconds=budget):
+        fn()  # warm-up
+        ts = []
+        t0 = time.perf_counter()
+        while len(ts) < max_calls and (len(ts) < min_calls or time.perf_counter() - t0 < seconds):
+            t = time.perf_counter()
+            fn()
+            ts.append(time.perf_counter() - t)
+        return ts
+
+    def stats(ts, nbytes=None):
+        import statistics
+        d = {"calls": len(ts), "mean_ms": round(statistics.fmean(ts) * 1e3, 2),
+             "median_ms": round(statistics.median(ts) * 1e3, 2)}
+        if nbytes:
+            d["GBps_mean"] = round(nbytes / statistics.fmean(ts) / 1e9, 2)
+        return d
+
+    M, N = A.shape[0], B.shape[1]
+    Bh = B.cpu().numpy()
+    Ch = np.zeros((M, N), np.float32)
+    ts = calls(lambda: oracle.rowBaseSpmm3(A.rowptr, A.colind, A.values, Bh, Ch, threads=threads))
+    head = stats(ts, headline_bytes)
+    others = {}
+    # sparse_dense_vector: y += A·x on the blocked CSR of the same matrix
+    v = data.csr_to_vcsr(A)
+    x = data.host_uniform(v.shape[1], data.CFG_SEED + 1, data.STREAM_DENSE + 1)
+    y = np.zeros(M, np.float32)
+    ts = calls(lambda: oracle.sparse_dense_vector(x, y, v.rowptr, v.colind, v.values, threads=threads),
+               seconds=budget / 4)
+    others["sparse_dense_vector_rmat1m"] = stats(ts, 4 * (M + 1) + 8 * v.nnz + 4 * v.shape[1] + 8 * M)
+    del v
+    # sparse_matrix_mul3: the literal ./SpMM 4096 4096 4096 product
+    As = data.uniform_csr(4096, 4096, 167_772, data.CFG_SEED + 0)
+    Bs = data.uniform_csr(4096, 4096, 167_772, data.CFG_SEED + 5)
+    Cs = np.zeros((4096, 4096), np.float32)
+    a3, b3 = (As.rowptr, As.colind, As.values), (Bs.rowptr, Bs.colind, Bs.values)
+    ts = calls(lambda: oracle.sparse_matrix_mul3(a3, b3, Cs, threads=threads), seconds=budget / 4)
+    others["sparse_matrix_mul3_4096"] = stats(ts, 2 * (4 * 4097 + 8 * 167_772) + 8 * 4096 * 4096)
+    others["gcn_pubmed_epoch"] = stats(_cpu_gcn_epochs(oracle, threads, budget / 4))
+    return {"value": head["GBps_mean"], "unit": "GB/s", "cores": threads, "kind": "port",
+            "sample": f"full 1M R-MAT N=128, rowBaseSpmm3 (C += A·B, accumulate bytes), 1 warm-up + "
+                      f"{head['calls']} timed calls, mean {head['mean_ms']} ms, median {head['median_ms']} ms",
+            "median_GBps": round(headline_bytes / (head["median_ms"] / 1e3) / 1e9, 2),
+            "omp": {k: os.environ.get(k) for k in ("OMP_NUM_THREADS", "OMP_PROC_BIND", "OMP_PLACES")},
+            "host": host_info(), "others": others}
+
+
+def _cpu_gcn_epochs(oracle, threads, seconds):
+    """Times of PubMed-size training epochs on the CPU: the reference's GCNConv maths
+    (gcn/pytorch-custom/op.py:102-124) with torch's CPU ops and the restated CPU op
+    (csr_spmm, spmm.cpp:31-71) for the aggregation and its transpose in backward."""
+    import numpy as np
+    import torch
+    import torch.nn.functional as F
+
+    from armsparse_amd import data
+
+    g = data.gcn_graph()
+    n = g.n
+    cp, ri, vt = oracle.csr_to_csc(g.rowptr, g.colind, g.value_csr, n)
+
+    class Agg(torch.autograd.Function):
+        @staticmethod
+        def forward(ctx, x):
+            return torch.from_numpy(oracle.csr_spmm(g.rowptr, g.colind, g.value_csr, x.detach().numpy(), threads))
+
+        @staticmethod
+        def backward(ctx, gy):
+            return torch.from_numpy(oracle.csr_spmm(cp, ri, vt, gy.contiguous().numpy(), threads))
+
+    r = np.random.default_rng(0)
+    xd = np.where(r.random((n, 500)) < 0.01, r.random((n, 500)), 0.0).astype(np.float32)
+    xd /= np.maximum(xd.sum(1, keepdims=True), 1e-30)
+    x = torch.from_numpy(xd)
+    y = torch.from_numpy(r.integers(0, 3, n))
+    train = torch.from_numpy(r.choice(n, 60, replace=False))
+    deg = torch.from_numpy(np.diff(g.rowptr).astype(np.float32))
+    norm = (1 / torch.sqrt(deg)).unsqueeze(1)
+    torch.manual_seed(0)
+
+    def glorot(a, b):
+        lim = (6.0 / (a + b)) ** 0.5
+        return torch.empty(a, b).uniform_(-lim, lim).requires_grad_(True)
+
+    W1, W2 = glorot(500, 64), glorot(64, 3)
+    b1, b2 = torch.zeros(64, requires_grad=True), torch.zeros(3, requires_grad=True)
+    opt = torch.optim.Adam([dict(params=[W1, b1], weight_decay=5e-4), dict(params=[W2, b2], weight_decay=0)], lr=0.01)
+    torch.set_num_threads(threads)
+
+    def epoch():
+        opt.zero_grad()
+        h = F.relu(Agg.apply((x @ W1) * norm) * norm + b1)
+        h = F.dropout(h, 0.5, training=True)
+        out = F.log_softmax(Agg.apply((h @ W2) * norm) * norm + b2, dim=1)
+        F.nll_loss(out[train], y[train]).backward()
+        opt.step()
+
+    for _ in range(3):
+        epoch()
+    ts = []
+    t0 = time.perf_counter()
+    while len(ts) < 50 and (len(ts) < 5 or time.perf_counter() - t0 < seconds):
+        t = time.perf_counter()
+        epoch()
+        ts.append(time.perf_counter() - t)
+    return ts
+
+
 def main():
     args = parse()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
         print(json.dumps({"error": "this build measures one GPU only; run with --gpus 1 (no multi-GPU path)"}))
         sys.exit(2)
+    # the CPU baseline's OpenMP threads spread over the cores (read when an OpenMP
+    # runtime starts, so set before anything loads one)
+    os.environ.setdefault("OMP_PROC_BIND", "spread")
+    os.environ.setdefault("OMP_PLACES", "cores")
     import numpy as np
     import torch
 
@@ -117,13 +351,23 @@ def main():
     log(f"[bench] spmm fp32: {value:.1f} GB/s end-to-end, main kernel {k_avg * 1e6:.1f} us "
         f"({achieved:.1f} GB/s), fixup {f_ms / max(f_n, 1) * 1e3:.1f} us")
     traffic = None
-    tpath = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "spmm_rmat1m_n128_traffic.json")
-    if os.path.exists(tpath):
-        traffic = json.load(open(tpath)).get("hbm_bytes_per_launch")
+    trec = stamped_profile("spmm_rmat1m_n128_traffic.json", SPMM_SOURCES) if cfg is None else None
+    if trec is not None:
+        traffic = trec.get("hbm_bytes_per_launch")
+    else:
+        log("[bench] no counter summary for the current SpMM sources: roofline.traffic is null")
     extras = {"spmm_main_kernel_us": round(k_avg * 1e6, 2),
               "spmm_fixup_kernel_us": round(f_ms / max(f_n, 1) * 1e3, 2),
               "gflops": round(2.0 * nnz * N * args.steps / el / 1e9, 1)}
     steps_x = max(10, args.steps // 2)
+    # error of every timed op against FP64 (scipy/numpy in float64) on a sample of rows:
+    # 4,096 seeded random rows plus the 16 longest
+    rows = sample_rows(A.rowptr, 4096, 16)
+    rows_t = torch.from_numpy(rows).to(dev).long()
+    errs = {}
+    Bh = B.cpu().numpy()
+    errs["spmm_fp32_n128"] = spmm_rows_err(A.rowptr, A.colind, A.values, Bh, C[rows_t].cpu().numpy(), rows)
+    log(f"[bench] spmm fp32 max row rel err vs FP64 (sampled rows): {errs['spmm_fp32_n128']:.2e}")
 
     if "mixed" in only:
         Bb = kernels.to_bf16(B)
@@ -135,6 +379,17 @@ def main():
         extras["spmm_fp8_GBps"] = round(spmm_bytes(1, scales=True) * steps_x / el_q / 1e9, 1)
         extras["spmm_bf16_us"] = round(el_b / steps_x * 1e6, 1)
         extras["spmm_fp8_us"] = round(el_q / steps_x * 1e6, 1)
+        kernels.spmm(rp, ci, va, Bb, out=C, config=cfg)
+        errs["spmm_bf16_n128_vs_dequantised"] = spmm_rows_err(A.rowptr, A.colind, A.values, Bb.float().cpu().numpy(),
+                                                              C[rows_t].cpu().numpy(), rows)
+        errs["spmm_bf16_n128_vs_fp32_B"] = spmm_rows_err(A.rowptr, A.colind, A.values, Bh, C[rows_t].cpu().numpy(),
+                                                         rows)
+        kernels.spmm(rp, ci, va, q, dense_scale=sc, out=C, config=cfg)
+        errs["spmm_fp8_n128_vs_dequantised"] = spmm_rows_err(A.rowptr, A.colind, A.values,
+                                                             kernels.dequantize_fp8_rows(q, sc).cpu().numpy(),
+                                                             C[rows_t].cpu().numpy(), rows)
+        errs["spmm_fp8_n128_vs_fp32_B"] = spmm_rows_err(A.rowptr, A.colind, A.values, Bh, C[rows_t].cpu().numpy(),
+                                                        rows)
         del Bb, q, sc
     if "regimes" in only:  # not in the default set: what bounds the gathers (see the counter summaries)
         rng = np.random.default_rng(0)
@@ -161,6 +416,9 @@ def main():
                 byt = 4 * (M + 1) + 8 * nnz + sb * K * n2 + 4 * M * n2
                 extras[f"spmm_n{n2}_{nm}_us"] = round(el_w / 10 * 1e6, 1)
                 extras[f"spmm_n{n2}_{nm}_GBps"] = round(byt * 10 / el_w / 1e9, 1)
+                ref_b = (BB.float() if nm == "bf16" else BB).cpu().numpy()
+                errs[f"spmm_{nm}_n{n2}"] = spmm_rows_err(A.rowptr, A.colind, A.values, ref_b,
+                                                         C2[rows_t].cpu().numpy(), rows)
             del B2, C2
     if "shards" in only:  # one GPU's share of an 8-way row split: heaviest and median 131,072-row blocks
         rows = M // 8
@@ -187,6 +445,9 @@ def main():
         el_b, _ = timed(call_b, 200, 10, [])
         extras["cfg0_spmm_us"] = round(el_a / 200 * 1e6, 2)  # per Python call (host-bound at this size)
         extras["cfg0_spmv_us"] = round(el_b / 200 * 1e6, 2)
+        all_s = np.arange(4096)
+        errs["cfg0_spmm_n64"] = spmm_rows_err(As.rowptr, As.colind, As.values, Bs, sC.cpu().numpy(), all_s)
+        errs["cfg0_spmv"] = spmv_rows_err(As.rowptr, As.colind, As.values, xs, sy.cpu().numpy(), all_s)
         # the same call captured once in a HIP graph and replayed: the device-side cost
         kernels.workspace_reserve(0)
         for nm, fn in (("spmm", call_a), ("spmv", call_b)):
@@ -210,6 +471,14 @@ def main():
         gbytes = 2 * (4 * 4097 + 8 * 167_772) + 8 * 4096 * 4096
         extras["spgemm_4096_kernel_us"] = round(kg * 1e6, 1)
         extras["spgemm_4096_GBps"] = round(gbytes / kg / 1e9, 1)
+        import scipy.sparse as sps
+        gcalls = 5 + 50  # warm-up and timed calls all accumulate into gc: C = 55·A·B
+        a64 = sps.csr_matrix((As.values.astype(np.float64), As.colind, As.rowptr), shape=(4096, 4096))
+        b64 = sps.csr_matrix((Bs2.values.astype(np.float64), Bs2.colind, Bs2.rowptr), shape=(4096, 4096))
+        ref_g = (a64 @ b64).toarray() * gcalls
+        got_g = gc.cpu().numpy().astype(np.float64)
+        den = np.linalg.norm(ref_g, axis=1)
+        errs["spgemm_4096"] = float(np.max(np.linalg.norm(got_g - ref_g, axis=1) / np.where(den > 0, den, 1)))
         del gc
         A6 = data.uniform_65k()
         r6, c6, v6 = (torch.from_numpy(t).to(dev) for t in (A6.rowptr, A6.colind, A6.values))
@@ -218,11 +487,17 @@ def main():
         y6 = torch.empty(65536, device=dev)
         el_c, _ = timed(lambda: kernels.spmv(r6, c6, v6, x6, out=y6), 100, 5, [])
         extras["spmv_65k_us"] = round(el_c / 100 * 1e6, 2)
+        rows6 = sample_rows(A6.rowptr, 4096, 16)
+        errs["spmv_65k"] = spmv_rows_err(A6.rowptr, A6.colind, A6.values, x6.cpu().numpy(),
+                                         y6.cpu().numpy()[rows6], rows6)
         for kd in (32, 128):
             U6 = torch.empty((65536, kd), device=dev)
             kernels.fill_normal_(U6, data.CFG_SEED + 1, data.STREAM_DENSE + 2)
             el_d, _ = timed(lambda: kernels.sddmm(r6, c6, U6, U6, scale=v6), 50, 5, [])
             extras[f"sddmm_65k_kd{kd}_us"] = round(el_d / 50 * 1e6, 2)
+            o6 = kernels.sddmm(r6, c6, U6, U6, scale=v6).cpu().numpy()
+            U6h = U6.cpu().numpy()
+            errs[f"sddmm_65k_kd{kd}"] = sddmm_rows_err(A6.rowptr, A6.colind, A6.values, U6h, U6h, o6, rows6)
     if "spmv" in only:
         x = torch.empty(K, device=dev)
         kernels.fill_uniform_(x, data.CFG_SEED + 1, data.STREAM_DENSE + 1)
@@ -231,12 +506,18 @@ def main():
         vb = 4 * (M + 1) + 8 * nnz + 4 * K + 4 * M
         extras["spmv_GBps"] = round(vb * steps_x * 4 / el_v / 1e9, 1)
         extras["spmv_us"] = round(el_v / (steps_x * 4) * 1e6, 1)
+        xh = x.cpu().numpy()
+        errs["spmv"] = spmv_rows_err(A.rowptr, A.colind, A.values, xh, y.cpu().numpy()[rows], rows)
         extras["spmv_kernel_us"] = round(fam_v["spmv"][0] / max(fam_v["spmv"][1], 1) * 1e3, 1)
         # the reference's vector-aware (zero-padded 1×4 blocks) CSR SpMV, y += A·x
         Av = data.csr_to_vcsr(A)
         rv, cv, vv = (torch.from_numpy(t).to(dev) for t in (Av.rowptr, Av.colind, Av.values))
         xv = torch.zeros(Av.shape[1], device=dev)
         xv[:K] = x
+        y.zero_()
+        kernels.vcsr_spmv(rv, cv, vv, xv, y)
+        errs["vcsr_spmv"] = spmv_rows_err(Av.rowptr, Av.colind, Av.values, xv.cpu().numpy(), y.cpu().numpy()[rows],
+                                          rows)
         el_w, fam_w = timed(lambda: kernels.vcsr_spmv(rv, cv, vv, xv, y), steps_x * 2, 5, ["spmv_vcsr"])
         pv = int(cv.numel())
         wb = 4 * (M + 1) + 8 * pv + 4 * K + 8 * M
@@ -256,6 +537,9 @@ def main():
             kernels.fill_normal_(V, data.CFG_SEED + 1, data.STREAM_DENSE + 3)
             el_s, fam_s = timed(lambda: kernels.sddmm(rp, ci, U, V, scale=va), steps_x, 3, ["sddmm"])
             sb = 4 * (M + 1) + 12 * nnz + 2 * 4 * M * kd
+            Uh, Vh = U.cpu().numpy(), V.cpu().numpy()
+            errs[f"sddmm_kd{kd}"] = sddmm_rows_err(A.rowptr, A.colind, A.values, Uh, Vh,
+                                                   kernels.sddmm(rp, ci, U, V, scale=va).cpu().numpy(), rows)
             extras[f"sddmm_kd{kd}_GBps"] = round(sb * steps_x / el_s / 1e9, 1)
             extras[f"sddmm_kd{kd}_us"] = round(el_s / steps_x * 1e6, 1)
             # bf16 operands (the GCN's dA: features in bf16), fp32 accumulation
@@ -264,12 +548,18 @@ def main():
             sbb = 4 * (M + 1) + 12 * nnz + 2 * 2 * M * kd
             extras[f"sddmm_kd{kd}_bf16_GBps"] = round(sbb * steps_x / el_b / 1e9, 1)
             extras[f"sddmm_kd{kd}_bf16_us"] = round(el_b / steps_x * 1e6, 1)
+            errs[f"sddmm_kd{kd}_bf16"] = sddmm_rows_err(A.rowptr, A.colind, A.values, Ub.float().cpu().numpy(),
+                                                        Vb.float().cpu().numpy(),
+                                                        kernels.sddmm(rp, ci, Ub, Vb, scale=va).cpu().numpy(), rows)
             del Ub, Vb
             if kd == 128:  # dot products split into column halves over the XCDs (a selector candidate)
                 split = _lib.Config.make("merge", unroll=2)
                 el_s, _ = timed(lambda: kernels.sddmm(rp, ci, U, V, scale=va, config=split), steps_x, 3, [])
                 extras["sddmm_kd128_split_GBps"] = round(sb * steps_x / el_s / 1e9, 1)
                 extras["sddmm_kd128_split_us"] = round(el_s / steps_x * 1e6, 1)
+                errs["sddmm_kd128_split"] = sddmm_rows_err(
+                    A.rowptr, A.colind, A.values, Uh, Vh, kernels.sddmm(rp, ci, U, V, scale=va, config=split).cpu().numpy(),
+                    rows)
             del U, V
     if "gcn" in only:
         from armsparse_amd import gcn
@@ -281,13 +571,17 @@ def main():
             extras[f"gcn_{prec}_ms_per_epoch"] = round(el_g / 30 * 1e3, 3)
             del task
         extras["gcn_bf16_speedup"] = round(extras["gcn_fp32_ms_per_epoch"] / extras["gcn_bf16_ms_per_epoch"], 3)
-        # the bf16 step captured once in a HIP graph and replayed (one launch per epoch)
-        torch.manual_seed(0)
-        task = gcn.make_task(precision="bf16", capturable=True)
-        step = gcn.GraphedStep(task)
-        el_g, _ = timed(step.replay, 30, 5, [])
-        extras["gcn_bf16_graph_ms_per_epoch"] = round(el_g / 30 * 1e3, 3)
-        del step, task
+        # each precision's step captured once in a HIP graph and replayed (one launch per
+        # epoch): the device-side cost of an epoch, like for like
+        for prec in ("fp32", "bf16"):
+            torch.manual_seed(0)
+            task = gcn.make_task(precision=prec, capturable=True)
+            step = gcn.GraphedStep(task)
+            el_g, _ = timed(step.replay, 30, 5, [])
+            extras[f"gcn_{prec}_graph_ms_per_epoch"] = round(el_g / 30 * 1e3, 3)
+            del step, task
+        extras["gcn_bf16_graph_speedup"] = round(
+            extras["gcn_fp32_graph_ms_per_epoch"] / extras["gcn_bf16_graph_ms_per_epoch"], 3)
     if "gcn_scaled" in only:
         from armsparse_amd import gcn
 
@@ -312,24 +606,13 @@ def main():
 
     cpu = None
     if "cpu" in only:
-        import oracle
-
-        threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-        Bh = B.cpu().numpy()
-        Ch = np.zeros((M, N), np.float32)
-        oracle.rowBaseSpmm3(A.rowptr, A.colind, A.values, Bh, Ch, threads=threads)  # warm-up
-        iters, t = 0, time.perf_counter()
-        while True:
-            oracle.rowBaseSpmm3(A.rowptr, A.colind, A.values, Bh, Ch, threads=threads)
-            iters += 1
-            if time.perf_counter() - t >= args.cpu_seconds or iters >= 30:
-                break
-        el_c = time.perf_counter() - t
-        cpu = {"value": round(spmm_bytes(4, accumulate=True) * iters / el_c / 1e9, 2), "unit": "GB/s",
-               "cores": threads, "kind": "port",
-               "sample": f"full 1M R-MAT N=128, rowBaseSpmm3 (C += A·B, accumulate bytes), 1 warm-up + {iters} "
-                         f"timed calls, {el_c / iters * 1e3:.0f} ms/call"}
+        cpu = cpu_baselines(args, A, B, spmm_bytes(4, accumulate=True))
         log(f"[bench] cpu baseline {cpu}")
+
+    mf = stamped_profile("gemm_mfma.json", GEMM_SOURCES)
+    if mf is not None:  # MFMA utilisation of the GCN's x·W GEMMs (counter summary of these sources)
+        extras["mfma"] = mf.get("kernels")
+    extras["err_vs_fp64"] = {k: float(f"{v:.3g}") for k, v in errs.items()}
 
     out = {
         "metric": "FP32 CSR SpMM GB/s (algorithmic bytes) on 1M x 1M R-MAT, N=128",
