@@ -90,6 +90,7 @@ _SIGS = {
     "armsp_timing_reset": (c_int, []),
     "armsp_timing_read": (c_int, [c_char_p, POINTER(c_double), POINTER(c_int64)]),
     "armsp_spmm_f32": (c_int, [_I, _I, _I, _I, _P, _P, _P, _P, _I, _P, _I, _I, _CFG, _P]),
+    "armsp_spmm_f32_noalloc": (c_int, [_I, _I, _I, _I, _P, _P, _P, _P, _I, _P, _I, _I, _CFG, _P]),
     "armsp_spmm_bf16": (c_int, [_I, _I, _I, _I, _P, _P, _P, _P, _I, _P, _I, _I, _CFG, _P]),
     "armsp_spmm_fp8": (c_int, [_I, _I, _I, _I, _P, _P, _P, _P, _P, _I, _P, _I, _I, _CFG, _P]),
     "armsp_spmm_ex": (c_int, [_I, _I, _I, _I, _P, _P, _P, _I, _P, _P, _I, _P, _I, _I, POINTER(Epilogue), _CFG,

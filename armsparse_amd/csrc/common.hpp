@@ -51,6 +51,13 @@ struct TimedLaunch {
     void stop();
 };
 
+// While alive, TimedLaunch records nothing on this thread (entry points that
+// must not create HIP objects).
+struct TimingPause {
+    TimingPause();
+    ~TimingPause();
+};
+
 // Number of compute units on the current device (cached).
 int device_cus();
 

@@ -162,10 +162,17 @@ int workspace_reserve(size_t bytes, size_t* reserved) {
     return ARMSP_SUCCESS;
 }
 
-bool timing_on() { return g_timing; }
+namespace {
+thread_local int g_timing_paused = 0;
+}
+
+bool timing_on() { return g_timing && !g_timing_paused; }
+
+TimingPause::TimingPause() { ++g_timing_paused; }
+TimingPause::~TimingPause() { --g_timing_paused; }
 
 TimedLaunch::TimedLaunch(const char* family, hipStream_t s) {
-    if (!g_timing) return;
+    if (!timing_on()) return;
     hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;  // no timing events inside a captured graph
     if (hipStreamIsCapturing(s, &cs) != hipSuccess || cs != hipStreamCaptureStatusNone) return;
     TimingRecord* r = new TimingRecord{family, nullptr, nullptr};

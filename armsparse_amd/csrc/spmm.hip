@@ -18,6 +18,7 @@
 //  carry buffer; a second kernel adds each run of carries into its row in
 //  wave order, so results are deterministic run to run.
 #include <climits>
+#include <mutex>
 
 #include "common.hpp"
 
@@ -1191,10 +1192,17 @@ int launch_split(bool wide, int U, dim3 grid, hipStream_t s, int M, int N, int n
                                    cval);
 }
 
+// Device memory a call may use instead of allocating its workspace (the entry
+// point that allocates nothing): `bytes` at `p`, 256-byte aligned.
+struct Scratch {
+    void* p;
+    size_t bytes;
+};
+
 template <typename TB>
 int spmm_impl(int M, int K, int N, int nnz, const int* rowptr, const int* colind, const float* val, const TB* B,
               const float* b_row_scale, int ldb, float* C, int ldc, int flags, const armsp_epilogue* epi,
-              const armsp_config* cfg, void* stream) {
+              const armsp_config* cfg, void* stream, const Scratch* scratch = nullptr) {
     if (M < 0 || K < 0 || N < 0 || nnz < 0) return ARMSP_ERROR_INVALID_VALUE;
     if (M == 0 || N == 0) return ARMSP_SUCCESS;
     if (!rowptr || !C || ldc < N) return ARMSP_ERROR_INVALID_VALUE;
@@ -1299,8 +1307,18 @@ int spmm_impl(int M, int K, int N, int nnz, const int* rowptr, const int* colind
     const size_t row_bytes = ((size_t)nwaves * sizeof(int) + 255) & ~(size_t)255;
     const size_t val_bytes = ((size_t)nwaves * N * sizeof(float) + 255) & ~(size_t)255;
     const size_t part_bytes = staged ? (size_t)(nwaves + 1) * sizeof(int) : 0;
-    int st = ws_alloc(&ws, row_bytes + val_bytes + part_bytes, s);
-    if (st) return st;
+    int st;
+    if (scratch) {  // no allocation: the caller's region, or a clear refusal
+        if (row_bytes + val_bytes + part_bytes > scratch->bytes) return ARMSP_ERROR_NOT_SUPPORTED;
+        ws = scratch->p;
+    } else {
+        st = ws_alloc(&ws, row_bytes + val_bytes + part_bytes, s);
+        if (st) return st;
+    }
+    // a caller's region is never freed here
+    auto release = [&](void* w, hipStream_t st_) {
+        if (!scratch) ws_free(w, st_);
+    };
     int* crow = static_cast<int*>(ws);
     float* cval = reinterpret_cast<float*>(static_cast<char*>(ws) + row_bytes);
     int* part = staged ? reinterpret_cast<int*>(static_cast<char*>(ws) + row_bytes + val_bytes) : nullptr;
@@ -1308,7 +1326,7 @@ int spmm_impl(int M, int K, int N, int nnz, const int* rowptr, const int* colind
         TimedLaunch t("merge_partition", s);
         st = merge_partition(M, nnz, chunk, nwaves, rowptr, part, s);
         if (st) {
-            ws_free(ws, s);
+            release(ws, s);
             return st;
         }
         t.stop();
@@ -1333,7 +1351,7 @@ int spmm_impl(int M, int K, int N, int nnz, const int* rowptr, const int* colind
                        (merge && !wide && c.group == 32 && chunk <= 2048 && (slb == 4 || slb == 8 || slb == 16) &&
                         N % SV == 0 && ldb % SV == 0 && ldc % SV == 0 && aligned(B, slb) && aligned(C, 4 * SV));
     if (c.group == 32 && !slice) {
-        ws_free(ws, s);
+        release(ws, s);
         return ARMSP_ERROR_NOT_SUPPORTED;
     }
     dim3 grid((unsigned)((nwaves + 3) / 4), tiles);
@@ -1343,7 +1361,7 @@ int spmm_impl(int M, int K, int N, int nnz, const int* rowptr, const int* colind
                                part, crow, cval);
         hipError_t e = st ? hipSuccess : hipGetLastError();
         if (st || e != hipSuccess) {
-            ws_free(ws, s);
+            release(ws, s);
             return st ? st : hip_fail(e);
         }
         t.stop();
@@ -1354,7 +1372,7 @@ int spmm_impl(int M, int K, int N, int nnz, const int* rowptr, const int* colind
                               bscale, C, ldc, flags, part, crow, cval);
         hipError_t e = st ? hipSuccess : hipGetLastError();
         if (st || e != hipSuccess) {
-            ws_free(ws, s);
+            release(ws, s);
             return st ? st : hip_fail(e);
         }
         t.stop();
@@ -1384,12 +1402,12 @@ int spmm_impl(int M, int K, int N, int nnz, const int* rowptr, const int* colind
         }
 #undef ARMSP_SPLIT_V
         if (st) {
-            ws_free(ws, s);
+            release(ws, s);
             return st;
         }
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) {
-            ws_free(ws, s);
+            release(ws, s);
             return hip_fail(e);
         }
         t.stop();
@@ -1403,10 +1421,39 @@ int spmm_impl(int M, int K, int N, int nnz, const int* rowptr, const int* colind
         else
             hipLaunchKernelGGL((k_carry_fixup<1>), fgrid, dim3(256), 0, s, (int)nwaves, N, crow, cval, C, ldc, bscale);
         hipError_t e = hipGetLastError();
-        ws_free(ws, s);
+        release(ws, s);
         if (e != hipSuccess) return hip_fail(e);
         t.stop();
     }
+    return ARMSP_SUCCESS;
+}
+
+// ------------------------------------------------- allocation-free entry point
+// The merge-path partition and carry buffers of armsp_spmm_f32_noalloc live in
+// this file-scope array: each device holds its own copy for the life of the
+// process.  Host code reaches a device's copy only through the address
+// hipGetSymbolAddress returns with that device current (the array's name on the
+// host is not a device address).
+constexpr size_t kNoAllocScratchBytes = (size_t)ARMSP_NOALLOC_SCRATCH_BYTES;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[64] = {};
+
+// The calling thread's current device's copy of `scratch`, looked up once per device.
+int scratch_address(void** out) {
+    int dev = 0;
+    ARMSP_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= 64) return ARMSP_ERROR_NOT_SUPPORTED;
+    std::lock_guard<std::mutex> g(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return hip_fail(e);
+        if (!p) return ARMSP_ERROR_HIP;
+        g_scratch_ptr[dev] = p;
+    }
+    *out = g_scratch_ptr[dev];
     return ARMSP_SUCCESS;
 }
 
@@ -1414,6 +1461,18 @@ int spmm_impl(int M, int K, int N, int nnz, const int* rowptr, const int* colind
 }  // namespace armsp
 
 extern "C" {
+
+int armsp_spmm_f32_noalloc(int M, int K, int N, int nnz, const int* rowptr, const int* colind, const float* val,
+                           const float* B, int ldb, float* C, int ldc, int flags, const armsp_config* cfg,
+                           void* stream) {
+    void* p = nullptr;
+    const int st = armsp::scratch_address(&p);
+    if (st) return st;
+    const armsp::Scratch sc{p, armsp::kNoAllocScratchBytes};
+    armsp::TimingPause quiet;  // no timing events either
+    return armsp::spmm_impl<float>(M, K, N, nnz, rowptr, colind, val, B, nullptr, ldb, C, ldc, flags, nullptr, cfg,
+                                   stream, &sc);
+}
 
 int armsp_spmm_f32(int M, int K, int N, int nnz, const int* rowptr, const int* colind, const float* val,
                    const float* B, int ldb, float* C, int ldc, int flags, const armsp_config* cfg, void* stream) {

@@ -121,6 +121,29 @@ int armsp_spmm_f32(int M, int K, int N, int nnz,
                    const float* B, int ldb, float* C, int ldc,
                    int flags, const armsp_config* cfg, void* stream);
 
+/* armsp_spmm_f32 without any allocation: the same kernels and results (bit for
+ * bit, for the same arguments), with the merge-path partition and carry buffers
+ * in a device-resident array of ARMSP_NOALLOC_SCRATCH_BYTES that the library
+ * holds on every GPU it runs on until the process ends, instead of workspace
+ * from the memory pool.  No hipMalloc, hipFree, pool, event or other allocating
+ * call is made.  A call whose workspace would exceed the array returns
+ * ARMSP_ERROR_NOT_SUPPORTED before launching anything.  The merge-path
+ * schedules need about 4·W·(N + 2) bytes for W = ceil((M + nnz) / chunk) waves,
+ * chunk being the items per wave the call runs with (cfg->chunk, else 256 to
+ * 1024 chosen by shape); the 1M×1M R-MAT with 2^24 nnz at N = 128 needs 36 MB.
+ * The row schedule needs none.  Every call on a GPU shares that GPU's
+ * array, so calls that overlap on one GPU (concurrent streams, host threads)
+ * race: serialise them.  The array keeps the last call's contents; nothing in it
+ * needs clearing between calls.
+ * No counterpart in the reference: for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc). */
+#define ARMSP_NOALLOC_SCRATCH_BYTES (64u << 20)
+int armsp_spmm_f32_noalloc(int M, int K, int N, int nnz,
+                           const int* rowptr, const int* colind, const float* val,
+                           const float* B, int ldb, float* C, int ldc,
+                           int flags, const armsp_config* cfg, void* stream);
+
 /* Same with B stored as bfloat16 bits; fp32 accumulate, fp32 C.  No
  * reference counterpart (the reference is fp32 only, spmm.cpp:112). */
 int armsp_spmm_bf16(int M, int K, int N, int nnz,
