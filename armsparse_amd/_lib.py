@@ -123,6 +123,9 @@ _SIGS = {
     "armsp_bias_relu_dropout_bf16": (c_int, [_I, _I, _P, c_int64, _P, c_float, _P, _P, _P, _P]),
     "armsp_bias_relu_dropout_backward_bf16": (c_int, [_I, _I, _P, _P, c_float, _P, _P, _P]),
     "armsp_adam_step": (c_int, [c_int64, _P, _P, _P, _P, c_float, c_float, c_float, c_float, c_float, _I, _P]),
+    "armsp_selector_set": (c_int, [_I, _I, _I, _I, _P, _P, _P, _P, _P, _P, _P, _P, _P]),
+    "armsp_selector_clear": (c_int, [_I]),
+    "armsp_selector_choose": (c_int, [_I, _I, _I, _I, _I, _P, _P, _P]),
     "armsp_mlp_infer": (c_int, [_I, _I, _I, _I, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P]),
     # host-side generators and format helpers (armsp_gen.h)
     "armsp_rng_u64": (c_uint64, [c_uint64, c_uint64, c_uint64]),

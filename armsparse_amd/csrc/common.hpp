@@ -58,6 +58,12 @@ struct TimingPause {
     ~TimingPause();
 };
 
+// The installed schedule selector's configuration for an automatic call of
+// operation op (0 SpMM, 1 SDDMM), if it has one (true: *out set).  may_compute:
+// a first choice for this matrix may allocate and synchronise `s`.
+bool selector_config(int op, int M, int nnz, int N, int esz, const int* rowptr, hipStream_t s, bool may_compute,
+                     armsp_config* out);
+
 // Number of compute units on the current device (cached).
 int device_cus();
 

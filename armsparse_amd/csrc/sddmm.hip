@@ -359,6 +359,10 @@ int sddmm_impl(bool coo, int M, int K, int Kd, int nnz, const int* rowidx, const
     if (!rowidx || !colind || !out || Kd == 0 || !U || !V || ldu < Kd || ldv < Kd) return ARMSP_ERROR_INVALID_VALUE;
     hipStream_t s = as_stream(stream);
     armsp_config c = cfg ? *cfg : armsp_config{0, 0, 0, 0, 0};
+    if (!coo && c.schedule == 0 && c.group == 0 && c.vec == 0 && c.chunk == 0 && c.unroll == 0) {
+        armsp_config sel;  // left to the library: the installed schedule selector, if any
+        if (selector_config(1, M, nnz, Kd, (int)sizeof(T), rowidx, s, true, &sel)) c = sel;
+    }
     if (c.schedule != ARMSP_SCHED_AUTO && c.schedule != ARMSP_SCHED_NNZ && c.schedule != ARMSP_SCHED_MERGE)
         return ARMSP_ERROR_NOT_SUPPORTED;
     // vector width: 16-byte loads when every row start is aligned

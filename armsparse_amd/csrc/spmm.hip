@@ -1215,6 +1215,13 @@ int spmm_impl(int M, int K, int N, int nnz, const int* rowptr, const int* colind
     hipStream_t s = as_stream(stream);
 
     armsp_config c = cfg ? *cfg : armsp_config{0, 0, 0, 0, 0};
+    // left to the library: the installed schedule selector, if any (never on the
+    // allocation-free path unless this matrix's choice is already known)
+    if (c.schedule == 0 && c.group == 0 && c.vec == 0 && c.chunk == 0 && c.unroll == 0 && !(flags & ARMSP_TRUNC4) &&
+        !epi && nnz > 0) {
+        armsp_config sel;
+        if (selector_config(0, M, nnz, N, (int)sizeof(TB), rowptr, s, scratch == nullptr, &sel)) c = sel;
+    }
     int sched = c.schedule;
     // AUTO: the merge path at every width — N ≤ 4 runs the per-lane item walk
     // (k_spmm_narrow), wider N a whole wave or half waves per nonzero.  (A row per

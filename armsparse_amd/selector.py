@@ -449,6 +449,39 @@ def install(p_min: float = 0.0) -> Selector:
     return sel
 
 
+NATIVE_OPS = {"spmm": 0, "sddmm": 1}  # ARMSP_SELECTOR_SPMM / _SDDMM
+
+
+def install_native(weights: Optional[dict] = None) -> None:
+    """Install the networks in the library itself (armsp_selector_set): every AUTO
+    SpMM / SDDMM call, from Python, the C ABI or the command-line drivers, then runs
+    the configuration the network picks (features and inference on the device, one
+    read-back per matrix, cached)."""
+    if weights is None:
+        with open(WEIGHTS_PATH) as f:
+            weights = json.load(f)
+    for op, code in NATIVE_OPS.items():
+        st = weights[op]
+        F, H, C = int(st["F"]), int(st["H"]), int(st["C"])
+        arrs = [np.ascontiguousarray(np.asarray(st[k], np.float32).reshape(-1))
+                for k in ("mu", "sd", "W1", "b1", "W2", "b2", "W3", "b3")]
+        classes = (Config * C)(*[Config.make(**c) if c else Config() for c in CONFIGS[op]])
+        _lib.call("armsp_selector_set", code, F, H, C, *[ctypes.c_void_p(a.ctypes.data) for a in arrs], classes)
+
+
+def uninstall_native() -> None:
+    for code in NATIVE_OPS.values():
+        _lib.call("armsp_selector_clear", code)
+
+
+def native_choice(op: str, rowptr: torch.Tensor, m: int, nnz: int, n: int, element_bytes: int = 4) -> Optional[dict]:
+    """The configuration the library's AUTO runs for this matrix and width (None: the heuristic)."""
+    out = Config()
+    _lib.call("armsp_selector_choose", NATIVE_OPS[op], m, nnz, n, element_bytes, _ptr(rowptr), ctypes.byref(out),
+              _stream())
+    return None if not any((out.schedule, out.group, out.vec, out.chunk, out.unroll)) else out.as_dict()
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--matrices", type=int, default=2000)

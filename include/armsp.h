@@ -372,6 +372,37 @@ int armsp_mlp_infer(int nvec, int F, int H, int C, const float* x, const float* 
                     const float* b1, const float* W2, const float* b2, const float* W3, const float* b3,
                     float* logits, int* cls, void* stream);
 
+/* The learned selector as the library's own AUTO: after armsp_selector_set,
+ * every SpMM (fp32 B, no epilogue) or CSR SDDMM (fp32) call whose armsp_config
+ * is NULL or all zero runs the configuration the network picks for its matrix
+ * and width.  The first such call on a matrix computes the features on the
+ * device (armsp_row_stats, a features kernel), runs armsp_mlp_infer and reads
+ * the chosen class back (one synchronisation of the call's stream; the
+ * network's weights are copied to the device at first use); the choice is then
+ * cached per (op, rowptr address, M, nnz, width, element size) until the next
+ * armsp_selector_set or armsp_selector_clear.  A caller that reuses a rowptr
+ * address for another structure may get a slower schedule, never a different
+ * result.  Under HIP graph capture, and in armsp_spmm_f32_noalloc, only cached
+ * choices are used (else the heuristic).  op: ARMSP_SELECTOR_SPMM or
+ * ARMSP_SELECTOR_SDDMM; F must be 17 (the features above: log2 of M, nnz + 1,
+ * nnz/M + 1, max row length + 1 and the width, then the fraction of rows in
+ * each of 12 log-scale row-length bins, as armsp_row_stats counts them); the
+ * arrays are host memory, W row-major [in×out]; classes[c] is the configuration
+ * of class c (all zero: the library's heuristic).  No reference counterpart
+ * (the reference picks its kernel by a fixed N threshold,
+ * gcn/pytorch-custom/spmm_kernel.cu:437-457). */
+#define ARMSP_SELECTOR_SPMM 0
+#define ARMSP_SELECTOR_SDDMM 1
+int armsp_selector_set(int op, int F, int H, int C, const float* mu, const float* sd, const float* W1,
+                       const float* b1, const float* W2, const float* b2, const float* W3, const float* b3,
+                       const armsp_config* classes);
+/* Removes op's network: its AUTO calls use the heuristic again. */
+int armsp_selector_clear(int op);
+/* *out = the configuration an AUTO call of op on this matrix and width would
+ * run (all zero: the heuristic); computes and caches the choice like a call. */
+int armsp_selector_choose(int op, int M, int nnz, int N, int element_bytes, const int* rowptr,
+                          armsp_config* out, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -117,3 +117,72 @@ def test_network_beats_heuristic_on_held_out_matrices():
         print(op, "mlp", r["mlp"], "heuristic", r["heuristic_auto"])
         assert r["mlp"]["total"] <= 0.10
         assert r["mlp"]["total"] < r["heuristic_auto"]["total"]
+
+
+@pytest.fixture
+def native_selector():
+    selector.install_native()
+    yield
+    selector.uninstall_native()
+
+
+def test_native_selector_drives_auto(native_selector):
+    """The shipped networks installed in the library (armsp_selector_set) decide every
+    AUTO call: on 40 held-out corpus matrices (the train/test split of train_eval) ×
+    5 widths × SpMM/SDDMM the library's choice equals the Python network's, an AUTO
+    SpMM runs exactly the chosen configuration (bit-identical output), and by the
+    recorded autotuner timings the choices' total regret is at most the heuristic's."""
+    import math
+
+    from armsparse_amd import kernels
+    from armsparse_amd._lib import Config
+
+    z = np.load(DATASET)
+    specs = selector.corpus_specs(2000, 2024)
+    mids = np.unique(z["spmm_mid"])
+    held = np.random.default_rng(7).permutation(mids)[: max(1, len(mids) // 5)]
+    held = [int(m) for m in held if specs[int(m)]["m"] <= 1 << 17][:40]
+    nets = {op: selector.MLP.from_state(st) for op, st in
+            __import__("json").load(open(selector.WEIGHTS_PATH)).items()}
+    norm = lambda c: None if c is None else Config.make(**c).as_dict()  # noqa: E731
+    agree = total = 0
+    t_sel = {"spmm": [], "sddmm": []}
+    t_best = {"spmm": [], "sddmm": []}
+    t_heur = {"spmm": [], "sddmm": []}
+    for mid in held:
+        sp = specs[mid]
+        a = data.corpus_csr(sp["kind"], sp["m"], sp["avg"], sp["param"], sp["seed"])
+        rp = torch.from_numpy(a.rowptr).cuda()
+        m, nnz = a.shape[0], a.nnz
+        hist, mx = selector.row_stats(rp, m)
+        for op in ("spmm", "sddmm"):
+            rows = np.nonzero(z[f"{op}_mid"] == mid)[0]
+            for n in selector.WIDTHS:
+                got = selector.native_choice(op, rp, m, nnz, n)
+                f = selector.features(m, nnz, hist, mx, n)
+                cls = int(nets[op].predict(f[None, :])[0])
+                total += 1
+                agree += got == norm(selector.CONFIGS[op][cls])
+                r = [i for i in rows if abs(z[f"{op}_x"][i, 4] - math.log2(n)) < 1e-6]
+                if not r:
+                    continue
+                t = np.where(np.isfinite(z[f"{op}_t"][r[0]]), z[f"{op}_t"][r[0]], np.inf)
+                picked = [i for i, c in enumerate(selector.CONFIGS[op]) if norm(c) == got]
+                t_sel[op].append(t[picked[0]])
+                t_best[op].append(t.min())
+                t_heur[op].append(t[0])
+        # AUTO runs the chosen configuration: same bits as asking for it explicitly
+        ci = torch.from_numpy(a.colind).cuda()
+        va = torch.from_numpy(a.values).cuda()
+        B = torch.randn(a.shape[1], 64, device="cuda")
+        auto = kernels.spmm(rp, ci, va, B)
+        chosen = selector.native_choice("spmm", rp, m, nnz, 64)
+        explicit = kernels.spmm(rp, ci, va, B, config=chosen if chosen else Config())
+        assert torch.equal(auto, explicit), (mid, chosen)
+    print(f"native vs python network choices agree on {agree}/{total}")
+    assert agree >= total - 1
+    for op in ("spmm", "sddmm"):
+        sel, best, heur = (float(np.sum(v[op])) for v in (t_sel, t_best, t_heur))
+        print(f"{op}: native-selector regret {sel / best - 1:.3%}, heuristic {heur / best - 1:.3%} "
+              f"over {len(t_sel[op])} samples")
+        assert sel <= heur
