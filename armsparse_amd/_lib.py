@@ -123,6 +123,7 @@ _SIGS = {
     "armsp_bias_relu_dropout_bf16": (c_int, [_I, _I, _P, c_int64, _P, c_float, _P, _P, _P, _P]),
     "armsp_bias_relu_dropout_backward_bf16": (c_int, [_I, _I, _P, _P, c_float, _P, _P, _P]),
     "armsp_adam_step": (c_int, [c_int64, _P, _P, _P, _P, c_float, c_float, c_float, c_float, c_float, _I, _P]),
+    "armsp_probe_gather_f32": (c_int, [_I, _P, _P, _I, _I, _I, _I, _I, _P, _P]),
     "armsp_selector_set": (c_int, [_I, _I, _I, _I, _P, _P, _P, _P, _P, _P, _P, _P, _P]),
     "armsp_selector_clear": (c_int, [_I]),
     "armsp_selector_choose": (c_int, [_I, _I, _I, _I, _I, _P, _P, _P]),

@@ -140,7 +140,8 @@ def parse():
     p.add_argument("--steps", type=int, default=50)
     p.add_argument("--warmup", type=int, default=5)
     p.add_argument("--only", default="", help="comma list: spmm,sddmm,spmv,mixed,widths,shards,small,gcn,gcn_scaled,cpu (default all); "
-                        "regimes (column-placement variants of the headline matrix) runs only when named")
+                        "regimes (column-placement variants of the headline matrix) and probe (the B-row gathers "
+                        "alone on those column patterns) run only when named")
     p.add_argument("--schedule", default="", help="override the SpMM schedule (row|nnz|merge)")
     p.add_argument("--chunk", type=int, default=0)
     p.add_argument("--unroll", type=int, default=0)
@@ -368,6 +369,16 @@ def main():
     Bh = B.cpu().numpy()
     errs["spmm_fp32_n128"] = spmm_rows_err(A.rowptr, A.colind, A.values, Bh, C[rows_t].cpu().numpy(), rows)
     log(f"[bench] spmm fp32 max row rel err vs FP64 (sampled rows): {errs['spmm_fp32_n128']:.2e}")
+    if cfg is None:  # the same call with the learned selector deciding AUTO inside the library
+        from armsparse_amd import selector
+
+        selector.install_native()
+        chosen = selector.native_choice("spmm", rp, M, nnz, N)
+        el_s, fam_s = timed(fn, max(10, args.steps // 2), 2, ["spmm"])
+        selector.uninstall_native()
+        extras["spmm_native_selector_config"] = chosen or "heuristic"
+        extras["spmm_native_selector_main_kernel_us"] = round(fam_s["spmm"][0] / max(fam_s["spmm"][1], 1) * 1e3, 2)
+        extras["spmm_native_selector_GBps"] = round(nbytes * max(10, args.steps // 2) / el_s / 1e9, 1)
 
     if "mixed" in only:
         Bb = kernels.to_bf16(B)
@@ -397,15 +408,32 @@ def main():
         variants = {"rmat": A.colind, "rmat_mod64k": (c64 % 65536).astype(np.int32),
                     "uniform": rng.integers(0, K, nnz).astype(np.int32),
                     "rmat_oddmul": ((c64 * 2654435761) & (K - 1)).astype(np.int32)}
-        rows = np.repeat(np.arange(M, dtype=np.int64), np.diff(A.rowptr))
+        row_of = np.repeat(np.arange(M, dtype=np.int64), np.diff(A.rowptr))
         for name, cv in variants.items():
-            order = np.argsort(rows * K + cv, kind="stable")  # keep every row's columns sorted
+            order = np.argsort(row_of * K + cv, kind="stable")  # keep every row's columns sorted
             vci = torch.from_numpy(np.ascontiguousarray(cv[order])).to(dev)
             vva = torch.from_numpy(np.ascontiguousarray(A.values[order])).to(dev)
             el_r, fam_r = timed(lambda: kernels.spmm(rp, vci, vva, B, out=C), max(2, args.steps // 5), 1, ["spmm"])
             extras[f"regime_{name}_kernel_us"] = round(fam_r["spmm"][0] / max(fam_r["spmm"][1], 1) * 1e3, 1)
             log(f"[bench] regime {name}: {extras[f'regime_{name}_kernel_us']} us")
             del vci, vva
+    if "probe" in only:  # not in the default set: the B-row gathers alone (armsp_probe_gather_f32)
+        rng = np.random.default_rng(0)
+        c64 = A.colind.astype(np.int64)
+        pats = {"rmat": A.colind, "rmat_oddmul": ((c64 * 2654435761) & (K - 1)).astype(np.int32),
+                "uniform": rng.integers(0, K, nnz).astype(np.int32), "rmat_mod64k": (c64 % 65536).astype(np.int32)}
+        pout = torch.empty(((nnz + 511) // 512 + 4) * 64 * 8, device=dev)
+        for pname, pidx in pats.items():
+            di = torch.from_numpy(np.ascontiguousarray(pidx)).to(dev)
+            for grp, vec in ((32, 2), (16, 4), (64, 2)):
+                call = lambda: _lib.call("armsp_probe_gather_f32", nnz, kernels._ptr(di), kernels._ptr(B), K, N,  # noqa: E731
+                                         N, grp, vec, kernels._ptr(pout), kernels._stream(dev))
+                el_p, fam_p = timed(call, 10, 2, ["probe"])
+                us = fam_p["probe"][0] / max(fam_p["probe"][1], 1) * 1e3
+                extras[f"probe_{pname}_g{grp}v{vec}_us"] = round(us, 1)
+                extras[f"probe_{pname}_g{grp}v{vec}_TBps"] = round(nnz * N * 4 / us / 1e6, 2)
+                log(f"[bench] gather probe {pname} group {grp} vec {vec}: {us:.1f} us, {nnz * N * 4 / us / 1e6:.2f} TB/s")
+            del di
     if "widths" in only:  # the same matrix at N = 64 and 256 (fp32 and bf16)
         for n2 in (64, 256):
             B2 = torch.empty((K, n2), device=dev)
@@ -421,18 +449,18 @@ def main():
                                                          C2[rows_t].cpu().numpy(), rows)
             del B2, C2
     if "shards" in only:  # one GPU's share of an 8-way row split: heaviest and median 131,072-row blocks
-        rows = M // 8
-        per = [int(A.rowptr[(i + 1) * rows] - A.rowptr[i * rows]) for i in range(8)]
+        srows = M // 8
+        per = [int(A.rowptr[(i + 1) * srows] - A.rowptr[i * srows]) for i in range(8)]
         order = sorted(range(8), key=lambda i: per[i])
         for tag, i in (("heavy", order[-1]), ("median", order[3])):
-            lo, hi = int(A.rowptr[i * rows]), int(A.rowptr[(i + 1) * rows])
-            srp = (rp[i * rows:(i + 1) * rows + 1] - lo).contiguous()
+            lo, hi = int(A.rowptr[i * srows]), int(A.rowptr[(i + 1) * srows])
+            srp = (rp[i * srows:(i + 1) * srows + 1] - lo).contiguous()
             sci, sva = ci[lo:hi], va[lo:hi]
-            Cs = C[i * rows:(i + 1) * rows]
+            Cs = C[i * srows:(i + 1) * srows]
             el_s, _ = timed(lambda: kernels.spmm(srp, sci, sva, B, out=Cs), 10, 2, [])
             extras[f"shard_{tag}_nnz"] = hi - lo
             extras[f"shard_{tag}_us"] = round(el_s / 10 * 1e6, 1)
-        extras["shard_rows"] = rows
+        extras["shard_rows"] = srows
     if "small" in only:  # the ./SpMM 4096 case and the 65,536² 0.1% case (latency-bound, reported anyway)
         As, Bs, xs = data.spmm_small()
         srp, sci, sva, sB, sx = (torch.from_numpy(np.ascontiguousarray(t)).to(dev) for t in

@@ -372,6 +372,18 @@ int armsp_mlp_infer(int nvec, int F, int H, int C, const float* x, const float* 
                     const float* b1, const float* W2, const float* b2, const float* W3, const float* b3,
                     float* logits, int* cls, void* stream);
 
+/* Measurement probe (no reference counterpart): the B-row gathers of the SpMM
+ * kernels alone.  For i in [0, n) it reads the column slice of row idx[i] of B
+ * (K rows of N fp32 columns, row stride ldb) that its lane group covers:
+ * `group` lanes (8, 16, 32, 64) of `vec` floats (2 or 4) each; the N / (group·vec)
+ * slices (a divisor of 8) are spread over the XCDs as in the SpMM slice kernel.
+ * Writes per-lane sums to out (at least 64·S·ceil(n/512) floats, S the slice
+ * count, rounded up to whole 4-wave groups).  Its rate on a matrix's column
+ * indices against relabelled or uniform ones shows what the gather pattern
+ * costs by itself. */
+int armsp_probe_gather_f32(int n, const int* idx, const float* B, int K, int ldb, int N, int group, int vec,
+                           float* out, void* stream);
+
 /* The learned selector as the library's own AUTO: after armsp_selector_set,
  * every SpMM (fp32 B, no epilogue) or CSR SDDMM (fp32) call whose armsp_config
  * is NULL or all zero runs the configuration the network picks for its matrix
