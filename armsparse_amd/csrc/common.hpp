@@ -35,6 +35,10 @@ inline hipStream_t as_stream(void* s) {
     return reinterpret_cast<hipStream_t>(s);
 }
 
+// True when ARMSP_FAIL_ALLOC asks every device allocation to fail (tests of
+// the failure paths, also without a GPU).
+bool alloc_injected();
+
 // Stream-ordered workspace from the device's default memory pool.  The pool's
 // release threshold is raised at first use so freed blocks stay cached.
 int ws_alloc(void** p, size_t bytes, hipStream_t s);

@@ -11,6 +11,7 @@
 
 #include "armsp.h"
 #include "armsp_gen.h"
+#include "common.hpp"
 
 namespace {
 
@@ -31,6 +32,7 @@ void status_or_die(const char* fn, int st) {
 struct Dev {
     void* p = nullptr;
     Dev(const char* fn, size_t bytes) {
+        if (bytes && armsp::alloc_injected()) die(fn, "device allocation failed");
         if (bytes) hip_or_die(fn, hipMalloc(&p, bytes));
     }
     ~Dev() {
@@ -83,10 +85,10 @@ void spmm_shim(const char* fn, int* rowPtr, int* colInd, float* val, float* B, f
     if (M == 0 || N == 0 || ncols == 0) return;
     if (!B || !C) die(fn, "null dense operand");
     const int nnz = rowPtr[M];
-    hipStream_t s;
-    hip_or_die(fn, hipStreamCreate(&s));
     Dev drp(fn, (size_t)(M + 1) * 4), dci(fn, (size_t)nnz * 4), dva(fn, (size_t)nnz * 4);
     Dev dB(fn, (size_t)K * N * 4), dC(fn, (size_t)M * N * 4);
+    hipStream_t s;
+    hip_or_die(fn, hipStreamCreate(&s));
     hip_or_die(fn, hipMemcpyAsync(drp.p, rowPtr, (size_t)(M + 1) * 4, hipMemcpyHostToDevice, s));
     if (nnz) {
         hip_or_die(fn, hipMemcpyAsync(dci.p, colInd, (size_t)nnz * 4, hipMemcpyHostToDevice, s));

@@ -1,6 +1,8 @@
 // Library runtime: status strings, HIP error capture, stream-ordered
 // workspace, optional event timing of kernel launches.
 #include <atomic>
+#include <cstdlib>
+#include <cstring>
 #include <mutex>
 #include <string>
 #include <utility>
@@ -69,6 +71,18 @@ void resolve_records_locked() {
 }
 }  // namespace
 
+// ARMSP_FAIL_ALLOC set to a non-empty value other than "0" (read once per
+// process): every device allocation of the library fails as if the device were
+// out of memory, before any HIP call — the failure paths can be exercised
+// without a GPU.
+bool alloc_injected() {
+    static const bool on = [] {
+        const char* e = std::getenv("ARMSP_FAIL_ALLOC");
+        return e && *e && std::strcmp(e, "0") != 0;
+    }();
+    return on;
+}
+
 int hip_fail(hipError_t e) {
     g_last_hip_error = (int)e;
     if (e == hipErrorOutOfMemory) return ARMSP_ERROR_ALLOC;
@@ -81,6 +95,7 @@ int hip_fail(hipError_t e) {
 int ws_alloc(void** p, size_t bytes, hipStream_t s) {
     *p = nullptr;
     if (bytes == 0) return ARMSP_SUCCESS;
+    if (alloc_injected()) return hip_fail(hipErrorOutOfMemory);
     int dev = 0;
     ARMSP_HIP(hipGetDevice(&dev));
     if (dev < 0 || dev >= 64) return ARMSP_ERROR_NOT_SUPPORTED;
@@ -142,6 +157,7 @@ void ws_free(void* p, hipStream_t s) {
 }
 
 int workspace_reserve(size_t bytes, size_t* reserved) {
+    if (alloc_injected()) return hip_fail(hipErrorOutOfMemory);
     int dev = 0;
     ARMSP_HIP(hipGetDevice(&dev));
     if (dev < 0 || dev >= 64) return ARMSP_ERROR_NOT_SUPPORTED;

@@ -84,6 +84,7 @@ bool selector_config(int op, int M, int nnz, int N, int esz, const int* rowptr, 
     DevNet& d = g_dev[op][dev];
     const size_t nw = packed_floats(net.F, net.H, net.C);
     if (d.version != net.version) {
+        if (alloc_injected()) return false;
         if (d.w) hipFree(d.w);
         if (d.stats) hipFree(d.stats);
         d.w = nullptr;

@@ -136,6 +136,45 @@ def test_compat_shims_accept_descending_rows(tmp_path):
     assert "column" not in r.stderr and "rowPtr" not in r.stderr
 
 
+_INJECT = r"""
+import ctypes, sys
+import numpy as np
+lib = ctypes.CDLL(sys.argv[1])
+class Cfg(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_int32) for n in ("schedule", "group", "vec", "chunk", "unroll")]
+rp = np.array([0, 1, 2, 3, 4], np.int32); ci = np.arange(4, dtype=np.int32)
+va = np.ones(4, np.float32); B = np.ones((4, 8), np.float32); C = np.zeros((4, 8), np.float32)
+p = lambda a: ctypes.c_void_p(a.ctypes.data)
+st = lib.armsp_spmm_f32(4, 4, 8, 4, p(rp), p(ci), p(va), p(B), 8, p(C), 8, 0, ctypes.byref(Cfg(3, 0, 0, 0, 0)), None)
+res = ctypes.c_size_t(0)
+print(st, lib.armsp_workspace_reserve(1 << 20, ctypes.byref(res)), lib.armsp_last_hip_error())
+"""
+
+
+def test_injected_allocation_failure_without_gpu():
+    """ARMSP_FAIL_ALLOC: every device allocation of the library fails before any HIP
+    call, so the failure path runs without a GPU — an entry point that needs
+    workspace (merge-path SpMM) and the capture-arena reservation return
+    ARMSP_ERROR_ALLOC (4) with hipErrorOutOfMemory (2) recorded, and a reference-name
+    shim exits with the reference's message format."""
+    env = dict(os.environ, ARMSP_FAIL_ALLOC="1")
+    r = subprocess.run([sys.executable, "-c", _INJECT, LIB], capture_output=True, text=True, env=env, timeout=120)
+    assert r.returncode == 0, r.stderr
+    assert r.stdout.split() == ["4", "4", "2"]
+    # without the flag the same call gets as far as the (absent) device instead
+    r = subprocess.run([sys.executable, "-c", _INJECT, LIB], capture_output=True, text=True, timeout=120,
+                       env={k: v for k, v in os.environ.items() if k != "ARMSP_FAIL_ALLOC"})
+    if not os.path.exists("/dev/kfd"):
+        assert r.stdout.split()[0] != "4"
+
+
+def test_injected_allocation_failure_in_a_shim(tmp_path):
+    env = dict(os.environ, ARMSP_FAIL_ALLOC="1")
+    r = subprocess.run([str(_shim_exe(tmp_path)), "descending"], capture_output=True, text=True, timeout=60, env=env)
+    assert r.returncode == 1
+    assert r.stderr.strip() == "armsparse: rowBaseSpmm3: device allocation failed"
+
+
 def test_one_hip_runtime_whatever_the_import_order():
     """Importing the package before torch must not map a second HIP runtime next to
     torch's (launches on torch memory from the other one fail)."""
