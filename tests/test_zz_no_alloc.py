@@ -79,3 +79,42 @@ def test_noalloc_refuses_oversized_workspace_and_allocates_nothing():
     assert run("armsp_spmm_f32", Sd, Bs, ref) == 0
     torch.cuda.synchronize()
     assert torch.equal(Cs, ref)
+
+
+_NO_ALLOC_CHILD = r"""
+import ctypes, sys
+sys.path.insert(0, sys.argv[1])
+import torch
+from armsparse_amd import _lib, data
+A = data.rmat_csr(12, 1 << 14, seed=9)
+rp, ci, va = (torch.from_numpy(a).cuda() for a in (A.rowptr, A.colind, A.values))
+B = torch.randn(A.shape[1], 64, device="cuda")
+C1, C2 = torch.empty(A.shape[0], 64, device="cuda"), torch.empty(A.shape[0], 64, device="cuda")
+s = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+p = lambda t: ctypes.c_void_p(t.data_ptr())
+args = (A.shape[0], A.shape[1], 64, A.nnz, p(rp), p(ci), p(va), p(B), 64)
+st_alloc = _lib.lib.armsp_spmm_f32(*args, p(C1), 64, 0, None, s)
+st_free = _lib.lib.armsp_spmm_f32_noalloc(*args, p(C2), 64, 0, None, s)
+torch.cuda.synchronize()
+ref = torch.zeros(A.shape[0], 64, device="cuda").index_add_(
+    0, torch.repeat_interleave(torch.arange(A.shape[0], device="cuda"), rp.diff()), va[:, None] * B[ci.long()])
+ok = bool(torch.allclose(C2, ref, rtol=1e-5, atol=1e-5))
+print(st_alloc, st_free, ok)
+"""
+
+
+def test_noalloc_runs_when_every_allocation_fails():
+    """In a process where the library's every device allocation fails
+    (ARMSP_FAIL_ALLOC), the allocating entry point reports ARMSP_ERROR_ALLOC and the
+    allocation-free one still computes A·B."""
+    import os
+    import subprocess
+    import sys
+
+    print("[case] noalloc with allocations failing", flush=True)
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    r = subprocess.run([sys.executable, "-c", _NO_ALLOC_CHILD, root], capture_output=True, text=True, timeout=240,
+                       env=dict(os.environ, ARMSP_FAIL_ALLOC="1"))
+    print(r.stdout, r.stderr[-2000:])
+    assert r.returncode == 0
+    assert r.stdout.split()[-3:] == [str(_lib.ERROR_ALLOC), "0", "True"]
