@@ -19,13 +19,73 @@ typedef float f32x16 __attribute__((ext_vector_type(16)));
 
 constexpr int BM = 64, BN = 64, BK = 32, LDK = BK + 8;  // LDS row pitch in bf16
 
-__device__ __forceinline__ uint16_t ld_elem(const uint16_t* __restrict__ p, int64_t off) { return p[off]; }
+// Staging of a 64×32 operand tile (rows r0..r0+63 of M or N, k0..k0+31): each
+// thread carries 8 consecutive elements, packed in a uint4.  Tile modes: 0 — the
+// operand is k-contiguous (stride 1 along k, 16-byte aligned rows), one 16-byte
+// load along k; 1 — row-contiguous (stride 1 along the row dimension, 16-byte
+// aligned k slices), one 16-byte load along the rows, transposed into LDS by
+// eight 2-byte stores with consecutive lanes on consecutive k (bank-conflict
+// free); 2 — any strides, eight element loads along k.  Out-of-range elements
+// (rows ≥ R, k ≥ ke) are zero.
+__device__ __forceinline__ uint32_t pack2(uint16_t lo, uint16_t hi) { return lo | ((uint32_t)hi << 16); }
 
-// C_part[z] = op(A)[m0:m0+64, kz0:kz1] · op(B)[kz0:kz1, n0:n0+64]
+__device__ __forceinline__ uint4 tile_fetch(const uint16_t* __restrict__ P, int64_t sr, int64_t sk, int mode, int R,
+                                            int r0, int k0, int ke, int tid) {
+    uint4 v = make_uint4(0u, 0u, 0u, 0u);
+    uint16_t t[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    if (mode == 1) {
+        const int r = r0 + (tid >> 5) * 8, gk = k0 + (tid & 31);
+        if (gk < ke && r < R) {
+            const uint16_t* q = P + (int64_t)r + (int64_t)gk * sk;
+            if (r + 8 <= R) return *reinterpret_cast<const uint4*>(q);
+#pragma unroll
+            for (int i = 0; i < 8; ++i)
+                if (r + i < R) t[i] = q[i];
+        }
+    } else {
+        const int r = r0 + (tid >> 2), gk = k0 + (tid & 3) * 8;
+        if (r < R && gk < ke) {
+            const uint16_t* q = P + (int64_t)r * sr + (int64_t)gk * sk;
+            if (mode == 0 && gk + 8 <= ke) return *reinterpret_cast<const uint4*>(q);
+#pragma unroll
+            for (int i = 0; i < 8; ++i)
+                if (gk + i < ke) t[i] = q[(int64_t)i * sk];
+        }
+    }
+    v.x = pack2(t[0], t[1]);
+    v.y = pack2(t[2], t[3]);
+    v.z = pack2(t[4], t[5]);
+    v.w = pack2(t[6], t[7]);
+    return v;
+}
+
+// tile in LDS, k-contiguous rows of pitch LDK
+__device__ __forceinline__ void tile_commit(uint16_t* __restrict__ S, int mode, uint4 v, int tid) {
+    if (mode == 1) {
+        const int r = (tid >> 5) * 8, kk = tid & 31;
+        const uint32_t w[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+        for (int i = 0; i < 8; ++i) S[(r + i) * LDK + kk] = (uint16_t)(w[i >> 1] >> (16 * (i & 1)));
+    } else {
+        *reinterpret_cast<uint4*>(&S[(tid >> 2) * LDK + (tid & 3) * 8]) = v;
+    }
+}
+
+// tile mode of an operand whose element (r, k) sits at P[r·sr + k·sk]
+int tile_mode(const void* P, int64_t sr, int64_t sk) {
+    const bool al = ((uintptr_t)P % 16) == 0;
+    if (al && sk == 1 && sr % 8 == 0) return 0;
+    if (al && sr == 1 && sk % 8 == 0) return 1;
+    return 2;
+}
+
+// C_part[z] = op(A)[m0:m0+64, kz0:kz1] · op(B)[kz0:kz1, n0:n0+64]; the next k
+// step's tiles are loaded into registers while the current one is multiplied.
 __global__ __launch_bounds__(256) void k_gemm_bf16(int M, int N, int K, int kchunk, const uint16_t* __restrict__ A,
-                                                   int64_t sam, int64_t sak, const uint16_t* __restrict__ B,
-                                                   int64_t sbk, int64_t sbn, float* __restrict__ C, int64_t ldc,
-                                                   int64_t zstride, int accumulate) {
+                                                   int64_t sam, int64_t sak, int amode,
+                                                   const uint16_t* __restrict__ B, int64_t sbk, int64_t sbn,
+                                                   int bmode, float* __restrict__ C, int64_t ldc, int64_t zstride,
+                                                   int accumulate) {
     __shared__ __attribute__((aligned(16))) uint16_t sA[BM * LDK];
     __shared__ __attribute__((aligned(16))) uint16_t sB[BN * LDK];
     const int tid = threadIdx.x;
@@ -37,48 +97,16 @@ __global__ __launch_bounds__(256) void k_gemm_bf16(int M, int N, int K, int kchu
     f32x16 acc;
 #pragma unroll
     for (int i = 0; i < 16; ++i) acc[i] = 0.0f;
-
-    // each thread stages 8 elements of each operand per K step
-    const bool a_mcontig = (sam == 1 && sak != 1);
-    const bool b_ncontig = (sbn == 1 && sbk != 1);
+    uint4 va = tile_fetch(A, sam, sak, amode, M, m0, kz0, kz1, tid);
+    uint4 vb = tile_fetch(B, sbn, sbk, bmode, N, n0, kz0, kz1, tid);
     for (int k0 = kz0; k0 < kz1; k0 += BK) {
-        // A tile: 64 rows × 32 k
-        if (!a_mcontig) {
-            const int r = tid >> 2, kk = (tid & 3) * 8;  // row r, k in [kk, kk+8)
-            const int gm = m0 + r;
-#pragma unroll
-            for (int i = 0; i < 8; ++i) {
-                const int gk = k0 + kk + i;
-                sA[r * LDK + kk + i] = (gm < M && gk < kz1) ? ld_elem(A, (int64_t)gm * sam + (int64_t)gk * sak) : 0;
-            }
-        } else {
-            const int kk = tid >> 3, r = (tid & 7) * 8;  // k kk, rows [r, r+8)
-            const int gk = k0 + kk;
-#pragma unroll
-            for (int i = 0; i < 8; ++i) {
-                const int gm = m0 + r + i;
-                sA[(r + i) * LDK + kk] = (gm < M && gk < kz1) ? ld_elem(A, (int64_t)gm * sam + (int64_t)gk * sak) : 0;
-            }
-        }
-        // B tile stored n-major: sB[n][k]
-        if (!b_ncontig) {
-            const int c = tid >> 2, kk = (tid & 3) * 8;
-            const int gn = n0 + c;
-#pragma unroll
-            for (int i = 0; i < 8; ++i) {
-                const int gk = k0 + kk + i;
-                sB[c * LDK + kk + i] = (gn < N && gk < kz1) ? ld_elem(B, (int64_t)gk * sbk + (int64_t)gn * sbn) : 0;
-            }
-        } else {
-            const int kk = tid >> 3, c = (tid & 7) * 8;
-            const int gk = k0 + kk;
-#pragma unroll
-            for (int i = 0; i < 8; ++i) {
-                const int gn = n0 + c + i;
-                sB[(c + i) * LDK + kk] = (gn < N && gk < kz1) ? ld_elem(B, (int64_t)gk * sbk + (int64_t)gn * sbn) : 0;
-            }
-        }
+        tile_commit(sA, amode, va, tid);
+        tile_commit(sB, bmode, vb, tid);
         __syncthreads();
+        if (k0 + BK < kz1) {
+            va = tile_fetch(A, sam, sak, amode, M, m0, k0 + BK, kz1, tid);
+            vb = tile_fetch(B, sbn, sbk, bmode, N, n0, k0 + BK, kz1, tid);
+        }
 #pragma unroll
         for (int ks = 0; ks < BK; ks += 16) {
             const int koff = ks + (lane >> 5) * 8;
@@ -109,7 +137,8 @@ __global__ __launch_bounds__(256) void k_gemm_bf16(int M, int N, int K, int kchu
 // contiguous) and loops over row tiles of A, so A is read once and B once per
 // block.  Four waves as WR row strips of 32 × (4/WR) column groups of 32·NTW;
 // a row tile is 32·WR rows, staged in LDS 32 k at a time with vector loads
-// along k when A's rows allow (va = 8, 4 or 1 elements).
+// along k when A's rows allow (va = 8, 4 or 1 elements).  B is staged with
+// 16-byte loads when it is contiguous along n (x·W) or along k (a Wᵀ view).
 constexpr int TK = 32, TKP = TK + 8;  // A stage: 32 k, row pitch 40 bf16
 constexpr int RTK = 64, RTKP = RTK + 8;  // reduction stage: 64 k, row pitch 72 bf16
 
@@ -126,9 +155,48 @@ __global__ __launch_bounds__(256) void k_gemm_tall(int M, int N, int K, int KP, 
     const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
     const int wr = wid % WR, wc = wid / WR;
     const int bp = KP + 8;
-    for (int idx = tid; idx < NP * KP; idx += 256) {
-        const int n = idx / KP, k = idx - n * KP;
-        sB[n * bp + k] = (n < N && k < K) ? B[(int64_t)k * sbk + (int64_t)n * sbn] : (uint16_t)0;
+    const bool bal = ((uintptr_t)B % 16) == 0;
+    if (bal && sbn == 1 && sbk % 8 == 0) {  // rows of B contiguous in n: 16-byte loads, consecutive lanes on k
+        for (int idx = tid; idx < (NP / 8) * KP; idx += 256) {
+            const int g = idx / KP, k = idx - g * KP, n = g * 8;
+            uint16_t t[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+            if (k < K && n < N) {
+                const uint16_t* q = B + (int64_t)k * sbk + n;
+                if (n + 8 <= N) {
+                    const uint4 v = *reinterpret_cast<const uint4*>(q);
+                    const uint32_t w[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+                    for (int i = 0; i < 8; ++i) t[i] = (uint16_t)(w[i >> 1] >> (16 * (i & 1)));
+                } else {
+                    for (int i = 0; n + i < N; ++i) t[i] = q[i];
+                }
+            }
+#pragma unroll
+            for (int i = 0; i < 8; ++i) sB[(n + i) * bp + k] = t[i];
+        }
+    } else if (bal && sbk == 1 && sbn % 8 == 0) {  // columns of B contiguous in k: 16-byte loads along k
+        const int per = KP / 8;
+        for (int idx = tid; idx < NP * per; idx += 256) {
+            const int n = idx / per, k = (idx - n * per) * 8;
+            uint4 v = make_uint4(0u, 0u, 0u, 0u);
+            if (n < N && k < K) {
+                const uint16_t* q = B + (int64_t)n * sbn + k;
+                if (k + 8 <= K) {
+                    v = *reinterpret_cast<const uint4*>(q);
+                } else {
+                    uint16_t t[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+                    for (int i = 0; k + i < K; ++i) t[i] = q[i];
+                    v = make_uint4(t[0] | ((uint32_t)t[1] << 16), t[2] | ((uint32_t)t[3] << 16),
+                                   t[4] | ((uint32_t)t[5] << 16), t[6] | ((uint32_t)t[7] << 16));
+                }
+            }
+            *reinterpret_cast<uint4*>(&sB[n * bp + k]) = v;
+        }
+    } else {
+        for (int idx = tid; idx < NP * KP; idx += 256) {
+            const int n = idx / KP, k = idx - n * KP;
+            sB[n * bp + k] = (n < N && k < K) ? B[(int64_t)k * sbk + (int64_t)n * sbn] : (uint16_t)0;
+        }
     }
     __syncthreads();
     const int ntiles = (M + TR - 1) / TR;
@@ -458,7 +526,7 @@ namespace {
 int gemm_tall(int M, int N, int Kd, const uint16_t* A, int64_t sam, int64_t sak, const uint16_t* B, int64_t sbk,
               int64_t sbn, void* C, bool outbf, int64_t ldc, int accumulate, hipStream_t s, bool& done) {
     done = false;
-    if (!(M >= 4096 && N <= 256 && Kd <= 256 && Kd > 0)) return ARMSP_SUCCESS;
+    if (!(M >= 4096 && N <= 256 && Kd <= 1024 && Kd > 0)) return ARMSP_SUCCESS;  // and B fits in LDS (below)
     const int KP = (Kd + TK - 1) / TK * TK;
     int WR = 2, NTW = 4;
     if (N <= 32) {
@@ -570,10 +638,11 @@ int gemm_f32_out(int M, int N, int Kd, const uint16_t* A, int64_t sam, int64_t s
     kchunk = (kchunk + BK - 1) / BK * BK;
     if (kchunk == 0) kchunk = BK;
     splits = Kd > 0 ? (Kd + kchunk - 1) / kchunk : 1;
+    const int amode = tile_mode(A, sam, sak), bmode = tile_mode(B, sbn, sbk);
     TimedLaunch t("gemm", s);
     if (splits == 1) {
-        hipLaunchKernelGGL(k_gemm_bf16, dim3(gx, gy, 1), dim3(256), 0, s, M, N, Kd, kchunk, A, sam, sak, B, sbk, sbn, C,
-                           ldc, (int64_t)0, accumulate);
+        hipLaunchKernelGGL(k_gemm_bf16, dim3(gx, gy, 1), dim3(256), 0, s, M, N, Kd, kchunk, A, sam, sak, amode, B, sbk,
+                           sbn, bmode, C, ldc, (int64_t)0, accumulate);
         ARMSP_LAUNCHED();
         t.stop();
         return ARMSP_SUCCESS;
@@ -582,8 +651,8 @@ int gemm_f32_out(int M, int N, int Kd, const uint16_t* A, int64_t sam, int64_t s
     int st = ws_alloc(&ws, (size_t)splits * M * N * sizeof(float), s);
     if (st) return st;
     float* part = static_cast<float*>(ws);
-    hipLaunchKernelGGL(k_gemm_bf16, dim3(gx, gy, splits), dim3(256), 0, s, M, N, Kd, kchunk, A, sam, sak, B, sbk, sbn,
-                       part, (int64_t)N, (int64_t)M * N, 0);
+    hipLaunchKernelGGL(k_gemm_bf16, dim3(gx, gy, splits), dim3(256), 0, s, M, N, Kd, kchunk, A, sam, sak, amode, B, sbk,
+                       sbn, bmode, part, (int64_t)N, (int64_t)M * N, 0);
     hipError_t e = hipGetLastError();
     if (e == hipSuccess) {
         const int64_t n = (int64_t)M * N;

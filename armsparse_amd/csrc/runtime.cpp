@@ -90,8 +90,61 @@ int hip_fail(hipError_t e) {
     return ARMSP_ERROR_HIP;
 }
 
-// The library's own stream-ordered pool per device (the device's default pool,
-// which other libraries in the process share, is left untouched).
+// Workspaces of calls outside graph capture come from a small cache per host
+// thread and (device, stream): two slots, since an entry point holds at most two
+// workspaces at once.  A slot keeps its buffer between calls, so a steady
+// sequence of calls makes no allocator call at all; calls on one stream run in
+// stream order, and each host thread has its own slots, so two threads that
+// enqueue on the same stream never share a buffer.  A slot that is too small is
+// released with hipFreeAsync on the same stream (after the work that used it)
+// and replaced.  Buffers come from the library's own stream-ordered pool per
+// device (the device's default pool, which other libraries in the process share,
+// is left untouched) and are kept until the process ends, as the pool (release
+// threshold: never) would keep them anyway.  A destroyed stream's handle reused
+// by a new stream finds its slots idle: hipStreamDestroy completes the old
+// stream's work first.
+namespace {
+struct WsSlot {
+    void* p = nullptr;
+    size_t bytes = 0;
+    bool busy = false;
+};
+struct WsEntry {
+    int dev;
+    hipStream_t s;
+    WsSlot slot[2];
+};
+thread_local std::vector<WsEntry> t_ws;
+
+WsEntry& ws_entry(int dev, hipStream_t s) {
+    for (auto& e : t_ws)
+        if (e.dev == dev && e.s == s) return e;
+    t_ws.push_back(WsEntry{dev, s, {}});
+    return t_ws.back();
+}
+
+int pool_alloc(void** p, size_t bytes, int dev, hipStream_t s) {
+    hipMemPool_t pool;
+    {
+        std::lock_guard<std::mutex> g(g_pool_mu);
+        if (!g_pool_ready[dev]) {
+            hipMemPoolProps props;
+            std::memset(&props, 0, sizeof(props));
+            props.allocType = hipMemAllocationTypePinned;
+            props.location.type = hipMemLocationTypeDevice;
+            props.location.id = dev;
+            ARMSP_HIP(hipMemPoolCreate(&g_pools[dev], &props));
+            uint64_t thr = ~0ULL;
+            ARMSP_HIP(hipMemPoolSetAttribute(g_pools[dev], hipMemPoolAttrReleaseThreshold, &thr));
+            g_pool_ready[dev] = true;
+        }
+        pool = g_pools[dev];
+    }
+    ARMSP_HIP(hipMallocFromPoolAsync(p, bytes, pool, s));
+    return ARMSP_SUCCESS;
+}
+}  // namespace
+
 int ws_alloc(void** p, size_t bytes, hipStream_t s) {
     *p = nullptr;
     if (bytes == 0) return ARMSP_SUCCESS;
@@ -114,23 +167,29 @@ int ws_alloc(void** p, size_t bytes, hipStream_t s) {
         std::lock_guard<std::mutex> g(g_arena_mu);
         if (bytes > g_ws_high[dev]) g_ws_high[dev] = bytes;
     }
-    hipMemPool_t pool;
-    {
-        std::lock_guard<std::mutex> g(g_pool_mu);
-        if (!g_pool_ready[dev]) {
-            hipMemPoolProps props;
-            std::memset(&props, 0, sizeof(props));
-            props.allocType = hipMemAllocationTypePinned;
-            props.location.type = hipMemLocationTypeDevice;
-            props.location.id = dev;
-            ARMSP_HIP(hipMemPoolCreate(&g_pools[dev], &props));
-            uint64_t thr = ~0ULL;
-            ARMSP_HIP(hipMemPoolSetAttribute(g_pools[dev], hipMemPoolAttrReleaseThreshold, &thr));
-            g_pool_ready[dev] = true;
-        }
-        pool = g_pools[dev];
+    WsEntry& e = ws_entry(dev, s);
+    WsSlot* fit = nullptr;
+    WsSlot* idle = nullptr;
+    for (auto& sl : e.slot) {
+        if (sl.busy) continue;
+        if (sl.bytes >= bytes && (!fit || sl.bytes < fit->bytes)) fit = &sl;
+        if (!idle || sl.bytes < idle->bytes) idle = &sl;
     }
-    ARMSP_HIP(hipMallocFromPoolAsync(p, bytes, pool, s));
+    if (fit) {
+        fit->busy = true;
+        *p = fit->p;
+        return ARMSP_SUCCESS;
+    }
+    if (!idle) return pool_alloc(p, bytes, dev, s);  // both slots held: an uncached buffer
+    const size_t want = (bytes + ((size_t)1 << 20) - 1) & ~(((size_t)1 << 20) - 1);  // whole MiB
+    void* q = nullptr;
+    const int st = pool_alloc(&q, want, dev, s);
+    if (st) return st;
+    if (idle->p) hipFreeAsync(idle->p, s);
+    idle->p = q;
+    idle->bytes = want;
+    idle->busy = true;
+    *p = q;
     return ARMSP_SUCCESS;
 }
 
@@ -152,6 +211,12 @@ void ws_free(void* p, hipStream_t s) {
             g_arena_used[dev] = marks.empty() ? 0 : marks.back().second;
             return;
         }
+        for (auto& e : t_ws)  // a cached slot goes back to its entry, whatever stream the caller names
+            for (auto& sl : e.slot)
+                if (sl.busy && sl.p == p) {
+                    sl.busy = false;
+                    return;
+                }
     }
     hipFreeAsync(p, s);
 }

@@ -62,8 +62,16 @@ def _ptr(t: Optional[torch.Tensor]):
     return ctypes.c_void_p(t.data_ptr()) if t is not None else None
 
 
+_raw_stream = getattr(torch._C, "_cuda_getCurrentRawStream", None)
+
+
 def _stream(dev: torch.device):
-    return ctypes.c_void_p(torch.cuda.current_stream(dev).cuda_stream)
+    """torch's current stream on `dev` as a raw handle (the direct accessor when this
+    torch has it: about 5 us less host time per call than torch.cuda.current_stream)."""
+    idx = dev.index if dev.index is not None else torch.cuda.current_device()
+    if _raw_stream is not None:
+        return ctypes.c_void_p(_raw_stream(idx))
+    return ctypes.c_void_p(torch.cuda.current_stream(idx).cuda_stream)
 
 
 def _check_csr(fn: str, rowptr: torch.Tensor, colind: torch.Tensor, values: Optional[torch.Tensor]) -> None:

@@ -267,7 +267,9 @@ def test_gemm_bf16_mfma_layout(m, n, k):
     (20000, 256, 100, "nn"), (20011, 250, 99, "nn"), (20000, 3, 256, "nn"), (20000, 64, 37, "nn"),
     (20000, 256, 3, "nt"), (12000, 100, 128, "tn"),              # tall: B resident, A streamed once
     (100, 256, 70000, "tn"), (256, 3, 70001, "tn"), (37, 70, 66000, "nn"),   # reduction over a long K
-    (256, 256, 66000, "tn")])                                                 # ... with the largest slabs
+    (256, 256, 66000, "tn"),                                                  # ... with the largest slabs
+    (19717, 64, 500, "nn"), (19717, 64, 500, "nt"), (8000, 32, 1000, "tn"),   # tall, B of up to 1024 k in LDS
+    (500, 64, 19717, "tn"), (64, 3, 19717, "tn"), (500, 64, 19717, "nt")])    # split-K tiles: xᵀ·dY at PubMed size
 def test_gemm_bf16_tall_and_reduction(m, n, k, layout):
     """The GCN layer's shapes: x·W (tall), the gradient back to x (tall, Wᵀ view) and
     xᵀ·dY (reduction, xᵀ view); against an FP64 GEMM of the same bf16 inputs."""
@@ -286,7 +288,28 @@ def test_gemm_bf16_tall_and_reduction(m, n, k, layout):
     assert bool(((c.cpu().double() - ref - 2).abs() <= TOL * (bound + 2) + 1e-30).all())
 
 
-@pytest.mark.parametrize("m,n,k", [(20000, 256, 100), (20000, 3, 256), (37, 70, 50), (100, 256, 70000)])
+@pytest.mark.parametrize("m,n,k", [(19717, 64, 500), (500, 64, 19717), (130, 70, 50)])
+def test_gemm_bf16_unaligned_views(m, n, k):
+    """Operands at an odd element offset (not 16-byte aligned) and with odd strides
+    take the element-load staging; same FP64 bound as the aligned layouts."""
+    case(f"gemm unaligned {m}x{n}x{k}")
+    r = np.random.default_rng(m + 5 * n + 11 * k)
+    a = torch.from_numpy(r.standard_normal((m, k)).astype(np.float32)).to(torch.bfloat16)
+    b = torch.from_numpy(r.standard_normal((k, n)).astype(np.float32)).to(torch.bfloat16)
+    ref = a.double() @ b.double()
+    bound = a.double().abs() @ b.double().abs()
+    abuf = torch.zeros(m * (k + 3) + 1, dtype=torch.bfloat16)
+    bbuf = torch.zeros((k + 1) * n + 1, dtype=torch.bfloat16)
+    abuf[1:].as_strided((m, k), (k + 3, 1)).copy_(a)      # row pitch k+3, offset 1
+    bbuf[1:].as_strided((k, n), (1, k + 1)).copy_(b)      # column-major, pitch k+1, offset 1
+    da = abuf.cuda()[1:].as_strided((m, k), (k + 3, 1))
+    db = bbuf.cuda()[1:].as_strided((k, n), (1, k + 1))
+    out = kernels.gemm_bf16(da, db).cpu().double()
+    assert bool(((out - ref).abs() <= TOL * bound + 1e-30).all())
+
+
+@pytest.mark.parametrize("m,n,k", [(20000, 256, 100), (20000, 3, 256), (37, 70, 50), (100, 256, 70000),
+                                   (19717, 64, 500)])
 def test_gemm_bf16_output(m, n, k):
     """bf16 output = the fp32 product rounded to nearest even (tall kernel writes it
     directly, other shapes round a workspace)."""

@@ -1,0 +1,95 @@
+"""The library's workspaces outside graph capture come from a cache per host thread
+and (device, stream) that keeps its buffers between calls.  These tests drive the
+cache through growth, reuse, a second stream and two host threads, and check every
+result bit for bit against the same call made alone on a quiet device (the merge
+schedules are deterministic), and the ROW schedule against the oracle."""
+import threading
+
+import numpy as np
+import pytest
+import torch
+
+import oracle
+from armsparse_amd import data, kernels
+
+pytestmark = pytest.mark.gpu
+
+
+def dev(*arrays):
+    return [torch.from_numpy(np.ascontiguousarray(a)).cuda() for a in arrays]
+
+
+def case(msg):
+    print(f"[case] {msg}", flush=True)
+
+
+def _problems():
+    out = []
+    for i, (scale, k, n) in enumerate([(10, 1 << 12, 32), (13, 1 << 15, 64), (8, 1 << 10, 16), (14, 1 << 16, 128)]):
+        a = data.rmat_csr(scale, k, seed=40 + i)
+        b = np.random.default_rng(i).uniform(-1, 1, (a.shape[1], n)).astype(np.float32)
+        out.append((a, b))
+    return out
+
+
+def _spmm(t, schedule):
+    rp, ci, va, b = t
+    return kernels.spmm(rp, ci, va, b, config=schedule)
+
+
+def test_workspace_grows_shrinks_and_reuses():
+    """Sizes up, down and up again on one stream: every result is bit-identical to the
+    first call of its size, and the ROW schedule matches the oracle."""
+    case("workspace cache: grow, shrink, reuse")
+    probs = _problems()
+    tens = [tuple(dev(a.rowptr, a.colind, a.values, b)) for a, b in probs]
+    first = {}
+    for rep in range(3):
+        for i in (0, 1, 2, 3, 2, 0):
+            c = _spmm(tens[i], "merge").cpu()
+            first.setdefault(i, c)
+            assert torch.equal(c, first[i]), f"problem {i} repeat {rep}"
+    for (a, b), (rp, ci, va, db) in zip(probs, tens):
+        c = torch.zeros(a.shape[0], b.shape[1], device="cuda")
+        kernels.spmm(rp, ci, va, db, out=c, accumulate=True, config="row")
+        expect = oracle.rowBaseSpmm3(a.rowptr, a.colind, a.values, b, np.zeros((a.shape[0], b.shape[1]), np.float32))
+        assert np.array_equal(c.cpu().numpy().view(np.uint32), expect.view(np.uint32))
+
+
+def _run_on_stream(tens, order, stream, outs, errors):
+    try:
+        with torch.cuda.stream(stream):
+            for i in order:
+                outs.append((i, _spmm(tens[i], "merge")))
+            stream.synchronize()
+    except Exception as e:  # reported by the main thread
+        errors.append(e)
+
+
+@pytest.mark.parametrize("shared_stream", [False, True])
+def test_workspace_two_host_threads(shared_stream):
+    """Two host threads enqueue SpMMs at once, on their own streams or on one shared
+    stream; each thread's workspaces are its own, so every result equals the lone call."""
+    case(f"workspace cache: two threads, shared stream {shared_stream}")
+    probs = _problems()
+    tens = [tuple(dev(a.rowptr, a.colind, a.values, b)) for a, b in probs]
+    alone = [_spmm(t, "merge").cpu() for t in tens]
+    torch.cuda.synchronize()
+    s1 = torch.cuda.Stream()
+    s2 = s1 if shared_stream else torch.cuda.Stream()
+    outs1, outs2, errors = [], [], []
+    order1 = [0, 1, 2, 3] * 6
+    order2 = [3, 2, 1, 0] * 6
+    t1 = threading.Thread(target=_run_on_stream, args=(tens, order1, s1, outs1, errors))
+    t2 = threading.Thread(target=_run_on_stream, args=(tens, order2, s2, outs2, errors))
+    t1.start()
+    t2.start()
+    t1.join(timeout=120)
+    t2.join(timeout=120)
+    assert not t1.is_alive() and not t2.is_alive()
+    assert not errors, errors
+    torch.cuda.synchronize()
+    for outs in (outs1, outs2):
+        assert len(outs) == 24
+        for i, c in outs:
+            assert torch.equal(c.cpu(), alone[i]), f"problem {i}"
