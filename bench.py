@@ -134,6 +134,9 @@ def host_info():
             "affinity_cpus": len(os.sched_getaffinity(0)), "cgroup_cpu_limit": quota}
 
 
+_HOST = None  # host_info() as main() found it at start
+
+
 def parse():
     p = argparse.ArgumentParser()
     p.add_argument("--gpus", type=int, default=1)
@@ -211,7 +214,7 @@ def cpu_baselines(args, A, B, headline_bytes):
                       f"{head['calls']} timed calls, mean {head['mean_ms']} ms, median {head['median_ms']} ms",
             "median_GBps": round(headline_bytes / (head["median_ms"] / 1e3) / 1e9, 2),
             "omp": {k: os.environ.get(k) for k in ("OMP_NUM_THREADS", "OMP_PROC_BIND", "OMP_PLACES")},
-            "host": host_info(), "others": others}
+            "host": _HOST or host_info(), "others": others}
 
 
 def _cpu_gcn_epochs(oracle, threads, seconds):
@@ -285,6 +288,8 @@ def main():
     # runtime starts, so set before anything loads one)
     os.environ.setdefault("OMP_PROC_BIND", "spread")
     os.environ.setdefault("OMP_PLACES", "cores")
+    global _HOST
+    _HOST = host_info()  # before an OpenMP runtime binds this thread to one core
     import numpy as np
     import torch
 
