@@ -143,8 +143,8 @@ def parse():
     p.add_argument("--steps", type=int, default=50)
     p.add_argument("--warmup", type=int, default=5)
     p.add_argument("--only", default="", help="comma list: spmm,sddmm,spmv,mixed,widths,shards,small,gcn,gcn_scaled,cpu (default all); "
-                        "regimes (column-placement variants of the headline matrix) and probe (the B-row gathers "
-                        "alone on those column patterns) run only when named")
+                        "regimes (column-placement variants of the headline matrix), probe (the B-row gathers "
+                        "alone on those column patterns) and spmv_variants (SpMV schedules) run only when named")
     p.add_argument("--schedule", default="", help="override the SpMM schedule (row|nnz|merge)")
     p.add_argument("--chunk", type=int, default=0)
     p.add_argument("--unroll", type=int, default=0)
@@ -319,11 +319,14 @@ def main():
         cfg = _lib.Config.make(args.schedule or "merge", chunk=args.chunk, unroll=args.unroll)
 
     def timed(fn, steps, warmup, families):
+        """Wall seconds of `steps` calls; with `families`, the library's HIP-event
+        kernel times too (recording those events costs host time per launch, so it
+        is switched on only when asked for)."""
         for _ in range(warmup):
             fn()
         torch.cuda.synchronize()
         _lib.lib.armsp_timing_reset()
-        _lib.lib.armsp_timing_enable(1)
+        _lib.lib.armsp_timing_enable(1 if families else 0)
         torch.cuda.synchronize()
         t = time.perf_counter()
         for _ in range(steps):
@@ -531,6 +534,21 @@ def main():
             o6 = kernels.sddmm(r6, c6, U6, U6, scale=v6).cpu().numpy()
             U6h = U6.cpu().numpy()
             errs[f"sddmm_65k_kd{kd}"] = sddmm_rows_err(A6.rowptr, A6.colind, A6.values, U6h, U6h, o6, rows6)
+    if "spmv_variants" in only:  # not in the default set: SpMV schedules side by side
+        x = torch.empty(K, device=dev)
+        kernels.fill_uniform_(x, data.CFG_SEED + 1, data.STREAM_DENSE + 1)
+        y = torch.empty(M, device=dev)
+        for name, vcfg in (("row", "row"), ("merge_ipt4", dict(schedule="merge", chunk=256)),
+                           ("merge_ipt8", dict(schedule="merge", chunk=512)),
+                           ("merge_ipt16", dict(schedule="merge", chunk=1024))):
+            n_v = 10 if name == "row" else 40
+            el_v, fam_v = timed(lambda: kernels.spmv(rp, ci, va, x, out=y, config=vcfg), n_v, 5,
+                                ["spmv", "merge_partition", "spmv_fixup"])
+            per = {f: round(ms / max(c, 1) * 1e3, 1) for f, (ms, c) in fam_v.items()}
+            extras[f"spmv_{name}_us"] = round(el_v / n_v * 1e6, 1)
+            log(f"[bench] spmv {name}: {el_v / n_v * 1e6:.1f} us/call, kernels {per}")
+        del x, y
+
     if "spmv" in only:
         x = torch.empty(K, device=dev)
         kernels.fill_uniform_(x, data.CFG_SEED + 1, data.STREAM_DENSE + 1)
