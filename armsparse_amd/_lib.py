@@ -109,6 +109,7 @@ _SIGS = {
     "armsp_gemm_bf16": (c_int, [_I, _I, _I, _P, c_int64, c_int64, _P, c_int64, c_int64, _P, c_int64, _I, _P]),
     "armsp_gemm_bf16_ex": (c_int, [_I, _I, _I, _P, c_int64, c_int64, _P, c_int64, c_int64, _P, _I, c_int64, _I, _P]),
     "armsp_convert_f32_to_bf16": (c_int, [c_int64, _P, _P, _P]),
+    "armsp_convert_rows_f32_to_bf16": (c_int, [_I, _I, _P, c_int64, _P, c_int64, _P]),
     "armsp_quantize_fp8_rows": (c_int, [_I, _I, _P, _I, _P, _P, _P]),
     "armsp_dequantize_fp8_rows": (c_int, [_I, _I, _P, _I, _P, _P, _I, _P]),
     "armsp_fill_uniform_f32": (c_int, [c_int64, c_uint64, c_uint64, c_float, c_float, _P, _P]),
@@ -121,7 +122,7 @@ _SIGS = {
     "armsp_relu_backward": (c_int, [c_int64, _P, _P, _P]),
     "armsp_colsum_f32": (c_int, [_I, _I, _P, _P, _P]),
     "armsp_bias_relu_dropout_bf16": (c_int, [_I, _I, _P, c_int64, _P, c_float, _P, _P, _P, _P]),
-    "armsp_bias_relu_dropout_backward_bf16": (c_int, [_I, _I, _P, _P, c_float, _P, _P, _P]),
+    "armsp_bias_relu_dropout_backward_bf16": (c_int, [_I, _I, _P, _P, c_float, _P, c_int64, _P, _P]),
     "armsp_adam_step": (c_int, [c_int64, _P, _P, _P, _P, c_float, c_float, c_float, c_float, c_float, _I, _P]),
     "armsp_probe_gather_f32": (c_int, [_I, _P, _P, _I, _I, _I, _I, _I, _P, _P]),
     "armsp_selector_set": (c_int, [_I, _I, _I, _I, _P, _P, _P, _P, _P, _P, _P, _P, _P]),

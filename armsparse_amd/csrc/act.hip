@@ -83,7 +83,8 @@ __global__ void k_zero(int n, float* __restrict__ x) {
 // the phases' fp32 sums joined in LDS in phase order into part[b][·].
 __global__ __launch_bounds__(256) void k_act_backward(int M, int N, int R, const uint16_t* __restrict__ G,
                                                       const uint8_t* __restrict__ mask, float scale,
-                                                      uint16_t* __restrict__ GY, float* __restrict__ part) {
+                                                      uint16_t* __restrict__ GY, int64_t ldgy,
+                                                      float* __restrict__ part) {
     extern __shared__ float s_acc[];  // [P][N]
     const int ipr = N >> 3;
     const int P = 256 / ipr;
@@ -110,7 +111,7 @@ __global__ __launch_bounds__(256) void k_act_backward(int M, int N, int R, const
                 else
                     o[j >> 1] = hb;
             }
-            *reinterpret_cast<uint4*>(GY + it * 8) = make_uint4(o[0], o[1], o[2], o[3]);
+            *reinterpret_cast<uint4*>(GY + m * ldgy + cg * 8) = make_uint4(o[0], o[1], o[2], o[3]);
         }
         if (part) {
 #pragma unroll
@@ -186,11 +187,11 @@ int armsp_bias_relu_dropout_bf16(int M, int N, const float* Y, int64_t ldy, cons
 }
 
 int armsp_bias_relu_dropout_backward_bf16(int M, int N, const uint16_t* G, const uint8_t* mask, float p,
-                                          uint16_t* GY, float* dbias, void* stream) {
+                                          uint16_t* GY, int64_t ldgy, float* dbias, void* stream) {
     using namespace armsp;
     if (M < 0 || N < 0 || !(p >= 0.0f && p < 1.0f)) return ARMSP_ERROR_INVALID_VALUE;
     if (N == 0) return ARMSP_SUCCESS;
-    if (N % 8 || N > 2048) return ARMSP_ERROR_INVALID_VALUE;
+    if (N % 8 || N > 2048 || ldgy < N || ldgy % 8) return ARMSP_ERROR_INVALID_VALUE;
     hipStream_t s = as_stream(stream);
     if (M == 0) {  // a fill kernel, not a memset node, so that the call stays capturable
         if (dbias) {
@@ -213,7 +214,8 @@ int armsp_bias_relu_dropout_backward_bf16(int M, int N, const uint16_t* G, const
         if (st) return st;
     }
     float* part = static_cast<float*>(ws);
-    hipLaunchKernelGGL(k_act_backward, dim3((unsigned)nb), dim3(256), lds, s, M, N, R, G, mask, scale, GY, part);
+    hipLaunchKernelGGL(k_act_backward, dim3((unsigned)nb), dim3(256), lds, s, M, N, R, G, mask, scale, GY, ldgy,
+                       part);
     hipError_t e = hipGetLastError();
     if (e == hipSuccess && dbias) {
         hipLaunchKernelGGL(k_parts_sum, dim3((unsigned)((N + 15) / 16)), dim3(256), 0, s, nb, N, part, dbias);

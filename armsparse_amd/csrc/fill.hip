@@ -43,6 +43,15 @@ __global__ void k_to_bf16(int64_t n, const float* __restrict__ x, uint16_t* __re
         y[i] = f32_to_bf16_rne(x[i]);
 }
 
+// rows of N floats at pitch ldx → rows of N bf16 at pitch ldy
+__global__ void k_rows_to_bf16_2d(int64_t total, int N, const float* __restrict__ x, int64_t ldx,
+                                  uint16_t* __restrict__ y, int64_t ldy) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < total; i += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t m = i / N, n = i - m * N;
+        y[m * ldy + n] = f32_to_bf16_rne(x[m * ldx + n]);
+    }
+}
+
 // fp32 → OCP e4m3fn, round to nearest even, saturating at ±448.
 __device__ __forceinline__ uint32_t f32_to_e4m3(float x) {
     const uint32_t bits = __float_as_uint(x);
@@ -137,6 +146,18 @@ int armsp_convert_f32_to_bf16(int64_t n, const float* x, uint16_t* y, void* stre
     if (n < 0 || (n > 0 && (!x || !y))) return ARMSP_ERROR_INVALID_VALUE;
     if (n == 0) return ARMSP_SUCCESS;
     hipLaunchKernelGGL(k_to_bf16, grid_for(n), dim3(256), 0, as_stream(stream), n, x, y);
+    ARMSP_LAUNCHED();
+    return ARMSP_SUCCESS;
+}
+
+int armsp_convert_rows_f32_to_bf16(int M, int N, const float* x, int64_t ldx, uint16_t* y, int64_t ldy,
+                                   void* stream) {
+    using namespace armsp;
+    if (M < 0 || N < 0 || ldx < N || ldy < N) return ARMSP_ERROR_INVALID_VALUE;
+    if (M == 0 || N == 0) return ARMSP_SUCCESS;
+    if (!x || !y) return ARMSP_ERROR_INVALID_VALUE;
+    const int64_t total = (int64_t)M * N;
+    hipLaunchKernelGGL(k_rows_to_bf16_2d, grid_for(total), dim3(256), 0, as_stream(stream), total, N, x, ldx, y, ldy);
     ARMSP_LAUNCHED();
     return ARMSP_SUCCESS;
 }

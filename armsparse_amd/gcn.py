@@ -70,12 +70,12 @@ def _adam(model: Net, capturable: bool):
 
 
 def line_pitched(x: torch.Tensor) -> torch.Tensor:
-    """x [n×f] as a view whose rows start on 128-byte lines (pitch rounded up to 128
-    bytes, the padding zero), so the aggregation's gathers of a row touch no more
-    lines than the row needs."""
+    """x [n×f] as a view with rows at kernels.row_pitch (the padding zero): rows of a
+    line or more start on 128-byte lines, so the aggregation's gathers of a row touch
+    no more lines than the row needs, and span an odd number of lines, which the
+    MI355X's memory system serves faster than a pitch that is a multiple of 256 bytes."""
     n, f = x.shape
-    per_line = 128 // x.element_size()
-    fp = (f + per_line - 1) // per_line * per_line
+    fp = kernels.row_pitch(f, x.element_size())
     if fp == f and x.is_contiguous():
         return x
     buf = torch.zeros((n, fp), dtype=x.dtype, device=x.device)

@@ -419,19 +419,21 @@ def bias_relu_dropout(y: torch.Tensor, bias: Optional[torch.Tensor], p: float,
     return h, mask
 
 
-def bias_relu_dropout_backward(g: torch.Tensor, mask: torch.Tensor, p: float, want_dbias: bool) -> tuple:
+def bias_relu_dropout_backward(g: torch.Tensor, mask: torch.Tensor, p: float, want_dbias: bool,
+                               pitched: bool = False) -> tuple:
     """(gy, dbias) of bias_relu_dropout for the bf16 gradient g of h: gy =
     bf16(mask ? g/(1-p) : 0), dbias = the fp32 column sums of the same values
-    (deterministic), or None."""
+    (deterministic), or None.  ``pitched``: gy's rows at row_pitch (for an SpMM
+    that gathers them)."""
     fn = "bias_relu_dropout_backward"
     g = g.contiguous()
     _require(g.is_cuda and g.dtype == torch.bfloat16 and g.dim() == 2, f"{fn}: needs a 2-D bfloat16 device tensor")
     m, n = g.shape
     _require(mask.dtype == torch.uint8 and mask.shape == (m, n // 8) and mask.is_contiguous(), f"{fn}: bad mask")
-    gy = torch.empty_like(g)
+    gy = empty_rows(m, n, torch.bfloat16, g.device) if pitched else torch.empty_like(g)
     dbias = torch.empty(n, device=g.device) if want_dbias else None
-    call("armsp_bias_relu_dropout_backward_bf16", m, n, _ptr(g), _ptr(mask), float(p), _ptr(gy), _ptr(dbias),
-         _stream(g.device))
+    call("armsp_bias_relu_dropout_backward_bf16", m, n, _ptr(g), _ptr(mask), float(p), _ptr(gy), gy.stride(0),
+         _ptr(dbias), _stream(g.device))
     return gy, dbias
 
 
@@ -453,13 +455,47 @@ def workspace_reserve(nbytes: int = 0, device: Optional[torch.device] = None) ->
     return r.value
 
 
-def to_bf16(x: torch.Tensor) -> torch.Tensor:
-    """float32 → bfloat16 (round to nearest even) with the library kernel."""
+def to_bf16(x: torch.Tensor, pitched: bool = False) -> torch.Tensor:
+    """float32 → bfloat16 (round to nearest even) with the library kernel;
+    ``pitched`` (2-D x with unit column stride): the result's rows at row_pitch,
+    for an SpMM that gathers them."""
     _require(x.is_cuda and x.dtype == torch.float32, "to_bf16: needs a float32 device tensor")
+    if pitched:
+        _require(x.dim() == 2 and x.stride(1) == 1, "to_bf16: pitched output needs a 2-D x with unit column stride")
+        m, n = x.shape
+        y = empty_rows(m, n, torch.bfloat16, x.device)
+        call("armsp_convert_rows_f32_to_bf16", m, n, _ptr(x), x.stride(0), _ptr(y), y.stride(0), _stream(x.device))
+        return y
     x = x.contiguous()
     y = torch.empty(x.shape, dtype=torch.bfloat16, device=x.device)
     call("armsp_convert_f32_to_bf16", x.numel(), _ptr(x), _ptr(y), _stream(x.device))
     return y
+
+
+def row_pitch(n: int, element_size: int) -> int:
+    """Row pitch (in elements) for an operand whose rows an SpMM or SDDMM gathers.
+    Rows shorter than a 128-byte line stay packed (several share a line).  Longer
+    rows start on 128-byte lines, and a row of two or more lines spans an odd
+    number of them.  Measured on the MI355X (1M R-MAT, 2^24 nnz): rows on a pitch
+    that is a multiple of 256 bytes gather 11-24% slower (fp32 N=128: 1011 us at
+    512 bytes, 841 us at 640; fp32 N=256: 2240 us at 1024 bytes, 1728 us at 1152;
+    bf16 N=256: 1091 us at 512 bytes, 968 us at 640)."""
+    _require(n >= 0 and element_size in (1, 2, 4, 8), "row_pitch: bad arguments")
+    if n * element_size < 128:
+        return n
+    lines = (n * element_size + 127) // 128
+    if lines >= 2 and lines % 2 == 0:
+        lines += 1
+    return lines * 128 // element_size
+
+
+def empty_rows(m: int, n: int, dtype: torch.dtype, device) -> torch.Tensor:
+    """An uninitialised [m×n] view with rows at row_pitch (the padding is never read)."""
+    esz = torch.empty((), dtype=dtype).element_size()
+    p = row_pitch(n, esz)
+    if p == n:
+        return torch.empty((m, n), dtype=dtype, device=device)
+    return torch.empty((m, p), dtype=dtype, device=device)[:, :n]
 
 
 def quantize_fp8_rows(b: torch.Tensor):

@@ -119,7 +119,7 @@ class GCNAggregate(torch.autograd.Function):
         g = gy.contiguous()
         dh = dcoef = None
         if ctx.needs_input_grad[0]:
-            gb = kernels.to_bf16(g.float()) if hdt == torch.bfloat16 else g
+            gb = kernels.to_bf16(g.float(), pitched=True) if hdt == torch.bfloat16 else g
             dh = kernels.spmm(colptr, rowind, coef_bwd, gb, row_scale=out_norm).to(hdt)
         if ctx.edge_grad is not None:
             dcoef = _edge_coef_grad(g, *ctx.edge_grad)
@@ -161,7 +161,10 @@ class _BF16Linear(torch.autograd.Function):
         wb = kernels.to_bf16(weight.detach())
         ctx.save_for_backward(xb, wb)
         ctx.x_dtype = x.dtype
-        return kernels.gemm_bf16(xb, wb, out_dtype=torch.bfloat16 if out_bf16 else torch.float32)
+        if out_bf16:  # bf16 features for the aggregation's gathers: rows at the gather pitch
+            y = kernels.empty_rows(xb.shape[0], wb.shape[1], torch.bfloat16, xb.device)
+            return kernels.gemm_bf16(xb, wb, out=y, out_dtype=torch.bfloat16)
+        return kernels.gemm_bf16(xb, wb)
 
     @staticmethod
     def backward(ctx, gy):
@@ -229,7 +232,8 @@ class _AggregateBiasReluDropout(torch.autograd.Function):
         (mask,) = ctx.saved_tensors
         colptr, rowind, coef_bwd, out_norm, hdt = ctx.graph
         gyb, dbias = kernels.bias_relu_dropout_backward(gout.to(torch.bfloat16), mask, ctx.p,
-                                                        ctx.has_bias and ctx.needs_input_grad[9])
+                                                        ctx.has_bias and ctx.needs_input_grad[9],
+                                                        pitched=ctx.needs_input_grad[0])
         dh = dcoef = None
         if ctx.needs_input_grad[0]:
             dh = kernels.spmm(colptr, rowind, coef_bwd, gyb, row_scale=out_norm).to(hdt)

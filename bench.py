@@ -388,6 +388,22 @@ def main():
         extras["spmm_native_selector_main_kernel_us"] = round(fam_s["spmm"][0] / max(fam_s["spmm"][1], 1) * 1e3, 2)
         extras["spmm_native_selector_GBps"] = round(nbytes * max(10, args.steps // 2) / el_s / 1e9, 1)
 
+    if cfg is None:  # the same B at the gather pitch (kernels.row_pitch: 640-byte rows, the same values)
+        pb = kernels.row_pitch(N, 4)
+        Bp = torch.empty((K, pb), device=dev)[:, :N]
+        Bp.copy_(B)
+        Cp = torch.empty_like(C)
+        el_p, fam_p = timed(lambda: kernels.spmm(rp, ci, va, Bp, out=Cp), max(10, args.steps // 2), 2,
+                            ["spmm", "spmm_fixup"])
+        kernels.spmm(rp, ci, va, B, out=C)  # AUTO picks the same kernel for both layouts: the same bits
+        assert torch.equal(Cp, C), "the pitched B must give the same C"
+        extras["spmm_pitch_bytes"] = pb * 4
+        extras["spmm_pitched_B_main_kernel_us"] = round(fam_p["spmm"][0] / max(fam_p["spmm"][1], 1) * 1e3, 2)
+        extras["spmm_pitched_B_GBps"] = round(nbytes * max(10, args.steps // 2) / el_p / 1e9, 1)
+        log(f"[bench] spmm fp32 with B rows at a {pb * 4}-byte pitch: main kernel "
+            f"{extras['spmm_pitched_B_main_kernel_us']} us, {extras['spmm_pitched_B_GBps']} GB/s end-to-end")
+        del Bp, Cp
+
     if "mixed" in only:
         Bb = kernels.to_bf16(B)
         el_b, fam_b = timed(lambda: kernels.spmm(rp, ci, va, Bb, out=C, config=cfg), steps_x, 3, ["spmm"])
@@ -611,6 +627,21 @@ def main():
                 errs["sddmm_kd128_split"] = sddmm_rows_err(
                     A.rowptr, A.colind, A.values, Uh, Vh, kernels.sddmm(rp, ci, U, V, scale=va, config=split).cpu().numpy(),
                     rows)
+                # U and V rows at the gather pitch (kernels.row_pitch), the same values
+                pk = kernels.row_pitch(kd, 4)
+                Up = torch.empty((M, pk), device=dev)[:, :kd]
+                Vp = torch.empty((K, pk), device=dev)[:, :kd]
+                Up.copy_(U)
+                Vp.copy_(V)
+                for tag, cf in (("", None), ("_split", split)):
+                    el_p, _ = timed(lambda: kernels.sddmm(rp, ci, Up, Vp, scale=va, config=cf), steps_x, 3, [])
+                    extras[f"sddmm_kd128{tag}_pitched_us"] = round(el_p / steps_x * 1e6, 1)
+                    errs[f"sddmm_kd128{tag}_pitched"] = sddmm_rows_err(
+                        A.rowptr, A.colind, A.values, Uh, Vh,
+                        kernels.sddmm(rp, ci, Up, Vp, scale=va, config=cf).cpu().numpy(), rows)
+                log(f"[bench] sddmm Kd=128 with U, V rows at a {pk * 4}-byte pitch: "
+                    f"{extras['sddmm_kd128_pitched_us']} us, split {extras['sddmm_kd128_split_pitched_us']} us")
+                del Up, Vp
             del U, V
     if "gcn" in only:
         from armsparse_amd import gcn

@@ -308,6 +308,11 @@ int armsp_gemm_bf16_ex(int M, int N, int Kd,
                        void* C, int c_dtype, int64_t ldc, int accumulate, void* stream);
 /* fp32 → bf16 (round to nearest even), n elements. */
 int armsp_convert_f32_to_bf16(int64_t n, const float* x, uint16_t* y, void* stream);
+/* The same for an M×N matrix: rows of pitch ldx in, rows of pitch ldy out (a
+ * bf16 operand laid out for the SpMM gathers, e.g. an odd number of 128-byte
+ * lines per row). */
+int armsp_convert_rows_f32_to_bf16(int M, int N, const float* x, int64_t ldx, uint16_t* y, int64_t ldy,
+                                   void* stream);
 /* Per-row fp8-e4m3 quantisation: scale[k] = max|B[k,:]| / 448 (1 if the row
  * is zero), Bq = RNE(B / scale) saturated to ±448. */
 int armsp_quantize_fp8_rows(int K, int N, const float* B, int ldb,
@@ -360,11 +365,13 @@ int armsp_colsum_f32(int M, int N, const float* X, float* out, void* stream);
  * ldy % 4 == 0, Y, H and bias 16-byte aligned; bias may be NULL; 0 <= p < 1. */
 int armsp_bias_relu_dropout_bf16(int M, int N, const float* Y, int64_t ldy, const float* bias, float p,
                                  uint64_t* seed, uint16_t* H, uint8_t* mask, void* stream);
-/* Its backward: GY = bf16(mask ? G/(1-p) : 0) (G, GY contiguous [M×N] bf16)
- * and, when dbias is not NULL, dbias[n] = Σ_m of the same fp32 values, summed
- * in a fixed order (deterministic).  N % 8 == 0, N <= 2048. */
+/* Its backward: GY = bf16(mask ? G/(1-p) : 0) (G contiguous [M×N] bf16, GY
+ * rows of pitch ldgy, a multiple of 8, so that the gradient can be written in
+ * the layout the next SpMM gathers best) and, when dbias is not NULL,
+ * dbias[n] = Σ_m of the same fp32 values, summed in a fixed order
+ * (deterministic).  N % 8 == 0, N <= 2048. */
 int armsp_bias_relu_dropout_backward_bf16(int M, int N, const uint16_t* G, const uint8_t* mask, float p,
-                                          uint16_t* GY, float* dbias, void* stream);
+                                          uint16_t* GY, int64_t ldgy, float* dbias, void* stream);
 /* One Adam step (L2 weight decay added to the gradient); step counts from 1. */
 int armsp_adam_step(int64_t n, float* param, const float* grad, float* m, float* v, float lr, float beta1,
                     float beta2, float eps, float weight_decay, int step, void* stream);

@@ -303,6 +303,9 @@ def test_bias_relu_dropout_kernels(n):
     assert ((db.double() - db64).abs().max() / db64.abs().max()).item() <= 1e-5
     gy2, db2 = kernels.bias_relu_dropout_backward(g, mask, p, True)
     assert torch.equal(gy2, gy) and torch.equal(db2, db)
+    # the same gradient written at the gather pitch
+    gyp, dbp = kernels.bias_relu_dropout_backward(g, mask, p, True, pitched=True)
+    assert gyp.stride(0) == kernels.row_pitch(n, 2) and torch.equal(gyp, gy) and torch.equal(dbp, db)
     # no rows: a zero bias gradient; N not a multiple of 8 is refused
     _, dbz = kernels.bias_relu_dropout_backward(g[:0], mask[:0], p, True)
     assert torch.equal(dbz, torch.zeros(n, device="cuda"))
@@ -454,3 +457,17 @@ def test_library_calls_replay_in_a_hip_graph():
         torch.cuda.synchronize()
         for s_, e_ in zip(static, eager):
             assert torch.equal(s_, e_), i
+
+
+@pytest.mark.parametrize("m,n", [(3001, 256), (1000, 100), (77, 64), (5, 3)])
+def test_to_bf16_pitched_rows(m, n):
+    """The pitched fp32 → bf16 conversion equals the flat one bit for bit, from a
+    contiguous or a strided source, with rows at row_pitch."""
+    print(f"[case] to_bf16 pitched {m}x{n}", flush=True)
+    torch.manual_seed(m + n)
+    x = torch.randn(m, n + 5, device="cuda")[:, :n]
+    y = kernels.to_bf16(x, pitched=True)
+    assert y.shape == (m, n) and y.stride() == (kernels.row_pitch(n, 2), 1)
+    assert torch.equal(y, kernels.to_bf16(x.contiguous()))
+    assert torch.equal(y, x.to(torch.bfloat16))
+

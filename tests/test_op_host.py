@@ -32,7 +32,28 @@ def test_line_pitched_features():
     from armsparse_amd.gcn import line_pitched
     x = torch.arange(6 * 100, dtype=torch.float32).reshape(6, 100).to(torch.bfloat16)
     v = line_pitched(x)
-    assert v.shape == (6, 100) and v.stride() == (128, 1) and torch.equal(v, x)
-    assert v.untyped_storage().nbytes() == 6 * 128 * 2  # rows on 128-byte lines, zero padding
+    # 200-byte rows: two lines, padded to an odd number of lines (three, 384 bytes)
+    assert v.shape == (6, 100) and v.stride() == (192, 1) and torch.equal(v, x)
+    assert v.untyped_storage().nbytes() == 6 * 192 * 2
+    base = v.untyped_storage()
+    pad = torch.tensor([], dtype=torch.bfloat16).set_(base, 0, (6, 192), (192, 1))[:, 100:]
+    assert not pad.any()  # the padding is zero
     y = torch.zeros(3, 64, dtype=torch.bfloat16)
-    assert line_pitched(y) is y  # already 128 bytes a row
+    assert line_pitched(y) is y  # exactly one 128-byte line a row
+
+
+def test_row_pitch_rule():
+    """Rows shorter than a line stay packed; longer rows start on 128-byte lines and
+    span an odd number of them."""
+    from armsparse_amd.kernels import row_pitch
+    cases = {(3, 2): 3, (3, 4): 3, (31, 4): 31, (64, 2): 64, (32, 4): 32,   # packed or one line
+             (100, 2): 192, (128, 2): 192, (64, 4): 96,                      # 2 lines -> 3
+             (128, 4): 160, (256, 2): 320,                                   # 4 lines -> 5
+             (256, 4): 288, (512, 2): 576, (160, 4): 160, (100, 4): 160,     # 8 -> 9; 5 stays; 400 B -> 4 -> 5 lines
+             (0, 4): 0}
+    for (n, esz), want in cases.items():
+        assert row_pitch(n, esz) == want, (n, esz)
+        p = row_pitch(n, esz)
+        assert p >= n
+        if n * esz >= 128:
+            assert (p * esz) % 128 == 0 and ((p * esz) // 128 == 1 or (p * esz) // 128 % 2 == 1)
