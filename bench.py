@@ -144,7 +144,8 @@ def parse():
     p.add_argument("--warmup", type=int, default=5)
     p.add_argument("--only", default="", help="comma list: spmm,sddmm,spmv,mixed,widths,shards,small,gcn,gcn_scaled,cpu (default all); "
                         "regimes (column-placement variants of the headline matrix), probe (the B-row gathers "
-                        "alone on those column patterns) and spmv_variants (SpMV schedules) run only when named")
+                        "alone on those column patterns), pitches (B at several row pitches) and spmv_variants "
+                        "(SpMV schedules) run only when named")
     p.add_argument("--schedule", default="", help="override the SpMM schedule (row|nnz|merge)")
     p.add_argument("--chunk", type=int, default=0)
     p.add_argument("--unroll", type=int, default=0)
@@ -457,7 +458,34 @@ def main():
                 extras[f"probe_{pname}_g{grp}v{vec}_us"] = round(us, 1)
                 extras[f"probe_{pname}_g{grp}v{vec}_TBps"] = round(nnz * N * 4 / us / 1e6, 2)
                 log(f"[bench] gather probe {pname} group {grp} vec {vec}: {us:.1f} us, {nnz * N * 4 / us / 1e6:.2f} TB/s")
+            if pname in ("rmat", "rmat_oddmul"):  # the same gathers with B rows at the gather pitch
+                pb = kernels.row_pitch(N, 4)
+                Bp = torch.empty((K, pb), device=dev)
+                callp = lambda: _lib.call("armsp_probe_gather_f32", nnz, kernels._ptr(di), kernels._ptr(Bp), K, pb,  # noqa: E731
+                                          N, 32, 2, kernels._ptr(pout), kernels._stream(dev))
+                _, fam_p = timed(callp, 10, 2, ["probe"])
+                us = fam_p["probe"][0] / max(fam_p["probe"][1], 1) * 1e3
+                extras[f"probe_{pname}_g32v2_pitch{pb * 4}_us"] = round(us, 1)
+                log(f"[bench] gather probe {pname} group 32 vec 2, {pb * 4}-byte pitch: {us:.1f} us")
+                del Bp
             del di
+    if "pitches" in only:  # not in the default set: the same B values at several row pitches
+        for dt, n2, pitches in (("fp32", 128, (128, 144, 160, 192, 256)), ("fp32", 64, (64, 96)),
+                                ("fp32", 256, (256, 288)), ("bf16", 256, (256, 320))):
+            esz = 4 if dt == "fp32" else 2
+            C2 = torch.empty((M, n2), device=dev)
+            for pe in pitches:
+                Bq = torch.empty((K, pe), device=dev)
+                kernels.fill_uniform_(Bq, data.CFG_SEED + 2, data.STREAM_DENSE)
+                if dt == "bf16":
+                    Bq = kernels.to_bf16(Bq)
+                Bv = Bq[:, :n2]
+                _, fam_q = timed(lambda: kernels.spmm(rp, ci, va, Bv, out=C2), 8, 2, ["spmm"])
+                us = fam_q["spmm"][0] / max(fam_q["spmm"][1], 1) * 1e3
+                extras[f"pitch_spmm_{dt}_n{n2}_{pe * esz}B_us"] = round(us, 1)
+                log(f"[bench] spmm {dt} N={n2} ({n2 * esz}-byte rows) at a {pe * esz}-byte pitch: main kernel {us:.1f} us")
+                del Bq, Bv
+            del C2
     if "widths" in only:  # the same matrix at N = 64 and 256 (fp32 and bf16)
         for n2 in (64, 256):
             B2 = torch.empty((K, n2), device=dev)
