@@ -1,5 +1,6 @@
 // Library runtime: status strings, HIP error capture, stream-ordered
 // workspace, optional event timing of kernel launches.
+#include <algorithm>
 #include <atomic>
 #include <cstdlib>
 #include <cstring>
@@ -99,11 +100,13 @@ int hip_fail(hipError_t e) {
 // released with hipFreeAsync on the same stream (after the work that used it)
 // and replaced.  Buffers come from the library's own stream-ordered pool per
 // device (the device's default pool, which other libraries in the process share,
-// is left untouched) and are kept until the process ends, as the pool (release
-// threshold: never) would keep them anyway.  A destroyed stream's handle reused
-// by a new stream finds its slots idle: hipStreamDestroy completes the old
-// stream's work first.
+// is left untouched).  A thread keeps entries for at most kWsStreams streams: the
+// least recently used idle entry makes room, and its buffers, like all of a
+// thread's buffers when the thread ends, are released after a device-wide
+// synchronisation, since the stream they were used on may no longer exist.
 namespace {
+constexpr size_t kWsStreams = 8;
+
 struct WsSlot {
     void* p = nullptr;
     size_t bytes = 0;
@@ -114,13 +117,49 @@ struct WsEntry {
     hipStream_t s;
     WsSlot slot[2];
 };
-thread_local std::vector<WsEntry> t_ws;
 
-WsEntry& ws_entry(int dev, hipStream_t s) {
-    for (auto& e : t_ws)
-        if (e.dev == dev && e.s == s) return e;
-    t_ws.push_back(WsEntry{dev, s, {}});
-    return t_ws.back();
+void ws_release_entry(WsEntry& e) {
+    int cur = -1;
+    const bool have_cur = hipGetDevice(&cur) == hipSuccess;
+    bool switched = false;
+    bool synced = false;
+    for (auto& sl : e.slot) {
+        if (!sl.p) continue;
+        if (!switched && have_cur && cur != e.dev) switched = hipSetDevice(e.dev) == hipSuccess;
+        if (!synced) synced = hipDeviceSynchronize() == hipSuccess;  // work of any stream that used it
+        (void)hipFreeAsync(sl.p, nullptr);
+        sl = WsSlot{};
+    }
+    if (switched) (void)hipSetDevice(cur);
+    (void)hipGetLastError();
+}
+
+struct WsCache {
+    std::vector<WsEntry> entries;  // least recently used first
+    ~WsCache() {
+        for (auto& e : entries) ws_release_entry(e);
+    }
+};
+thread_local WsCache t_ws;
+
+// The cache entry of (dev, s), most recently used last; nullptr when every entry
+// holds a busy slot and none can make room (the caller then allocates uncached).
+WsEntry* ws_entry(int dev, hipStream_t s) {
+    auto& v = t_ws.entries;
+    for (size_t i = 0; i < v.size(); ++i)
+        if (v[i].dev == dev && v[i].s == s) {
+            if (i + 1 != v.size()) std::rotate(v.begin() + (std::ptrdiff_t)i, v.begin() + (std::ptrdiff_t)i + 1, v.end());
+            return &v.back();
+        }
+    if (v.size() >= kWsStreams) {
+        size_t i = 0;
+        while (i < v.size() && (v[i].slot[0].busy || v[i].slot[1].busy)) ++i;
+        if (i == v.size()) return nullptr;
+        ws_release_entry(v[i]);
+        v.erase(v.begin() + (std::ptrdiff_t)i);
+    }
+    v.push_back(WsEntry{dev, s, {}});
+    return &v.back();
 }
 
 int pool_alloc(void** p, size_t bytes, int dev, hipStream_t s) {
@@ -167,7 +206,9 @@ int ws_alloc(void** p, size_t bytes, hipStream_t s) {
         std::lock_guard<std::mutex> g(g_arena_mu);
         if (bytes > g_ws_high[dev]) g_ws_high[dev] = bytes;
     }
-    WsEntry& e = ws_entry(dev, s);
+    WsEntry* ep = ws_entry(dev, s);
+    if (!ep) return pool_alloc(p, bytes, dev, s);
+    WsEntry& e = *ep;
     WsSlot* fit = nullptr;
     WsSlot* idle = nullptr;
     for (auto& sl : e.slot) {
@@ -211,7 +252,7 @@ void ws_free(void* p, hipStream_t s) {
             g_arena_used[dev] = marks.empty() ? 0 : marks.back().second;
             return;
         }
-        for (auto& e : t_ws)  // a cached slot goes back to its entry, whatever stream the caller names
+        for (auto& e : t_ws.entries)  // a cached slot goes back to its entry, whatever stream the caller names
             for (auto& sl : e.slot)
                 if (sl.busy && sl.p == p) {
                     sl.busy = false;

@@ -93,3 +93,25 @@ def test_workspace_two_host_threads(shared_stream):
         assert len(outs) == 24
         for i, c in outs:
             assert torch.equal(c.cpu(), alone[i]), f"problem {i}"
+
+
+def test_workspace_entries_evicted_across_many_streams():
+    """One host thread cycles through more streams than its cache keeps (8): the
+    least recently used entries are released and rebuilt, and every result stays
+    bit-identical to the call made alone."""
+    case("workspace cache: eviction over 11 streams")
+    probs = _problems()
+    tens = [tuple(dev(a.rowptr, a.colind, a.values, b)) for a, b in probs]
+    alone = [_spmm(t, "merge").cpu() for t in tens]
+    streams = [torch.cuda.Stream() for _ in range(11)]
+    outs = []
+    for rep in range(2):
+        for j, st in enumerate(streams):
+            st.wait_stream(torch.cuda.current_stream())
+            with torch.cuda.stream(st):
+                i = (j + rep) % len(tens)
+                outs.append((i, _spmm(tens[i], "merge")))
+    torch.cuda.synchronize()
+    for i, c in outs:
+        assert torch.equal(c.cpu(), alone[i]), f"problem {i}"
+
