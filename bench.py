@@ -142,7 +142,7 @@ def parse():
     p.add_argument("--gpus", type=int, default=1)
     p.add_argument("--steps", type=int, default=50)
     p.add_argument("--warmup", type=int, default=5)
-    p.add_argument("--only", default="", help="comma list: spmm,sddmm,spmv,mixed,widths,shards,small,gcn,gcn_scaled,cpu (default all); "
+    p.add_argument("--only", default="", help="comma list: spmm,pitched,sddmm,spmv,mixed,widths,shards,small,gcn,gcn_scaled,cpu (default all); "
                         "regimes (column-placement variants of the headline matrix), probe (the B-row gathers "
                         "alone on those column patterns), pitches (B at several row pitches) and spmv_variants "
                         "(SpMV schedules) run only when named")
@@ -296,8 +296,8 @@ def main():
 
     from armsparse_amd import _lib, data, kernels
 
-    only = set(filter(None, args.only.split(","))) or {"spmm", "sddmm", "spmv", "mixed", "widths", "shards", "small",
-                                                        "gcn", "gcn_scaled", "cpu"}
+    only = set(filter(None, args.only.split(","))) or {"spmm", "pitched", "sddmm", "spmv", "mixed", "widths", "shards",
+                                                        "small", "gcn", "gcn_scaled", "cpu"}
     dev = torch.device("cuda", 0)
     torch.cuda.set_device(dev)
     stream = torch.cuda.current_stream(dev)
@@ -389,7 +389,7 @@ def main():
         extras["spmm_native_selector_main_kernel_us"] = round(fam_s["spmm"][0] / max(fam_s["spmm"][1], 1) * 1e3, 2)
         extras["spmm_native_selector_GBps"] = round(nbytes * max(10, args.steps // 2) / el_s / 1e9, 1)
 
-    if cfg is None:  # the same B at the gather pitch (kernels.row_pitch: 640-byte rows, the same values)
+    if cfg is None and "pitched" in only:  # the same B at the gather pitch (kernels.row_pitch: 640 bytes a row)
         pb = kernels.row_pitch(N, 4)
         Bp = torch.empty((K, pb), device=dev)[:, :N]
         Bp.copy_(B)
