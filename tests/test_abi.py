@@ -64,6 +64,13 @@ def test_status_and_invalid_arguments_without_gpu():
     assert lib.armsp_spmv_vcsr_f32(4, 6, None, None, None, None, None, 0, None) == 1  # not a multiple of 4
     assert lib.armsp_sddmm_csr_f32(4, 4, -1, 1, None, None, None, None, 1, None, 1, None, None, None) == 1
     assert lib.armsp_gemm_bf16(4, 4, 4, None, 1, 1, None, 1, 1, None, 2, 0, None) == 1  # ldc < N
+    # pitched conversion and the activation backward's output pitch
+    assert lib.armsp_convert_rows_f32_to_bf16(4, 4, None, 2, None, 4, None) == 1  # ldx < N
+    assert lib.armsp_convert_rows_f32_to_bf16(4, 4, None, 4, None, 3, None) == 1  # ldy < N
+    assert lib.armsp_convert_rows_f32_to_bf16(0, 4, None, 4, None, 4, None) == 0  # nothing to do
+    assert lib.armsp_convert_rows_f32_to_bf16(4, 4, None, 4, None, 4, None) == 1  # null operands
+    assert lib.armsp_bias_relu_dropout_backward_bf16(4, 16, None, None, 0.0, None, 8, None, None) == 1  # ldgy < N
+    assert lib.armsp_bias_relu_dropout_backward_bf16(4, 16, None, None, 0.0, None, 20, None, None) == 1  # ldgy % 8
     cfg = _lib.Config.make("row", group=3)
     import ctypes
 
