@@ -37,11 +37,17 @@ GEMM_SOURCES = ["armsparse_amd/csrc/gemm.hip", "armsparse_amd/csrc/common.hpp", 
 
 
 def source_stamp(paths):
+    """Hash of the sources' code: C and C++ comments and whitespace are left out, so
+    an edit of the documentation alone keeps a counter summary current."""
     import hashlib
+    import re
     h = hashlib.sha256()
     for p in paths:
-        with open(os.path.join(ROOT, p), "rb") as f:
-            h.update(f.read())
+        with open(os.path.join(ROOT, p), encoding="utf-8") as f:
+            text = f.read()
+        text = re.sub(r"/\*.*?\*/", " ", text, flags=re.S)
+        text = re.sub(r"//[^\n]*", " ", text)
+        h.update(" ".join(text.split()).encode())
     return h.hexdigest()[:16]
 
 

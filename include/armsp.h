@@ -388,9 +388,10 @@ int armsp_mlp_infer(int nvec, int F, int H, int C, const float* x, const float* 
  * `group` lanes (8, 16, 32, 64) of `vec` floats (2 or 4) each; the N / (group·vec)
  * slices (a divisor of 8) are spread over the XCDs as in the SpMM slice kernel.
  * Writes per-lane sums to out (at least 64·S·ceil(n/512) floats, S the slice
- * count, rounded up to whole 4-wave groups).  Its rate on a matrix's column
- * indices against relabelled or uniform ones shows what the gather pattern
- * costs by itself. */
+ * count, rounded up to whole 4-wave groups).  idx values must lie in [0, K):
+ * like the SpMM kernels' column indices they are not range-checked on the
+ * device.  Its rate on a matrix's column indices against relabelled or uniform
+ * ones shows what the gather pattern costs by itself. */
 int armsp_probe_gather_f32(int n, const int* idx, const float* B, int K, int ldb, int N, int group, int vec,
                            float* out, void* stream);
 
