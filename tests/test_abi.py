@@ -79,6 +79,43 @@ def test_status_and_invalid_arguments_without_gpu():
                               ctypes.byref(cfg), None) == 1
 
 
+def test_selector_and_probe_arguments_checked_on_the_host():
+    """The selector's install/clear/choose and the gather probe reject bad arguments
+    before any device call; an empty probe is a no-op."""
+    import ctypes
+
+    import numpy as np
+
+    from armsparse_amd import _lib
+
+    lib = _lib.lib
+    F, H, C = 17, 4, 2
+    f = lambda n: np.zeros(n, np.float32)  # noqa: E731
+    mu, sd, W1, b1, W2, b2, W3, b3 = f(F), np.ones(F, np.float32), f(F * H), f(H), f(H * H), f(H), f(H * C), f(C)
+    classes = (_lib.Config * C)(_lib.Config.make("merge"), _lib.Config.make("row", group=8))
+    P = lambda a: a.ctypes.data_as(ctypes.c_void_p)  # noqa: E731
+    args = [P(a) for a in (mu, sd, W1, b1, W2, b2, W3, b3)]
+    assert lib.armsp_selector_set(2, F, H, C, *args, classes) == 1          # no such op
+    assert lib.armsp_selector_set(0, F + 1, H, C, *args, classes) == 1      # 17 features
+    assert lib.armsp_selector_set(0, F, 0, C, *args, classes) == 1          # empty hidden layer
+    assert lib.armsp_selector_set(0, F, H, C, *args[:-1], None, classes) == 1  # a missing array
+    assert lib.armsp_selector_set(0, F, H, C, *args, None) == 1             # no class table
+    assert lib.armsp_selector_clear(2) == 1 and lib.armsp_selector_clear(0) == 0
+    out = _lib.Config()
+    assert lib.armsp_selector_choose(0, 4, 4, 4, 4, None, None, None) == 1    # no output
+    assert lib.armsp_selector_choose(3, 4, 4, 4, 4, None, ctypes.byref(out), None) == 1
+    # no network installed: the heuristic's all-zero configuration, without a device call
+    assert lib.armsp_selector_choose(0, 4, 4, 4, 4, None, ctypes.byref(out), None) == 0
+    assert (out.schedule, out.group, out.vec, out.chunk, out.unroll) == (0, 0, 0, 0, 0)
+    d = ctypes.c_void_p(16)
+    assert lib.armsp_probe_gather_f32(-1, d, d, 8, 128, 128, 32, 2, d, None) == 1
+    assert lib.armsp_probe_gather_f32(8, d, d, 8, 64, 128, 32, 2, d, None) == 1    # ldb < N
+    assert lib.armsp_probe_gather_f32(8, d, d, 8, 128, 128, 24, 2, d, None) == 1   # group
+    assert lib.armsp_probe_gather_f32(8, d, d, 8, 128, 128, 32, 3, d, None) == 1   # vec
+    assert lib.armsp_probe_gather_f32(8, d, d, 8, 96, 96, 16, 2, d, None) == 2     # 3 slices: not a divisor of 8
+    assert lib.armsp_probe_gather_f32(0, d, d, 8, 128, 128, 32, 2, d, None) == 0   # nothing to gather
+
+
 def test_no_gpu_fails_loudly():
     import torch
 
