@@ -248,10 +248,30 @@ __device__ __forceinline__ void vcsr_short_rows(unsigned blk, int M, const int* 
             add(v1, x1);
         }
     }
-    if (fused) {
-        for (; idx < end; idx += 4) s = fmaf(val[idx + l], x[colind[idx] + l], s);
-    } else {
-        for (; idx < end; idx += 4) s += unfused_product(val[idx + l], x[colind[idx] + l]);
+    // the rest, fewer than BB blocks (most rows of a power-law graph are this short):
+    // every load in flight at once, then the adds in block order (blocks past the
+    // row are skipped, not added as zeros, which would turn a -0 sum into +0)
+    const int nrem = (end - idx) >> 2;
+    if (nrem > 0) {
+        float v[BB], xv[BB];
+#pragma unroll
+        for (int u = 0; u < BB; ++u) {
+            v[u] = 0.0f;
+            xv[u] = 0.0f;
+            if (u < nrem) {
+                v[u] = val[idx + 4 * u + l];
+                xv[u] = x[colind[idx + 4 * u] + l];
+            }
+        }
+        if (fused) {
+#pragma unroll
+            for (int u = 0; u < BB; ++u)
+                if (u < nrem) s = fmaf(v[u], xv[u], s);
+        } else {
+#pragma unroll
+            for (int u = 0; u < BB; ++u)
+                if (u < nrem) s += unfused_product(v[u], xv[u]);
+        }
     }
     const float t = __fadd_rn(s, __shfl_xor(s, 2, 4));   // lanes 0/2: s0+s2, lanes 1/3: s1+s3
     const float tot = __fadd_rn(t, __shfl_xor(t, 1, 4));  // (s0+s2)+(s1+s3)
@@ -573,7 +593,7 @@ int armsp_spmv_vcsr_f32(int M, int nnz_padded, const int* rowptr, const int* col
     TimedLaunch t("spmv_vcsr", s);
     const unsigned short_wg = (unsigned)(((int64_t)M * 4 + 255) / 256);
     if (split_long) {  // long-row workgroups first: the hub rows are the critical path
-        const int n_long_wg = 2 * device_cus();
+        const int n_long_wg = 2 * device_cus();  // 1, 2, 4, 8 per CU: 730, 478, 516, 535 us on the 1M R-MAT
         const dim3 grid((unsigned)n_long_wg + short_wg);
         if (fused)
             hipLaunchKernelGGL(k_spmv_vcsr_split<true>, grid, dim3(256), 0, s, M, rowptr, colind, val, x, y,
