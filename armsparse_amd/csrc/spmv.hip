@@ -9,8 +9,10 @@
 //          consecutive items, a wave-level segmented scan joins rows cut
 //          between lanes, and a carry per wave is added by a fixup kernel.
 //  VCSR  — the reference's blocked format with its exact summation order:
-//          four lanes per row keep the four partial sums s0..s3 and the row
-//          total is (s0+s2)+(s1+s3), as vadd/vpadd compute it.
+//          every row's four partial sums s0..s3 are each added in block order
+//          (by four lanes of a short row, by a wave for a medium row, by a
+//          workgroup for a long one) and the row total is (s0+s2)+(s1+s3), as
+//          vadd/vpadd compute it.
 #include <climits>
 #include <cstdlib>
 
@@ -284,10 +286,12 @@ __global__ __launch_bounds__(256) void k_spmv_vcsr(int M, const int* __restrict_
     vcsr_short_rows(blockIdx.x, M, rowptr, colind, val, x, y, fused, 0);
 }
 
-// Rows of more than `long_blocks` blocks: k_vcsr_find_long lists them and
-// workgroups of 256 lanes take listed rows from a shared counter, while the
-// other workgroups of the same launch walk the short rows (so the two overlap,
-// and the hub rows, listed and started first, are the critical path).  A
+// One launch, three walkers: rows of at most VCSR_MED_MIN blocks get four lanes
+// each, rows of up to `long_blocks` blocks a wave each (vcsr_medium_rows), and
+// longer rows a workgroup each: k_vcsr_find_long lists the medium and the long
+// rows, and workgroups of 256 lanes take listed long rows from a shared counter
+// while the other workgroups of the same launch walk the rest (so the walkers
+// overlap, and the hub rows, listed and started first, are the critical path).  A
 // workgroup walks its row in batches of 256 blocks, one block per lane: each
 // lane keeps the loads of its next D batches in flight (values and x, 16 bytes
 // each; a block's column index one ring further ahead, so no x load waits for
@@ -298,10 +302,16 @@ __global__ __launch_bounds__(256) void k_spmv_vcsr(int M, const int* __restrict_
 // the lanes form the rounded products a·b, so lanes 0-3 only add; with the
 // fused one they stage values and x and lanes 0-3 run the fmaf chain.  Either
 // way each s_l sees the reference's sequence of operations: bit-exact.
-constexpr int VCSR_LONG_DEFAULT = 128;
+constexpr int VCSR_LONG_DEFAULT = 1024;    // 512, 1024, 2048, 4096: 381, 351, 390, 385 us (1M R-MAT)
+constexpr int VCSR_MED_MIN = 16;          // rows of more blocks than this (and at most long_blocks) take a wave each
+constexpr int VCSR_MED_AHEAD = 4;         // steps of 16 blocks in flight per medium-row wave
 constexpr int VCSR_BATCH = 256;           // blocks per batch, one per lane
 constexpr int VCSR_PAD = VCSR_BATCH + 4;  // row pitch: lanes 0-3 read from distinct LDS banks
 constexpr int VCSR_AHEAD = 4;             // batches in flight per lane
+
+// Workgroups of medium-row waves per CU (1M R-MAT, 64.5M padded entries: 1, 2, 4, 8,
+// 16, 32 per CU give 987, 596, 383, 355, 351, 357 us).
+constexpr int VCSR_MED_WG_PER_CU = 16;
 
 // Rows longer than this many blocks take the workgroup-per-row path
 // (ARMSP_VCSR_LONG_BLOCKS overrides it, read once per process).
@@ -314,17 +324,103 @@ int vcsr_long_blocks() {
     return v;
 }
 
+// Lists the long rows (more than long_blocks blocks) in list[0, count[0]) and the
+// medium rows (more than VCSR_MED_MIN blocks, at most long_blocks) in
+// list_med[0, count[2]); the order within a list does not matter, each row is
+// summed by one walker.
 __global__ void k_vcsr_find_long(int M, const int* __restrict__ rowptr, int long_blocks, int* __restrict__ list,
-                                 int* __restrict__ count) {
+                                 int* __restrict__ list_med, int* __restrict__ count) {
     const int r = blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= M) return;
-    if (rowptr[r + 1] - rowptr[r] > 4 * long_blocks) list[atomicAdd(count, 1)] = r;
+    const int n = rowptr[r + 1] - rowptr[r];
+    if (n > 4 * long_blocks)
+        list[atomicAdd(&count[0], 1)] = r;
+    else if (n > 4 * VCSR_MED_MIN)
+        list_med[atomicAdd(&count[2], 1)] = r;
 }
 
-// count[0] = listed rows, count[1] = the work counter
+// count[0] = listed long rows, count[1] = the long rows' work counter, count[2] =
+// listed medium rows
 __global__ void k_zero_count(int* count) {
     count[0] = 0;
     count[1] = 0;
+    count[2] = 0;
+}
+
+// Medium rows, one wave each: wave gw of nmw takes list_med[gw], list_med[gw + nmw],
+// ... (a fixed assignment, no shared counter).  A step covers 16 blocks, lane
+// 4q + l holding position l of block q; each lane keeps its next VCSR_MED_AHEAD
+// steps' value and x loads in flight (the column index one ring further ahead).
+// The lanes form the rounded products (the unfused reading) or keep value and x
+// (fused), and lane l's partial sum s_l takes block 0..15's position-l term by a
+// shuffle from lane 4q + l, in block order: the reference's sequence of
+// operations per s_l, as in the other walkers.
+template <bool FUSED>
+__device__ void vcsr_medium_rows(int gw, int nmw, const int* __restrict__ list_med, const int* __restrict__ count,
+                                 const int* __restrict__ rowptr, const int* __restrict__ colind,
+                                 const float* __restrict__ val, const float* __restrict__ x, float* __restrict__ y) {
+    constexpr int D = VCSR_MED_AHEAD;
+    const int lane = threadIdx.x & 63;
+    const int q = lane >> 2, l = lane & 3;
+    const int nmed = __builtin_amdgcn_readfirstlane(count[2]);
+    for (int i = gw; i < nmed; i += nmw) {
+        const int r = __builtin_amdgcn_readfirstlane(list_med[i]);
+        const int beg = __builtin_amdgcn_readfirstlane(rowptr[r]);
+        const int nb = (__builtin_amdgcn_readfirstlane(rowptr[r + 1]) - beg) >> 2;
+        const int nsteps = (nb + 15) >> 4;
+        auto col_of = [&](int k) {
+            const int b = k * 16 + q;
+            return b < nb ? colind[beg + 4 * b] : 0;
+        };
+        auto fetch = [&](int k, int c, float& v, float& xv) {
+            const int b = k * 16 + q;
+            v = 0.0f;
+            xv = 0.0f;
+            if (b < nb) {
+                v = val[beg + 4 * b + l];
+                xv = x[c + l];
+            }
+        };
+        int rc[D];
+        float rv[D], rx[D];
+#pragma unroll
+        for (int u = 0; u < D; ++u) rc[u] = col_of(u);
+#pragma unroll
+        for (int u = 0; u < D; ++u) {
+            fetch(u, rc[u], rv[u], rx[u]);
+            rc[u] = col_of(u + D);
+        }
+        float s = 0.0f;
+        for (int k0 = 0; k0 < nsteps; k0 += D) {
+#pragma unroll
+            for (int u = 0; u < D; ++u) {
+                const int k = k0 + u;
+                if (k >= nsteps) break;  // uniform: nsteps is the wave's
+                const int cnt = min(16, nb - k * 16);
+                const float pv = rv[u], px = rx[u];
+                float p = 0.0f;
+                if constexpr (!FUSED) p = unfused_product(pv, px);  // vmlaq_f32 rounds the product on its own
+                fetch(k + D, rc[u], rv[u], rx[u]);  // in flight for D steps
+                rc[u] = col_of(k + 2 * D);
+#pragma unroll
+                for (int b = 0; b < 16; ++b) {
+                    if (b < cnt) {  // uniform
+                        if constexpr (FUSED) {
+                            const float a = __shfl(pv, 4 * b + l);
+                            const float c = __shfl(px, 4 * b + l);
+                            s = fmaf(a, c, s);
+                        } else {
+                            s = __fadd_rn(s, __shfl(p, 4 * b + l));
+                        }
+                    }
+                }
+            }
+        }
+        // every lane group now holds s0..s3 of the row
+        const float t = __fadd_rn(s, __shfl_xor(s, 2, 4));   // lanes 0/2: s0+s2, lanes 1/3: s1+s3
+        const float tot = __fadd_rn(t, __shfl_xor(t, 1, 4));  // (s0+s2)+(s1+s3)
+        if (lane == 0) y[r] = __fadd_rn(y[r], tot);
+    }
 }
 
 template <bool FUSED>
@@ -439,18 +535,24 @@ template <bool FUSED>
 __global__ __launch_bounds__(256) void k_spmv_vcsr_split(int M, const int* __restrict__ rowptr,
                                                          const int* __restrict__ colind,
                                                          const float* __restrict__ val, const float* __restrict__ x,
-                                                         float* __restrict__ y, int long_blocks,
-                                                         const int* __restrict__ list, int* __restrict__ count,
-                                                         int n_long_wg) {
+                                                         float* __restrict__ y, const int* __restrict__ list,
+                                                         const int* __restrict__ list_med, int* __restrict__ count,
+                                                         int n_long_wg, int n_med_wg) {
     __shared__ __align__(16) float s_a[2][4][VCSR_PAD];
     __shared__ __align__(16) float s_b[FUSED ? 2 : 1][4][FUSED ? VCSR_PAD : 4];
     __shared__ int s_row;
-    if ((int)blockIdx.x < n_long_wg) {
+    const int b = (int)blockIdx.x;
+    if (b < n_long_wg) {
         vcsr_long_rows<FUSED>(list, count, rowptr, colind, val, x, y, s_a,
                               reinterpret_cast<float (*)[4][VCSR_PAD]>(&s_b[0][0][0]), &s_row);
         return;
     }
-    vcsr_short_rows(blockIdx.x - (unsigned)n_long_wg, M, rowptr, colind, val, x, y, FUSED ? 1 : 0, long_blocks);
+    if (b < n_long_wg + n_med_wg) {
+        const int gw = (b - n_long_wg) * 4 + (int)(threadIdx.x >> 6);
+        vcsr_medium_rows<FUSED>(gw, n_med_wg * 4, list_med, count, rowptr, colind, val, x, y);
+        return;
+    }
+    vcsr_short_rows((unsigned)(b - n_long_wg - n_med_wg), M, rowptr, colind, val, x, y, FUSED ? 1 : 0, VCSR_MED_MIN);
 }
 
 template <int IPT>
@@ -572,35 +674,40 @@ int armsp_spmv_vcsr_f32(int M, int nnz_padded, const int* rowptr, const int* col
         return launch_merge<8>(M, nnz_padded, rowptr, colind, val, x, y, ARMSP_ACCUMULATE, s);
     }
     const int fused = (flags & ARMSP_FUSED_MLA) ? 1 : 0;
-    // rows of more than long_blocks blocks (possible only above that many blocks in all)
-    // go to the wave-per-row kernel; its 16-byte loads need 16-byte-aligned values and x
-    // (VCSR rows hold whole 4-blocks and block columns are multiples of 4, so rowptr
-    // entries and the gathered x offsets are multiples of 4 as well)
-    const int long_blocks = vcsr_long_blocks();
-    const bool split_long = nnz_padded / 4 > long_blocks && ((uintptr_t)val % 16) == 0 && ((uintptr_t)x % 16) == 0;
+    // Rows of more than VCSR_MED_MIN blocks take a wave each, rows of more than
+    // long_blocks a workgroup each; the workgroup walker's 16-byte loads need
+    // 16-byte-aligned values and x (VCSR rows hold whole 4-blocks and block columns
+    // are multiples of 4, so rowptr entries and the gathered x offsets are multiples
+    // of 4 as well), so with unaligned operands every long row goes to the waves.
+    const bool aligned = ((uintptr_t)val % 16) == 0 && ((uintptr_t)x % 16) == 0;
+    const int long_blocks = aligned ? vcsr_long_blocks() : INT_MAX / 8;
+    const bool split_long = nnz_padded / 4 > VCSR_MED_MIN;
     void* ws = nullptr;
     int* list = nullptr;
+    int* list_med = nullptr;
     int* count = nullptr;
     if (split_long) {
-        const int st = ws_alloc(&ws, ((size_t)M + 64) * sizeof(int), s);
+        const int st = ws_alloc(&ws, (2 * (size_t)M + 64) * sizeof(int), s);
         if (st) return st;
         count = static_cast<int*>(ws);
         list = count + 64;
+        list_med = list + M;
         hipLaunchKernelGGL(k_zero_count, dim3(1), dim3(1), 0, s, count);
         hipLaunchKernelGGL(k_vcsr_find_long, dim3((unsigned)((M + 255) / 256)), dim3(256), 0, s, M, rowptr,
-                           long_blocks, list, count);
+                           long_blocks, list, list_med, count);
     }
     TimedLaunch t("spmv_vcsr", s);
     const unsigned short_wg = (unsigned)(((int64_t)M * 4 + 255) / 256);
-    if (split_long) {  // long-row workgroups first: the hub rows are the critical path
-        const int n_long_wg = 2 * device_cus();  // 1, 2, 4, 8 per CU: 730, 478, 516, 535 us on the 1M R-MAT
-        const dim3 grid((unsigned)n_long_wg + short_wg);
+    if (split_long) {  // long-row workgroups first (the hub rows are the critical path), then medium-row waves
+        const int n_long_wg = 2 * device_cus();
+        const int n_med_wg = VCSR_MED_WG_PER_CU * device_cus();
+        const dim3 grid((unsigned)(n_long_wg + n_med_wg) + short_wg);
         if (fused)
-            hipLaunchKernelGGL(k_spmv_vcsr_split<true>, grid, dim3(256), 0, s, M, rowptr, colind, val, x, y,
-                               long_blocks, list, count, n_long_wg);
+            hipLaunchKernelGGL(k_spmv_vcsr_split<true>, grid, dim3(256), 0, s, M, rowptr, colind, val, x, y, list,
+                               list_med, count, n_long_wg, n_med_wg);
         else
-            hipLaunchKernelGGL(k_spmv_vcsr_split<false>, grid, dim3(256), 0, s, M, rowptr, colind, val, x, y,
-                               long_blocks, list, count, n_long_wg);
+            hipLaunchKernelGGL(k_spmv_vcsr_split<false>, grid, dim3(256), 0, s, M, rowptr, colind, val, x, y, list,
+                               list_med, count, n_long_wg, n_med_wg);
     } else {
         hipLaunchKernelGGL(k_spmv_vcsr, dim3(short_wg), dim3(256), 0, s, M, rowptr, colind, val, x, y, fused);
     }

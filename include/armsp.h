@@ -197,11 +197,12 @@ int armsp_spmv_f32(int M, int K, int nnz,
  * colind[idx] of a block start is a multiple of 4.  Replaces
  * sparse_dense_vector (SpMV.cpp:155-179): per row four partial sums
  * s_l += val*x (unfused, or fused with ARMSP_FUSED_MLA) reduced as
- * (s0+s2)+(s1+s3), bit-exact with the reference's order.  Short rows get a
- * 4-lane group each; rows of more than 128 blocks (ARMSP_VCSR_LONG_BLOCKS in
- * the environment overrides it) get a workgroup each in the same launch when
- * val and x are 16-byte aligned, which still adds each row's blocks in order,
- * so the longest row's chain of adds bounds the call.  ARMSP_VCSR_BALANCED runs
+ * (s0+s2)+(s1+s3), bit-exact with the reference's order.  Rows of up to 16
+ * blocks get a 4-lane group each, rows of up to 1024 blocks a wave each, longer
+ * rows (ARMSP_VCSR_LONG_BLOCKS in the environment moves the 1024) a workgroup
+ * each when val and x are 16-byte aligned, all in one launch; every walker
+ * still adds each row's blocks in order, so the longest row's chain of adds
+ * bounds the call.  ARMSP_VCSR_BALANCED runs
  * the merge-path SpMV over the same arrays instead (rows of any length spread
  * over the device; the merge path's order of additions).  x must hold
  * K rounded up to a multiple of 4 entries. */

@@ -84,18 +84,56 @@ def test_vcsr_bitexact_skewed_rows(fused):
 
 @pytest.mark.parametrize("fused", [False, True])
 def test_vcsr_bitexact_more_long_rows_than_waves(fused):
-    """More rows of over 512 blocks than the long-row kernel has waves (2 blocks of 4
-    waves per CU): waves take several rows from the shared counter and restage their
-    LDS for each; still bit-exact.  Half the rows are empty (short-row kernel)."""
-    case(f"vcsr many long rows fused={fused}")
+    """More rows of 17 to 1024 blocks than the medium-row walker has waves (one
+    block of 4 waves per CU): waves walk several rows each; still bit-exact.  Half
+    the rows are empty (short-row walker)."""
+    case(f"vcsr many medium rows fused={fused}")
     M, K = 4400, 16384
     rp, ci, va = random_csr(M, K, 0.042, 21, empty_rows=0.5)
     v = data.csr_to_vcsr(data.CSR(rp, ci, va, (M, K)))
     nb = np.diff(v.rowptr) // 4
     cus = torch.cuda.get_device_properties(0).multi_processor_count
-    assert (nb > 512).sum() > 2 * cus * 4 and (nb == 0).sum() > 0
+    assert ((nb > 16) & (nb <= 1024)).sum() > cus * 4 and (nb == 0).sum() > 0
     x = data.host_uniform(K, 6, 1)
     y0 = np.random.default_rng(5).uniform(-1, 1, M).astype(np.float32)
+    d_rp, d_ci, d_va, d_x, d_y = dev(v.rowptr, v.colind, v.values, x, y0)
+    kernels.vcsr_spmv(d_rp, d_ci, d_va, d_x, d_y, fused=fused)
+    expect = oracle.sparse_dense_vector(x, y0.copy(), v.rowptr, v.colind, v.values, fused=fused)
+    assert np.array_equal(d_y.cpu().numpy().view(np.uint32), expect.view(np.uint32))
+
+
+def _rows_of_lengths(lengths, K, seed):
+    """CSR whose row r has lengths[r] distinct uniform columns of [0, K), U(-1,1) values."""
+    r = np.random.default_rng(seed)
+    cols = [np.sort(r.choice(K, n, replace=False)) for n in lengths]
+    rp = np.zeros(len(lengths) + 1, np.int32)
+    rp[1:] = np.cumsum(lengths)
+    ci = np.concatenate(cols).astype(np.int32) if len(cols) else np.zeros(0, np.int32)
+    va = r.uniform(-1, 1, ci.size).astype(np.float32)
+    return data.CSR(rp, ci, va, (len(lengths), K))
+
+
+@pytest.mark.parametrize("fused", [False, True])
+def test_vcsr_bitexact_all_three_walkers(fused):
+    """Rows for every walker, each class outnumbering its workers: more rows of over
+    1024 blocks than long-row workgroups (2 per CU), more of 17-1024 blocks than
+    medium-row waves (4 per CU), short and empty rows, shuffled; bit-exact with the
+    restated sparse_dense_vector in either reading."""
+    case(f"vcsr three walkers fused={fused}")
+    cus = torch.cuda.get_device_properties(0).multi_processor_count
+    K = 65536
+    r = np.random.default_rng(33)
+    lengths = np.concatenate([r.integers(5000, 7000, 2 * cus + 40),      # > 1024 blocks of 16384
+                              r.integers(70, 1000, 4 * cus + 200),       # 17..1024 blocks
+                              r.integers(1, 17, 3000), np.zeros(500, np.int64)])
+    r.shuffle(lengths)
+    v = data.csr_to_vcsr(_rows_of_lengths(lengths, K, 34))
+    nb = np.diff(v.rowptr) // 4
+    assert (nb > 1024).sum() > 2 * cus and ((nb > 16) & (nb <= 1024)).sum() > 4 * cus
+    assert ((nb > 0) & (nb <= 16)).sum() > 1000 and (nb == 0).sum() > 0
+    x = data.host_uniform(K, 7, 1)
+    y0 = np.random.default_rng(6).uniform(-1, 1, len(lengths)).astype(np.float32)
+    y0[::7] = -0.0  # empty rows must still turn -0 into +0 as the reference's y += 0 does
     d_rp, d_ci, d_va, d_x, d_y = dev(v.rowptr, v.colind, v.values, x, y0)
     kernels.vcsr_spmv(d_rp, d_ci, d_va, d_x, d_y, fused=fused)
     expect = oracle.sparse_dense_vector(x, y0.copy(), v.rowptr, v.colind, v.values, fused=fused)
