@@ -7,7 +7,7 @@ on timings from the exhaustive autotuner below, with the library's own HIP
 kernels doing every forward, backward and Adam step (armsp_sgemm_f32,
 armsp_softmax_xent, armsp_relu_backward, armsp_colsum_f32,
 armsp_adam_step), and runs inference in armsp_mlp_infer.  The autotuner is
-the fallback when the network is unsure and the judge of its regret.
+the fallback when the network is unsure and the yardstick of its regret.
 
 The reference has no counterpart: it hard-codes an N threshold
 (gcn/pytorch-custom/spmm_kernel.cu:437-457).
