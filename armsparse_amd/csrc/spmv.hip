@@ -535,7 +535,8 @@ template <bool FUSED>
 __global__ __launch_bounds__(256) void k_spmv_vcsr_split(int M, const int* __restrict__ rowptr,
                                                          const int* __restrict__ colind,
                                                          const float* __restrict__ val, const float* __restrict__ x,
-                                                         float* __restrict__ y, const int* __restrict__ list,
+                                                         float* __restrict__ y, int long_blocks,
+                                                         const int* __restrict__ list,
                                                          const int* __restrict__ list_med, int* __restrict__ count,
                                                          int n_long_wg, int n_med_wg) {
     __shared__ __align__(16) float s_a[2][4][VCSR_PAD];
@@ -552,7 +553,10 @@ __global__ __launch_bounds__(256) void k_spmv_vcsr_split(int M, const int* __res
         vcsr_medium_rows<FUSED>(gw, n_med_wg * 4, list_med, count, rowptr, colind, val, x, y);
         return;
     }
-    vcsr_short_rows((unsigned)(b - n_long_wg - n_med_wg), M, rowptr, colind, val, x, y, FUSED ? 1 : 0, VCSR_MED_MIN);
+    // the short walker takes what neither list holds: rows of at most
+    // min(VCSR_MED_MIN, long_blocks) blocks
+    vcsr_short_rows((unsigned)(b - n_long_wg - n_med_wg), M, rowptr, colind, val, x, y, FUSED ? 1 : 0,
+                    long_blocks < VCSR_MED_MIN ? long_blocks : VCSR_MED_MIN);
 }
 
 template <int IPT>
@@ -681,7 +685,7 @@ int armsp_spmv_vcsr_f32(int M, int nnz_padded, const int* rowptr, const int* col
     // of 4 as well), so with unaligned operands every long row goes to the waves.
     const bool aligned = ((uintptr_t)val % 16) == 0 && ((uintptr_t)x % 16) == 0;
     const int long_blocks = aligned ? vcsr_long_blocks() : INT_MAX / 8;
-    const bool split_long = nnz_padded / 4 > VCSR_MED_MIN;
+    const bool split_long = nnz_padded / 4 > (long_blocks < VCSR_MED_MIN ? long_blocks : VCSR_MED_MIN);
     void* ws = nullptr;
     int* list = nullptr;
     int* list_med = nullptr;
@@ -703,11 +707,11 @@ int armsp_spmv_vcsr_f32(int M, int nnz_padded, const int* rowptr, const int* col
         const int n_med_wg = VCSR_MED_WG_PER_CU * device_cus();
         const dim3 grid((unsigned)(n_long_wg + n_med_wg) + short_wg);
         if (fused)
-            hipLaunchKernelGGL(k_spmv_vcsr_split<true>, grid, dim3(256), 0, s, M, rowptr, colind, val, x, y, list,
-                               list_med, count, n_long_wg, n_med_wg);
+            hipLaunchKernelGGL(k_spmv_vcsr_split<true>, grid, dim3(256), 0, s, M, rowptr, colind, val, x, y,
+                               long_blocks, list, list_med, count, n_long_wg, n_med_wg);
         else
-            hipLaunchKernelGGL(k_spmv_vcsr_split<false>, grid, dim3(256), 0, s, M, rowptr, colind, val, x, y, list,
-                               list_med, count, n_long_wg, n_med_wg);
+            hipLaunchKernelGGL(k_spmv_vcsr_split<false>, grid, dim3(256), 0, s, M, rowptr, colind, val, x, y,
+                               long_blocks, list, list_med, count, n_long_wg, n_med_wg);
     } else {
         hipLaunchKernelGGL(k_spmv_vcsr, dim3(short_wg), dim3(256), 0, s, M, rowptr, colind, val, x, y, fused);
     }

@@ -140,6 +140,44 @@ def test_vcsr_bitexact_all_three_walkers(fused):
     assert np.array_equal(d_y.cpu().numpy().view(np.uint32), expect.view(np.uint32))
 
 
+_VCSR_ENV_CHILD = r"""
+import sys
+sys.path.insert(0, sys.argv[1])
+sys.path.insert(0, sys.argv[1] + "/tests")
+import numpy as np, torch
+import oracle
+from armsparse_amd import data, kernels
+a = data.rmat_csr(14, 1 << 18, seed=11)
+v = data.csr_to_vcsr(a)
+x = data.host_uniform(v.shape[1], 5, 1)
+ok = True
+for fused in (False, True):
+    y0 = np.random.default_rng(4).uniform(-1, 1, v.shape[0]).astype(np.float32)
+    t = [torch.from_numpy(np.ascontiguousarray(u)).cuda() for u in (v.rowptr, v.colind, v.values, x, y0)]
+    kernels.vcsr_spmv(*t, fused=fused)
+    expect = oracle.sparse_dense_vector(x, y0.copy(), v.rowptr, v.colind, v.values, fused=fused)
+    ok = ok and bool(np.array_equal(t[4].cpu().numpy().view(np.uint32), expect.view(np.uint32)))
+print("bitexact", ok)
+"""
+
+
+@pytest.mark.parametrize("long_blocks", [8, 64])
+def test_vcsr_long_threshold_from_the_environment(long_blocks):
+    """ARMSP_VCSR_LONG_BLOCKS below and above the 16-block short-row limit: every row
+    is still summed once, by one walker, bit-exact (a fresh process, since the value
+    is read once per process)."""
+    import os
+    import subprocess
+    import sys
+
+    case(f"vcsr ARMSP_VCSR_LONG_BLOCKS={long_blocks}")
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    r = subprocess.run([sys.executable, "-c", _VCSR_ENV_CHILD, root], capture_output=True, text=True, timeout=240,
+                       env=dict(os.environ, ARMSP_VCSR_LONG_BLOCKS=str(long_blocks)))
+    print(r.stdout, r.stderr[-2000:])
+    assert r.returncode == 0 and r.stdout.split()[-2:] == ["bitexact", "True"]
+
+
 def test_vcsr_unaligned_operands_bitexact():
     """Values or x off 16-byte alignment (a sliced view) take the 4-lane kernel for
     every row: same bits as the aligned call."""
