@@ -140,6 +140,33 @@ def test_vcsr_bitexact_all_three_walkers(fused):
     assert np.array_equal(d_y.cpu().numpy().view(np.uint32), expect.view(np.uint32))
 
 
+@pytest.mark.parametrize("fused", [False, True])
+def test_vcsr_walker_boundaries(fused):
+    """Rows of exactly 0, 1, 15, 16, 17, 1023, 1024, 1025 and more blocks (one nonzero
+    per block, at a random offset in it): the limits between the 4-lane, the wave and
+    the workgroup walkers; bit-exact."""
+    case(f"vcsr boundaries fused={fused}")
+    K = 1 << 14
+    nbs = [0, 1, 15, 16, 17, 31, 32, 33, 1023, 1024, 1025, 1040, 2048, 4000] * 3
+    r = np.random.default_rng(41)
+    rows = []
+    for nb in nbs:
+        blk = np.sort(r.choice(K // 4, nb, replace=False))
+        rows.append(4 * blk + r.integers(0, 4, nb))
+    rp = np.zeros(len(rows) + 1, np.int32)
+    rp[1:] = np.cumsum([len(c) for c in rows])
+    ci = np.concatenate(rows).astype(np.int32)
+    va = r.uniform(-1, 1, ci.size).astype(np.float32)
+    v = data.csr_to_vcsr(data.CSR(rp, ci, va, (len(rows), K)))
+    assert list(np.diff(v.rowptr) // 4) == nbs
+    x = data.host_uniform(K, 8, 1)
+    y0 = np.random.default_rng(7).uniform(-1, 1, len(rows)).astype(np.float32)
+    d_rp, d_ci, d_va, d_x, d_y = dev(v.rowptr, v.colind, v.values, x, y0)
+    kernels.vcsr_spmv(d_rp, d_ci, d_va, d_x, d_y, fused=fused)
+    expect = oracle.sparse_dense_vector(x, y0.copy(), v.rowptr, v.colind, v.values, fused=fused)
+    assert np.array_equal(d_y.cpu().numpy().view(np.uint32), expect.view(np.uint32))
+
+
 _VCSR_ENV_CHILD = r"""
 import sys
 sys.path.insert(0, sys.argv[1])
