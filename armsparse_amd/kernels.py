@@ -491,7 +491,7 @@ def row_pitch(n: int, element_size: int) -> int:
 
 def empty_rows(m: int, n: int, dtype: torch.dtype, device) -> torch.Tensor:
     """An uninitialised [m×n] view with rows at row_pitch (the padding is never read)."""
-    esz = torch.empty((), dtype=dtype).element_size()
+    esz = dtype.itemsize
     p = row_pitch(n, esz)
     if p == n:
         return torch.empty((m, n), dtype=dtype, device=device)
