@@ -605,13 +605,19 @@ def main():
         x = torch.empty(K, device=dev)
         kernels.fill_uniform_(x, data.CFG_SEED + 1, data.STREAM_DENSE + 1)
         y = torch.empty(M, device=dev)
-        el_v, fam_v = timed(lambda: kernels.spmv(rp, ci, va, x, out=y), steps_x * 4, 5, ["spmv"])
+        # wall time per call with the library's timing events off, then the kernels
+        # on their own (events on) in a second pass
+        el_v, _ = timed(lambda: kernels.spmv(rp, ci, va, x, out=y), steps_x * 4, 5, [])
+        _, fam_v = timed(lambda: kernels.spmv(rp, ci, va, x, out=y), steps_x * 4, 2,
+                         ["spmv", "merge_partition", "spmv_fixup"])
         vb = 4 * (M + 1) + 8 * nnz + 4 * K + 4 * M
         extras["spmv_GBps"] = round(vb * steps_x * 4 / el_v / 1e9, 1)
         extras["spmv_us"] = round(el_v / (steps_x * 4) * 1e6, 1)
         xh = x.cpu().numpy()
         errs["spmv"] = spmv_rows_err(A.rowptr, A.colind, A.values, xh, y.cpu().numpy()[rows], rows)
         extras["spmv_kernel_us"] = round(fam_v["spmv"][0] / max(fam_v["spmv"][1], 1) * 1e3, 1)
+        extras["spmv_partition_fixup_us"] = round(sum(fam_v[f][0] / max(fam_v[f][1], 1) * 1e3
+                                                      for f in ("merge_partition", "spmv_fixup")), 1)
         # the reference's vector-aware (zero-padded 1×4 blocks) CSR SpMV, y += A·x
         Av = data.csr_to_vcsr(A)
         rv, cv, vv = (torch.from_numpy(t).to(dev) for t in (Av.rowptr, Av.colind, Av.values))
