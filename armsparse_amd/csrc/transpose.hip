@@ -16,6 +16,10 @@
 // in input order; duplicates are summed in that order, entries whose sum is
 // zero are dropped on request (toCSR keeps only `!= 0`), and rowptr is read off
 // the sorted rows.
+//
+// CSR → vector-aware CSR (toVectorAwareCSR, SpMV.cpp:15-38) on the device: a
+// decision per block-opening entry, an exclusive scan of the decisions, and a
+// scatter of the stored blocks.
 #include <climits>
 
 #include "common.hpp"
@@ -267,6 +271,100 @@ void radix_sort(int n, int range, const int* keys_in, const int* idx_in, int* ke
     *iout_final = iin;
 }
 
+// ------------------------------------------------ CSR → vector-aware CSR
+// toVectorAwareCSR (SpMV.cpp:15-38) on the device, from CSR instead of a dense
+// matrix.  A row's aligned block of columns [4b, 4b+4) is stored iff one of the
+// row's values in it triggers (v != 0, or |v| > threshold when threshold > 0:
+// numaAware/numaAwareSpMV.cpp:48), as four slots holding the row's values (0
+// where absent) with column indices 4b..4b+3.  The entry that opens a block (the
+// row's first entry in it) decides it alone, reading the at most three entries
+// after it (columns strictly increase within a row, so a block holds at most
+// four).  An exclusive scan over those decisions gives every stored block its
+// place, and the blocked rowptr is read off the scan at the row starts.
+
+// start[p] = 1 where a non-empty row begins; err bit 0: malformed rowptr.
+__global__ void k_vcsr_rows(int M, int nnz, const int* __restrict__ rowptr, uint8_t* __restrict__ start,
+                            int* __restrict__ err) {
+    const int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (r >= M) return;
+    const int a = rowptr[r], b = rowptr[r + 1];
+    if ((r == 0 && a != 0) || (r == M - 1 && b != nnz) || a < 0 || b < a || b > nnz) {
+        atomicOr(err, 1);
+        return;
+    }
+    if (a < b) start[a] = 1;
+}
+
+__global__ void k_vcsr_clear(int64_t n, uint8_t* __restrict__ start, int* __restrict__ err) {
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i < n) start[i] = 0;
+    if (i == 0) *err = 0;
+}
+
+__device__ __forceinline__ bool vcsr_trig(float v, float threshold) {
+    return threshold > 0.0f ? fabsf(v) > threshold : v != 0.0f;
+}
+
+// keep[p] = 1 iff entry p opens a stored block, keep[nnz] = 0 (the scan then
+// leaves the block count there).  err bit 1: a column out of [0, K) or a row
+// whose columns do not strictly increase.
+__global__ void k_vcsr_mark(int nnz, int K, const int* __restrict__ colind, const float* __restrict__ val,
+                            float threshold, const uint8_t* __restrict__ start, int* __restrict__ keep,
+                            int* __restrict__ err) {
+    const int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (p > nnz) return;
+    if (p == nnz) {
+        keep[p] = 0;
+        return;
+    }
+    const int c = colind[p];
+    const bool first = p == 0 || start[p] != 0;
+    const int cp = first ? -1 : colind[p - 1];
+    if (c < 0 || c >= K || (!first && cp >= c)) atomicOr(err, 2);
+    const bool opens = first || (cp >> 2) != (c >> 2);
+    bool on = opens && vcsr_trig(val ? val[p] : 1.0f, threshold);
+    for (int d = 1; opens && !on && d <= 3; ++d) {
+        const int64_t q = p + d;
+        if (q >= nnz || start[q] || (colind[q] >> 2) != (c >> 2)) break;
+        on = vcsr_trig(val ? val[q] : 1.0f, threshold);
+    }
+    keep[p] = on ? 1 : 0;
+}
+
+// out_rowptr[r] = 4 · (stored blocks before row r), r = 0..M (with no rows, the one
+// entry is 0 and rowptr is not read).
+__global__ void k_vcsr_rowptr(int M, const int* __restrict__ rowptr, const int* __restrict__ pos,
+                              int* __restrict__ out_rowptr) {
+    const int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (r > M) return;
+    out_rowptr[r] = M == 0 ? 0 : 4 * pos[rowptr[r]];
+}
+
+// The opening entry of every stored block writes its four slots.
+__global__ void k_vcsr_fill(int nnz, const int* __restrict__ colind, const float* __restrict__ val,
+                            const uint8_t* __restrict__ start, const int* __restrict__ pos,
+                            int* __restrict__ out_colind, float* __restrict__ out_val) {
+    const int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (p >= nnz) return;
+    const int b = pos[p];
+    if (pos[p + 1] == b) return;
+    const int c = colind[p];
+    const int c0 = (c >> 2) * 4;
+    float slot[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+    slot[c & 3] = val ? val[p] : 1.0f;
+    for (int d = 1; d <= 3; ++d) {
+        const int64_t q = p + d;
+        if (q >= nnz || start[q] || (colind[q] >> 2) != (c >> 2)) break;
+        slot[colind[q] & 3] = val ? val[q] : 1.0f;
+    }
+    const int64_t o = 4 * (int64_t)b;
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+        out_colind[o + t] = c0 + t;
+        out_val[o + t] = slot[t];
+    }
+}
+
 }  // namespace
 }  // namespace armsp
 
@@ -404,6 +502,64 @@ extern "C" int armsp_coo_to_csr(int M, int K, int nnz, const int* rows, const in
                        val, nnz_out);
     hipLaunchKernelGGL(k_rowptr_dev, dim3((unsigned)(((int64_t)nnz + 1 + 255) / 256)), dim3(256), 0, s,
                        (const int*)nnz_out, M, urow, rowptr);
+    e = hipGetLastError();
+    ws_free(ws, s);
+    if (e != hipSuccess) return hip_fail(e);
+    t.stop();
+    return ARMSP_SUCCESS;
+}
+
+extern "C" int armsp_csr_to_vcsr_device(int M, int K, int nnz, const int* rowptr, const int* colind,
+                                        const float* val, float threshold, int* out_rowptr, int* out_colind,
+                                        float* out_val, int64_t capacity, int64_t* padded, void* stream) {
+    using namespace armsp;
+    if (M < 0 || K < 0 || nnz < 0 || !padded || !out_rowptr || !rowptr) return ARMSP_ERROR_INVALID_VALUE;
+    if (nnz > 0 && !colind) return ARMSP_ERROR_INVALID_VALUE;
+    if ((out_colind == nullptr) != (out_val == nullptr)) return ARMSP_ERROR_INVALID_VALUE;
+    if (M == 0 && nnz > 0) return ARMSP_ERROR_INVALID_VALUE;
+    if (nnz > INT_MAX - 2) return ARMSP_ERROR_NOT_SUPPORTED;
+    hipStream_t s = as_stream(stream);
+    TimedLaunch t("csr_to_vcsr", s);
+    const int64_t n1 = (int64_t)nnz + 1;  // scanned entries: nnz decisions and the total
+    const size_t p4 = ((size_t)(n1 + 1) * 4 + 255) & ~(size_t)255;
+    const size_t s4 = ((size_t)scan_ws_ints(n1) * 4 + 255) & ~(size_t)255;
+    const size_t b1 = ((size_t)(nnz > 0 ? nnz : 1) + 255) & ~(size_t)255;
+    void* ws = nullptr;
+    int st = ws_alloc(&ws, p4 + s4 + b1, s);
+    if (st) return st;
+    char* w = static_cast<char*>(ws);
+    int* pos = reinterpret_cast<int*>(w);
+    int* err = pos + n1;  // one int past the scanned range
+    int* scanws = reinterpret_cast<int*>(w + p4);
+    uint8_t* start = reinterpret_cast<uint8_t*>(w + p4 + s4);
+    auto fail = [&](int code) {
+        ws_free(ws, s);
+        return code;
+    };
+    const unsigned gs = (unsigned)(((int64_t)(nnz > 0 ? nnz : 1) + 255) / 256);
+    hipLaunchKernelGGL(k_vcsr_clear, dim3(gs), dim3(256), 0, s, (int64_t)nnz, start, err);
+    if (M > 0)
+        hipLaunchKernelGGL(k_vcsr_rows, dim3((unsigned)(((int64_t)M + 255) / 256)), dim3(256), 0, s, M, nnz, rowptr,
+                           start, err);
+    hipLaunchKernelGGL(k_vcsr_mark, dim3((unsigned)((n1 + 255) / 256)), dim3(256), 0, s, nnz, K, colind, val,
+                       threshold, start, pos, err);
+    exclusive_scan(n1, pos, scanws, s);
+    int h[2] = {0, 0};  // the block count and the error bits
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipMemcpyAsync(h, pos + nnz, sizeof(h), hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e != hipSuccess) return fail(hip_fail(e));
+    if (h[1]) return fail(ARMSP_ERROR_INVALID_VALUE);
+    if (4 * (int64_t)h[0] > INT_MAX) return fail(ARMSP_ERROR_NOT_SUPPORTED);
+    *padded = 4 * (int64_t)h[0];
+    hipLaunchKernelGGL(k_vcsr_rowptr, dim3((unsigned)(((int64_t)M + 1 + 255) / 256)), dim3(256), 0, s, M, rowptr,
+                       (const int*)pos, out_rowptr);
+    if (out_colind) {
+        if (capacity < *padded) return fail(ARMSP_ERROR_INVALID_VALUE);
+        if (nnz > 0)
+            hipLaunchKernelGGL(k_vcsr_fill, dim3(gs), dim3(256), 0, s, nnz, colind, val, (const uint8_t*)start,
+                               (const int*)pos, out_colind, out_val);
+    }
     e = hipGetLastError();
     ws_free(ws, s);
     if (e != hipSuccess) return hip_fail(e);

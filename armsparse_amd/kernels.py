@@ -215,6 +215,31 @@ def vcsr_spmv(rowptr: torch.Tensor, colind: torch.Tensor, values: torch.Tensor, 
     return y
 
 
+def csr_to_vcsr(rowptr: torch.Tensor, colind: torch.Tensor, values: Optional[torch.Tensor], num_cols: int, *,
+                threshold: float = 0.0):
+    """(rowptr, colind, values) of the zero-padded 1×4-blocked CSR of
+    toVectorAwareCSR (SpMV.cpp:15-38), built on the device from canonical CSR;
+    ``threshold`` > 0 stores a block only when some |v| > threshold
+    (numaAwareSpMV.cpp:48).  The blocked matrix has ceil(K/4)·4 columns."""
+    fn = "csr_to_vcsr"
+    _check_csr(fn, rowptr, colind, values)
+    m, nnz = rowptr.numel() - 1, colind.numel()
+    host = not rowptr.is_cuda
+    dev = _device() if host else rowptr.device
+    rp, ci, va = (_on_device(t, dev) for t in (rowptr, colind, values))
+    out_rp = torch.empty(m + 1, dtype=torch.int32, device=dev)
+    padded = ctypes.c_int64(0)
+    call("armsp_csr_to_vcsr_device", m, num_cols, nnz, _ptr(rp), _ptr(ci), _ptr(va), float(threshold),
+         _ptr(out_rp), None, None, 0, ctypes.byref(padded), _stream(dev))
+    n = padded.value
+    out_ci = torch.empty(max(n, 1), dtype=torch.int32, device=dev)
+    out_va = torch.empty(max(n, 1), dtype=torch.float32, device=dev)
+    call("armsp_csr_to_vcsr_device", m, num_cols, nnz, _ptr(rp), _ptr(ci), _ptr(va), float(threshold),
+         _ptr(out_rp), _ptr(out_ci), _ptr(out_va), out_ci.numel(), ctypes.byref(padded), _stream(dev))
+    res = (out_rp, out_ci[:n], out_va[:n])
+    return tuple(t.cpu() for t in res) if host else res
+
+
 # ---------------------------------------------------------------- SDDMM
 def sddmm(rows: torch.Tensor, colind: torch.Tensor, U: torch.Tensor, V: torch.Tensor, *,
           scale: Optional[torch.Tensor] = None, coo: bool = False, config: ConfigLike = None) -> torch.Tensor:

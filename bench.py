@@ -618,8 +618,20 @@ def main():
         extras["spmv_kernel_us"] = round(fam_v["spmv"][0] / max(fam_v["spmv"][1], 1) * 1e3, 1)
         extras["spmv_partition_fixup_us"] = round(sum(fam_v[f][0] / max(fam_v[f][1], 1) * 1e3
                                                       for f in ("merge_partition", "spmv_fixup")), 1)
-        # the reference's vector-aware (zero-padded 1×4 blocks) CSR SpMV, y += A·x
+        # the reference's vector-aware (zero-padded 1×4 blocks) CSR SpMV, y += A·x; the
+        # blocked arrays built on the host and on the device (same arrays: tests)
+        t0 = time.perf_counter()
         Av = data.csr_to_vcsr(A)
+        extras["vcsr_convert_host_ms"] = round((time.perf_counter() - t0) * 1e3, 1)
+        conv = []
+        for _ in range(3):
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            dv = kernels.csr_to_vcsr(rp, ci, va, K)
+            torch.cuda.synchronize()
+            conv.append(time.perf_counter() - t0)
+            del dv
+        extras["vcsr_convert_device_ms"] = round(min(conv) * 1e3, 2)
         rv, cv, vv = (torch.from_numpy(t).to(dev) for t in (Av.rowptr, Av.colind, Av.values))
         xv = torch.zeros(Av.shape[1], device=dev)
         xv[:K] = x

@@ -291,6 +291,24 @@ int armsp_coo_to_csr(int M, int K, int nnz,
                      const int* rows, const int* cols, const float* vals, int drop_zeros,
                      int* rowptr, int* colind, float* val, int* nnz_out, void* stream);
 
+/* CSR → the zero-padded 1×4-blocked CSR of toVectorAwareCSR (SpMV.cpp:15-38),
+ * on the device: the same arrays armsp_csr_to_vcsr builds on the host, bit for
+ * bit.  A block [4b, 4b+4) of a row is stored iff one of the row's values in it
+ * triggers (v != 0; with threshold > 0, |v| > threshold as in
+ * numaAware/numaAwareSpMV.cpp:48; val == NULL: every value is 1), as four
+ * slots (the row's values, 0 where absent) with column indices 4b..4b+3, so
+ * x needs ceil(K/4)·4 entries.  Always writes out_rowptr[M+1] and *padded (a
+ * host int64: the stored entries); with out_colind and out_val (both or
+ * neither) of capacity >= *padded it also writes the blocks.  Call once with
+ * NULL outputs to size them.  The input must be canonical CSR: rowptr[0] = 0,
+ * non-decreasing, rowptr[M] = nnz, columns in [0, K) strictly increasing in a
+ * row (else ARMSP_ERROR_INVALID_VALUE, with nothing written).  The check
+ * synchronises the stream once; the blocks are written after it, in stream
+ * order. */
+int armsp_csr_to_vcsr_device(int M, int K, int nnz, const int* rowptr, const int* colind, const float* val,
+                             float threshold, int* out_rowptr, int* out_colind, float* out_val, int64_t capacity,
+                             int64_t* padded, void* stream);
+
 /* -------------------------------------------------------- dense helpers */
 /* C[M×N] (=|+=) op(A)[M×Kd] · op(B)[Kd×N] in bfloat16 on MFMA with fp32
  * accumulation; element (i,k) of op(A) is A[i*sam + k*sak], element (k,j) of

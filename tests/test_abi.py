@@ -71,8 +71,16 @@ def test_status_and_invalid_arguments_without_gpu():
     assert lib.armsp_convert_rows_f32_to_bf16(4, 4, None, 4, None, 4, None) == 1  # null operands
     assert lib.armsp_bias_relu_dropout_backward_bf16(4, 16, None, None, 0.0, None, 8, None, None) == 1  # ldgy < N
     assert lib.armsp_bias_relu_dropout_backward_bf16(4, 16, None, None, 0.0, None, 20, None, None) == 1  # ldgy % 8
-    cfg = _lib.Config.make("row", group=3)
     import ctypes
+
+    # device CSR → VCSR: sizes, a missing count pointer, one of the two outputs, rows without entries
+    n = ctypes.c_int64(0)
+    p8 = ctypes.c_void_p(8)
+    assert lib.armsp_csr_to_vcsr_device(-1, 4, 0, p8, None, None, 0.0, p8, None, None, 0, ctypes.byref(n), None) == 1
+    assert lib.armsp_csr_to_vcsr_device(4, 4, 4, p8, p8, None, 0.0, p8, None, None, 0, None, None) == 1
+    assert lib.armsp_csr_to_vcsr_device(4, 4, 4, p8, p8, None, 0.0, p8, p8, None, 16, ctypes.byref(n), None) == 1
+    assert lib.armsp_csr_to_vcsr_device(0, 4, 3, p8, p8, None, 0.0, p8, None, None, 0, ctypes.byref(n), None) == 1
+    cfg = _lib.Config.make("row", group=3)
 
     # group must be a power of two in [4, 64]: rejected on the host
     assert lib.armsp_spmm_f32(4, 4, 4, 0, ctypes.c_void_p(8), None, None, None, 4, ctypes.c_void_p(8), 4, 0,

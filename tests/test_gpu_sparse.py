@@ -241,6 +241,110 @@ def test_vcsr_balanced_schedule():
         kernels.vcsr_spmv(d_rp, d_ci, d_va, d_x, d_y, fused=True, balanced=True)
 
 
+# ------------------------------------------------- CSR → VCSR on the device
+def _vcsr_expected(a, threshold=0.0):
+    """toVectorAwareCSR restated on the dense matrix, its columns padded with zeros
+    to a multiple of 4 (the blocks past K hold zeros)."""
+    k4 = (a.shape[1] + 3) // 4 * 4
+    return oracle.toVectorAwareCSR(oracle.densify(a.rowptr, a.colind, a.values, k4), threshold=threshold)
+
+
+def _with_zeros(a, every=3):
+    """a with every `every`-th stored value set to an explicit 0."""
+    v = a.values.copy()
+    v[::every] = 0.0
+    return data.CSR(a.rowptr, a.colind, v, a.shape)
+
+
+@pytest.mark.parametrize("name", ["uniform", "rmat_skewed", "empty_rows", "k_not_multiple_of_4", "pattern",
+                                  "explicit_zeros", "threshold"])
+def test_csr_to_vcsr_device_matches_toVectorAwareCSR(name):
+    """The device converter gives toVectorAwareCSR's arrays bit for bit: rows of
+    every length (R-MAT hub rows), empty rows, K not a multiple of 4, a pattern
+    (values 1), explicit zeros (a block of zeros only is not stored) and the
+    |v| > 0.1 trigger of numaAwareSpMV."""
+    case(f"csr_to_vcsr_device {name}")
+    thr = 0.0
+    if name == "uniform":
+        a = data.uniform_csr(64, 96, 400, seed=4)
+    elif name == "rmat_skewed":
+        a = data.rmat_csr(12, 1 << 16, seed=6, a=0.7, b=0.1, c=0.1)
+    elif name == "empty_rows":
+        a = data.CSR(*random_csr(300, 120, 0.05, seed=8, empty_rows=0.3), (300, 120))
+        assert (np.diff(a.rowptr) == 0).any()
+    elif name == "k_not_multiple_of_4":
+        a = data.uniform_csr(50, 37, 300, seed=9)
+    elif name == "pattern":
+        a = data.uniform_csr(64, 96, 400, seed=10, values=False)
+    elif name == "explicit_zeros":
+        a = _with_zeros(data.uniform_csr(64, 96, 400, seed=11))
+    else:
+        a, thr = data.uniform_csr(128, 256, 3000, seed=12), 0.1
+    rp, ci, va = _vcsr_expected(a, thr)
+    d_rp, d_ci, d_va = dev(a.rowptr, a.colind, a.values)
+    g_rp, g_ci, g_va = (t.cpu().numpy() for t in kernels.csr_to_vcsr(d_rp, d_ci, d_va, a.shape[1], threshold=thr))
+    assert np.array_equal(g_rp, rp) and np.array_equal(g_ci, ci)
+    assert np.array_equal(g_va.view(np.uint32), va.view(np.uint32))
+    # the host converter builds the same arrays
+    h = data.csr_to_vcsr(a, threshold=thr)
+    assert np.array_equal(h.rowptr, g_rp) and np.array_equal(h.colind, g_ci) and np.array_equal(h.values, g_va)
+
+
+def test_csr_to_vcsr_device_full_rmat_1m():
+    """The 1M R-MAT (64.5M padded entries): same arrays as the host converter, and
+    the bit-exact SpMV over them gives the same y."""
+    case("csr_to_vcsr_device 1M R-MAT")
+    a = rmat_1m_cached()
+    h = data.csr_to_vcsr(a)
+    d_rp, d_ci, d_va = dev(a.rowptr, a.colind, a.values)
+    g_rp, g_ci, g_va = kernels.csr_to_vcsr(d_rp, d_ci, d_va, a.shape[1])
+    assert np.array_equal(g_rp.cpu().numpy(), h.rowptr)
+    assert np.array_equal(g_ci.cpu().numpy(), h.colind)
+    assert np.array_equal(g_va.cpu().numpy().view(np.uint32), h.values.view(np.uint32))
+    x = data.host_uniform(h.shape[1], 5, 1)
+    d_x = dev(x)[0]
+    y1 = torch.zeros(a.shape[0], device="cuda")
+    y2 = torch.zeros(a.shape[0], device="cuda")
+    h_rp, h_ci, h_va = dev(h.rowptr, h.colind, h.values)
+    kernels.vcsr_spmv(h_rp, h_ci, h_va, d_x, y1)
+    kernels.vcsr_spmv(g_rp, g_ci, g_va, d_x, y2)
+    assert torch.equal(y1.view(torch.int32), y2.view(torch.int32))
+
+
+def test_csr_to_vcsr_device_edges_and_errors():
+    """No rows, no entries, and malformed input: columns out of order or out of
+    range, rowptr not ending at nnz, and outputs too small are refused."""
+    from armsparse_amd import _lib
+    import ctypes
+
+    case("csr_to_vcsr_device edges")
+    g_rp, g_ci, g_va = kernels.csr_to_vcsr(*dev(np.zeros(1, np.int32), np.zeros(0, np.int32),
+                                                np.zeros(0, np.float32)), 8)
+    assert g_rp.cpu().tolist() == [0] and g_ci.numel() == 0
+    g_rp, g_ci, _ = kernels.csr_to_vcsr(*dev(np.zeros(5, np.int32), np.zeros(0, np.int32), None), 8)
+    assert g_rp.cpu().tolist() == [0] * 5 and g_ci.numel() == 0
+    bad = [
+        (np.array([0, 2, 3]), np.array([2, 0, 1])),  # columns not increasing in row 0
+        (np.array([0, 2, 3]), np.array([0, 1, 9])),  # column out of range (K = 8)
+        (np.array([0, 2, 2]), np.array([0, 1, 3])),  # rowptr[M] != nnz
+        (np.array([0, 3, 2]), np.array([0, 1, 3])),  # decreasing rowptr
+    ]
+    for rp, ci in bad:
+        d = dev(rp.astype(np.int32), ci.astype(np.int32), np.ones(len(ci), np.float32))
+        with pytest.raises(RuntimeError):
+            kernels.csr_to_vcsr(*d, 8)
+    a = data.uniform_csr(16, 32, 40, seed=13)
+    d_rp, d_ci, d_va = dev(a.rowptr, a.colind, a.values)
+    out_rp = torch.empty(17, dtype=torch.int32, device="cuda")
+    out_ci = torch.empty(4, dtype=torch.int32, device="cuda")
+    out_va = torch.empty(4, dtype=torch.float32, device="cuda")
+    padded = ctypes.c_int64(0)
+    st = _lib.lib.armsp_csr_to_vcsr_device(16, 32, a.nnz, d_rp.data_ptr(), d_ci.data_ptr(), d_va.data_ptr(), 0.0,
+                                           out_rp.data_ptr(), out_ci.data_ptr(), out_va.data_ptr(), 4,
+                                           ctypes.byref(padded), None)
+    assert st == _lib.ERROR_INVALID_VALUE and padded.value > 4
+
+
 def test_spmv_full_rmat_1m():
     case("spmv full 1M R-MAT")
     a = rmat_1m_cached()
