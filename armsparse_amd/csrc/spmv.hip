@@ -529,14 +529,14 @@ __device__ void vcsr_long_rows(const int* __restrict__ list, int* __restrict__ c
     }
 }
 
-// One launch: workgroups [0, n_long_wg) walk the listed long rows, the rest the
-// short rows (64 per workgroup).
+// One launch: workgroups [0, n_long_wg) walk the listed long rows, the next
+// n_med_wg the listed medium rows (a wave each), the rest the short rows (64 per
+// workgroup).
 template <bool FUSED>
 __global__ __launch_bounds__(256) void k_spmv_vcsr_split(int M, const int* __restrict__ rowptr,
                                                          const int* __restrict__ colind,
                                                          const float* __restrict__ val, const float* __restrict__ x,
-                                                         float* __restrict__ y, int long_blocks,
-                                                         const int* __restrict__ list,
+                                                         float* __restrict__ y, const int* __restrict__ list,
                                                          const int* __restrict__ list_med, int* __restrict__ count,
                                                          int n_long_wg, int n_med_wg) {
     __shared__ __align__(16) float s_a[2][4][VCSR_PAD];
@@ -554,9 +554,8 @@ __global__ __launch_bounds__(256) void k_spmv_vcsr_split(int M, const int* __res
         return;
     }
     // the short walker takes what neither list holds: rows of at most VCSR_MED_MIN
-    // blocks (long_blocks is never below it; a constant here lets the compiler drop
-    // the walker's batch loop for longer rows: 66 VGPRs instead of 111)
-    (void)long_blocks;
+    // blocks (the long-row limit is never below it; a constant here lets the
+    // compiler drop the walker's batch loop for longer rows: 66 VGPRs, not 111)
     vcsr_short_rows((unsigned)(b - n_long_wg - n_med_wg), M, rowptr, colind, val, x, y, FUSED ? 1 : 0, VCSR_MED_MIN);
 }
 
@@ -710,11 +709,11 @@ int armsp_spmv_vcsr_f32(int M, int nnz_padded, const int* rowptr, const int* col
         const int n_med_wg = VCSR_MED_WG_PER_CU * device_cus();
         const dim3 grid((unsigned)(n_long_wg + n_med_wg) + short_wg);
         if (fused)
-            hipLaunchKernelGGL(k_spmv_vcsr_split<true>, grid, dim3(256), 0, s, M, rowptr, colind, val, x, y,
-                               long_blocks, list, list_med, count, n_long_wg, n_med_wg);
+            hipLaunchKernelGGL(k_spmv_vcsr_split<true>, grid, dim3(256), 0, s, M, rowptr, colind, val, x, y, list,
+                               list_med, count, n_long_wg, n_med_wg);
         else
-            hipLaunchKernelGGL(k_spmv_vcsr_split<false>, grid, dim3(256), 0, s, M, rowptr, colind, val, x, y,
-                               long_blocks, list, list_med, count, n_long_wg, n_med_wg);
+            hipLaunchKernelGGL(k_spmv_vcsr_split<false>, grid, dim3(256), 0, s, M, rowptr, colind, val, x, y, list,
+                               list_med, count, n_long_wg, n_med_wg);
     } else {
         hipLaunchKernelGGL(k_spmv_vcsr, dim3(short_wg), dim3(256), 0, s, M, rowptr, colind, val, x, y, fused);
     }
