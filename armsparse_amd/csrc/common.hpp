@@ -172,6 +172,13 @@ __device__ __forceinline__ int merge_path_search_wave(int d, const int* __restri
     return lo;
 }
 
+// LDS writes of a wave visible to its own later LDS reads (no workgroup barrier).
+__device__ __forceinline__ void wave_sync_lds() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // Row boundary w of a precomputed merge-path partition (merge_partition) for
 // diagonal d, clamped to the interval a search could return so that a
 // malformed rowptr cannot carry a kernel outside its chunk.

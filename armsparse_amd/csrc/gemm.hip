@@ -151,7 +151,7 @@ __global__ __launch_bounds__(256) void k_gemm_tall(int M, int N, int K, int KP, 
     constexpr int NP = 32 * NTW * (4 / WR);  // columns held
     extern __shared__ __attribute__((aligned(16))) uint16_t lds[];
     uint16_t* sB = lds;                      // [NP][KP + 8]
-    uint16_t* sA = lds + NP * (KP + 8);      // [TR][TKP]
+    uint16_t* sA = lds + NP * (KP + 8);      // [TR][TKP], at least 4096 elements
     const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
     const int wr = wid % WR, wc = wid / WR;
     const int bp = KP + 8;
@@ -200,6 +200,141 @@ __global__ __launch_bounds__(256) void k_gemm_tall(int M, int N, int K, int KP, 
     }
     __syncthreads();
     const int ntiles = (M + TR - 1) / TR;
+    // MFMAs of one staged k step
+    auto mma_step = [&](f32x16 (&acc)[NTW], int k0) {
+#pragma unroll
+        for (int ks = 0; ks < TK; ks += 16) {
+            const int koff = ks + (lane >> 5) * 8;
+            const bf16x8 a = *reinterpret_cast<const bf16x8*>(&sA[(wr * 32 + (lane & 31)) * TKP + koff]);
+#pragma unroll
+            for (int j = 0; j < NTW; ++j) {
+                const int n = (wc * NTW + j) * 32 + (lane & 31);
+                const bf16x8 b = *reinterpret_cast<const bf16x8*>(&sB[n * bp + k0 + koff]);
+                acc[j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc[j], 0, 0, 0);
+            }
+        }
+    };
+    // the tile's results.  bf16 output with 16-byte aligned rows: each 32×32
+    // block goes through the wave's 2 KB of LDS (the A stage, free once the last k
+    // step is done) and leaves as 16-byte stores, two per lane, 64 bytes of a row
+    // per lane pair; otherwise 32 consecutive columns of one row per half wave.
+    const bool vout = OUTBF && ((uintptr_t)Cv % 16) == 0 && ldc % 8 == 0;
+    auto store_tile = [&](const f32x16 (&acc)[NTW], int m0) {
+        if (vout) {
+            uint16_t* stg = sA + wid * 1024;  // [32][32] bf16
+            const int orow = lane >> 1, ocol = (lane & 1) * 16;
+            const int gm = m0 + wr * 32 + orow;
+#pragma unroll
+            for (int j = 0; j < NTW; ++j) {
+#pragma unroll
+                for (int i = 0; i < 16; ++i)
+                    stg[(8 * (i >> 2) + 4 * (lane >> 5) + (i & 3)) * 32 + (lane & 31)] = f32_to_bf16_rne(acc[j][i]);
+                wave_sync_lds();
+                const uint4 v0 = *reinterpret_cast<const uint4*>(&stg[orow * 32 + ocol]);
+                const uint4 v1 = *reinterpret_cast<const uint4*>(&stg[orow * 32 + ocol + 8]);
+                const int gn = (wc * NTW + j) * 32 + ocol;
+                if (gm < M) {
+                    uint16_t* q = static_cast<uint16_t*>(Cv) + (int64_t)gm * ldc + gn;
+                    if (gn + 16 <= N) {
+                        *reinterpret_cast<uint4*>(q) = v0;
+                        *reinterpret_cast<uint4*>(q + 8) = v1;
+                    } else {
+                        const uint32_t w[8] = {v0.x, v0.y, v0.z, v0.w, v1.x, v1.y, v1.z, v1.w};
+                        for (int c = 0; c < 16 && gn + c < N; ++c) q[c] = (uint16_t)(w[c >> 1] >> (16 * (c & 1)));
+                    }
+                }
+                wave_sync_lds();  // the next block reuses the wave's staging area
+            }
+            __syncthreads();  // the staging areas are the next tile's A stage
+            return;
+        }
+#pragma unroll
+        for (int j = 0; j < NTW; ++j) {
+            const int gn = (wc * NTW + j) * 32 + (lane & 31);
+            if (gn < N) {
+#pragma unroll
+                for (int i = 0; i < 16; ++i) {
+                    const int gm = m0 + wr * 32 + 8 * (i >> 2) + 4 * (lane >> 5) + (i & 3);
+                    if (gm < M) {
+                        if constexpr (OUTBF) {  // bf16 output (round to nearest even), overwrite
+                            static_cast<uint16_t*>(Cv)[(int64_t)gm * ldc + gn] = f32_to_bf16_rne(acc[j][i]);
+                        } else {
+                            float* p = static_cast<float*>(Cv) + (int64_t)gm * ldc + gn;
+                            *p = accumulate ? *p + acc[j][i] : acc[j][i];
+                        }
+                    }
+                }
+            }
+        }
+    };
+    // Rows of A contiguous in k with 16-byte rows and at most PF k steps: every
+    // k step of the next tile is loaded into registers while this tile runs (the
+    // register of step s is refilled as soon as step s is staged), so a tile
+    // waits on one load latency, not on one per k step.
+    constexpr int PF = 8;          // k steps held in registers (K ≤ 256)
+    constexpr int NVW = TR / 64;   // 16-byte vectors per thread per k step
+    const int nsteps = KP / TK;
+    if (sak == 1 && va == 8 && nsteps <= PF) {
+        uint4 pre[PF][NVW];
+        const int kk = (tid & 3) * 8;  // this thread's k offset in every step (idx & 3 is tid & 3)
+        // rows of the tile being fetched, at this thread's k offset; rows past M read
+        // row M-1 (their results are never stored)
+        const uint16_t* rp[NVW];
+        auto rows_of = [&](int m0) {
+#pragma unroll
+            for (int v = 0; v < NVW; ++v) {
+                const int gm = m0 + ((tid + v * 256) >> 2);
+                rp[v] = A + (int64_t)(gm < M ? gm : M - 1) * sam + kk;
+            }
+        };
+        // every step but the last lies below K (KP is K rounded up to TK)
+        auto fetch = [&](int st, uint4 (&dst)[NVW]) {
+            const int gk = st * TK + kk;
+#pragma unroll
+            for (int v = 0; v < NVW; ++v) {
+                if (st + 1 < nsteps || gk + 8 <= K) {
+                    dst[v] = *reinterpret_cast<const uint4*>(rp[v] + st * TK);
+                } else {  // the k tail: zero past K
+                    uint16_t t[8];
+#pragma unroll
+                    for (int i = 0; i < 8; ++i) t[i] = gk + i < K ? rp[v][st * TK + i] : (uint16_t)0;
+                    dst[v] = make_uint4(t[0] | ((uint32_t)t[1] << 16), t[2] | ((uint32_t)t[3] << 16),
+                                        t[4] | ((uint32_t)t[5] << 16), t[6] | ((uint32_t)t[7] << 16));
+                }
+            }
+        };
+        if (blockIdx.x < (unsigned)ntiles) {
+            rows_of((int)blockIdx.x * TR);
+#pragma unroll
+            for (int st = 0; st < PF; ++st)
+                if (st < nsteps) fetch(st, pre[st]);
+        }
+        for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+            const int m0 = tile * TR;
+            const int next = tile + (int)gridDim.x;
+            const bool more = next < ntiles;
+            if (more) rows_of(next * TR);
+            f32x16 acc[NTW];
+#pragma unroll
+            for (int j = 0; j < NTW; ++j)
+#pragma unroll
+                for (int i = 0; i < 16; ++i) acc[j][i] = 0.0f;
+#pragma unroll
+            for (int st = 0; st < PF; ++st) {
+                if (st < nsteps) {
+#pragma unroll
+                    for (int v = 0; v < NVW; ++v)
+                        *reinterpret_cast<uint4*>(&sA[((tid + v * 256) >> 2) * TKP + kk]) = pre[st][v];
+                    if (more) fetch(st, pre[st]);
+                    __syncthreads();
+                    mma_step(acc, st * TK);
+                    __syncthreads();
+                }
+            }
+            store_tile(acc, m0);
+        }
+        return;
+    }
     for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
         const int m0 = tile * TR;
         f32x16 acc[NTW];
@@ -239,37 +374,10 @@ __global__ __launch_bounds__(256) void k_gemm_tall(int M, int N, int K, int KP, 
                 }
             }
             __syncthreads();
-#pragma unroll
-            for (int ks = 0; ks < TK; ks += 16) {
-                const int koff = ks + (lane >> 5) * 8;
-                const bf16x8 a = *reinterpret_cast<const bf16x8*>(&sA[(wr * 32 + (lane & 31)) * TKP + koff]);
-#pragma unroll
-                for (int j = 0; j < NTW; ++j) {
-                    const int n = (wc * NTW + j) * 32 + (lane & 31);
-                    const bf16x8 b = *reinterpret_cast<const bf16x8*>(&sB[n * bp + k0 + koff]);
-                    acc[j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc[j], 0, 0, 0);
-                }
-            }
+            mma_step(acc, k0);
             __syncthreads();
         }
-#pragma unroll
-        for (int j = 0; j < NTW; ++j) {
-            const int gn = (wc * NTW + j) * 32 + (lane & 31);
-            if (gn < N) {
-#pragma unroll
-                for (int i = 0; i < 16; ++i) {
-                    const int gm = m0 + wr * 32 + 8 * (i >> 2) + 4 * (lane >> 5) + (i & 3);
-                    if (gm < M) {
-                        if constexpr (OUTBF) {  // bf16 output (round to nearest even), overwrite
-                            static_cast<uint16_t*>(Cv)[(int64_t)gm * ldc + gn] = f32_to_bf16_rne(acc[j][i]);
-                        } else {
-                            float* p = static_cast<float*>(Cv) + (int64_t)gm * ldc + gn;
-                            *p = accumulate ? *p + acc[j][i] : acc[j][i];
-                        }
-                    }
-                }
-            }
-        }
+        store_tile(acc, m0);
     }
 }
 
@@ -291,12 +399,31 @@ __global__ void k_rows_to_bf16(int M, int N, const float* __restrict__ X, uint16
 // so the loads of one slab are all in flight together and overlap the MFMAs.
 // The partials are summed in block order afterwards (deterministic).
 //
-// vector along the contiguous dimension: 4 bf16 (8 bytes)
+// vector along the contiguous dimension: 4 bf16 (8 bytes), or 8 (16 bytes) in mode 3
 struct Slab {
-    int along;     // 0: vectors along the row dimension (m or n), 1: along k
+    int along;     // 0: vectors along the row dimension (m or n), 1: along k, 2: element loads,
+                   // 3: as 0 with 16-byte vectors
     int64_t srow;  // stride of the row dimension
     int64_t sk;    // stride of k
 };
+
+
+// mode 3: elements (r..r+7, k) of a row-contiguous operand, zero past R or ke
+__device__ __forceinline__ uint4 load8(const uint16_t* __restrict__ P, const Slab& sl, int r, int gk, int R, int ke) {
+    uint4 v = make_uint4(0u, 0u, 0u, 0u);
+    if (r < R && gk < ke) {
+        const uint16_t* p = P + r + (int64_t)gk * sl.sk;
+        if (r + 8 <= R) {
+            v = *reinterpret_cast<const uint4*>(p);
+        } else {
+            uint16_t t[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+            for (int i = 0; i < 8 && r + i < R; ++i) t[i] = p[i];
+            v = make_uint4(t[0] | ((uint32_t)t[1] << 16), t[2] | ((uint32_t)t[3] << 16), t[4] | ((uint32_t)t[5] << 16),
+                           t[6] | ((uint32_t)t[7] << 16));
+        }
+    }
+    return v;
+}
 
 // element (r, k) of a slab operand, 4 consecutive along the vector dimension
 __device__ __forceinline__ uint2 load4(const uint16_t* __restrict__ P, const Slab& sl, int r, int gk, int R, int ke) {
@@ -337,7 +464,13 @@ __device__ __forceinline__ uint2 load4(const uint16_t* __restrict__ P, const Sla
 }
 
 // slab vector q of this thread: (r, kk) of its first element
+template <bool V16>
 __device__ __forceinline__ void slab_pos(const Slab& sl, int v, int RP, int& r, int& kk) {
+    if constexpr (V16) {  // mode 3: as mode 0, 8 rows per vector
+        kk = v % RTK;
+        r = (v / RTK) * 8;
+        return;
+    }
     if (sl.along == 0) {
         // consecutive lanes take consecutive k: the four transposing 2-byte LDS stores
         // of a wave then each fill one 128-byte row segment, bank-conflict free (lanes
@@ -354,6 +487,13 @@ __device__ __forceinline__ void slab_pos(const Slab& sl, int v, int RP, int& r, 
     }
 }
 
+// mode 3: eight transposing 2-byte stores (consecutive lanes on consecutive k)
+__device__ __forceinline__ void store8(uint16_t* __restrict__ S, int r, int kk, uint4 v) {
+    const uint32_t w[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+    for (int i = 0; i < 8; ++i) S[(r + i) * RTKP + kk] = (uint16_t)(w[i >> 1] >> (16 * (i & 1)));
+}
+
 __device__ __forceinline__ void store4(uint16_t* __restrict__ S, const Slab& sl, int r, int kk, uint2 v) {
     const uint16_t e[4] = {(uint16_t)(v.x & 0xffffu), (uint16_t)(v.x >> 16), (uint16_t)(v.y & 0xffffu),
                            (uint16_t)(v.y >> 16)};
@@ -367,7 +507,7 @@ __device__ __forceinline__ void store4(uint16_t* __restrict__ S, const Slab& sl,
 
 constexpr int RW = 16;  // waves per reduction block: ≤ 4 accumulator tiles per wave for 256×256
 
-template <int TPW>
+template <int TPW, bool VA, bool VB>
 __global__ __launch_bounds__(64 * RW) void k_gemm_reduce(int M, int N, int K, int ks, const uint16_t* __restrict__ A,
                                                      Slab sa, const uint16_t* __restrict__ B, Slab sb,
                                                      float* __restrict__ part) {
@@ -378,38 +518,64 @@ __global__ __launch_bounds__(64 * RW) void k_gemm_reduce(int M, int N, int K, in
     uint16_t* sB = lds + MP * RTKP;  // [NP][RTKP]
     const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
     const int kb = blockIdx.x * ks, ke = min(K, kb + ks);
-    // vectors per slab: MP·RTK/4 = 512·MT for A, 512·NT for B → ≤ 4 per thread each
+    // vectors per slab: MP·RTK/4 = 512·MT for A, 512·NT for B → ≤ 4 per thread each;
+    // an operand in mode 3 (VA, VB) takes 16-byte vectors, half as many → ≤ 2 per thread
     constexpr int NT_ = 64 * RW;
-    uint2 ra[4], rb[4];
+    constexpr int NQA = VA ? 2 : 4, NQB = VB ? 2 : 4;
+    using VecA = typename std::conditional<VA, uint4, uint2>::type;
+    using VecB = typename std::conditional<VB, uint4, uint2>::type;
+    const int nva = (VA ? 256 : 512) * MT, nvb = (VB ? 256 : 512) * NT;
+    VecA ra[NQA];
+    VecB rb[NQB];
     auto fetch = [&](int k0) {
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
+        for (int q = 0; q < NQA; ++q) {
             const int v = tid + NT_ * q;
-            if (v < 512 * MT) {
+            if (v < nva) {
                 int r, kk;
-                slab_pos(sa, v, MP, r, kk);
-                ra[q] = load4(A, sa, r, k0 + kk, M, ke);
+                slab_pos<VA>(sa, v, MP, r, kk);
+                if constexpr (VA)
+                    ra[q] = load8(A, sa, r, k0 + kk, M, ke);
+                else
+                    ra[q] = load4(A, sa, r, k0 + kk, M, ke);
             }
-            if (v < 512 * NT) {
+        }
+#pragma unroll
+        for (int q = 0; q < NQB; ++q) {
+            const int v = tid + NT_ * q;
+            if (v < nvb) {
                 int r, kk;
-                slab_pos(sb, v, NP, r, kk);
-                rb[q] = load4(B, sb, r, k0 + kk, N, ke);
+                slab_pos<VB>(sb, v, NP, r, kk);
+                if constexpr (VB)
+                    rb[q] = load8(B, sb, r, k0 + kk, N, ke);
+                else
+                    rb[q] = load4(B, sb, r, k0 + kk, N, ke);
             }
         }
     };
     auto commit = [&]() {
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
+        for (int q = 0; q < NQA; ++q) {
             const int v = tid + NT_ * q;
-            if (v < 512 * MT) {
+            if (v < nva) {
                 int r, kk;
-                slab_pos(sa, v, MP, r, kk);
-                store4(sA, sa, r, kk, ra[q]);
+                slab_pos<VA>(sa, v, MP, r, kk);
+                if constexpr (VA)
+                    store8(sA, r, kk, ra[q]);
+                else
+                    store4(sA, sa, r, kk, ra[q]);
             }
-            if (v < 512 * NT) {
+        }
+#pragma unroll
+        for (int q = 0; q < NQB; ++q) {
+            const int v = tid + NT_ * q;
+            if (v < nvb) {
                 int r, kk;
-                slab_pos(sb, v, NP, r, kk);
-                store4(sB, sb, r, kk, rb[q]);
+                slab_pos<VB>(sb, v, NP, r, kk);
+                if constexpr (VB)
+                    store8(sB, r, kk, rb[q]);
+                else
+                    store4(sB, sb, r, kk, rb[q]);
             }
         }
     };
@@ -538,7 +704,9 @@ int gemm_tall(int M, int N, int Kd, const uint16_t* A, int64_t sam, int64_t sak,
         NTW = 2;
     }
     const int NP = 32 * NTW * (4 / WR);
-    const size_t lds = ((size_t)NP * (KP + 8) + (size_t)32 * WR * TKP) * sizeof(uint16_t);
+    // the A stage doubles as the bf16 output staging: 2 KB per wave
+    const size_t stage = (size_t)32 * WR * TKP > 4096 ? (size_t)32 * WR * TKP : 4096;
+    const size_t lds = ((size_t)NP * (KP + 8) + stage) * sizeof(uint16_t);
     if (lds > 96 * 1024) return ARMSP_SUCCESS;
     int va = 1;
     if (sak == 1 && sam % 8 == 0 && ((uintptr_t)A % 16) == 0)
@@ -583,7 +751,9 @@ int gemm_f32_out(int M, int N, int Kd, const uint16_t* A, int64_t sam, int64_t s
     // (A = xᵀ: sam == 1; B = dY: sbn == 1), else along k; 8-byte aligned
     auto slab = [](const uint16_t* P, int64_t srow, int64_t sk, Slab& sl) {
         const bool al = ((uintptr_t)P % 8) == 0;
-        if (al && srow == 1 && sk % 4 == 0)
+        if (((uintptr_t)P % 16) == 0 && srow == 1 && sk % 8 == 0)
+            sl = Slab{3, srow, sk};
+        else if (al && srow == 1 && sk % 4 == 0)
             sl = Slab{0, srow, sk};
         else if (al && sk == 1 && srow % 4 == 0)
             sl = Slab{1, srow, sk};
@@ -606,14 +776,29 @@ int gemm_f32_out(int M, int N, int Kd, const uint16_t* A, int64_t sam, int64_t s
             if (st) return st;
             float* part = static_cast<float*>(ws);
             TimedLaunch t("gemm", s);
-#define ARMSP_RED(TT)                                                                                              \
-    hipLaunchKernelGGL((k_gemm_reduce<TT>), dim3(nblk), dim3(64 * RW), lds, s, M, N, Kd, ks, A, sa, B, sb, part)
-            if (tpw <= 1)
-                ARMSP_RED(1);
-            else if (tpw <= 2)
-                ARMSP_RED(2);
+            // 16-byte vectors for an operand that is row-contiguous with 16-byte aligned rows
+            const bool va16 = sa.along == 3, vb16 = sb.along == 3;
+#define ARMSP_RED(TT, AA, BB)                                                                                      \
+    hipLaunchKernelGGL((k_gemm_reduce<TT, AA, BB>), dim3(nblk), dim3(64 * RW), lds, s, M, N, Kd, ks, A, sa, B, sb, \
+                       part)
+#define ARMSP_RED_T(AA, BB)       \
+    do {                          \
+        if (tpw <= 1)             \
+            ARMSP_RED(1, AA, BB); \
+        else if (tpw <= 2)        \
+            ARMSP_RED(2, AA, BB); \
+        else                      \
+            ARMSP_RED(4, AA, BB); \
+    } while (0)
+            if (va16 && vb16)
+                ARMSP_RED_T(true, true);
+            else if (va16)
+                ARMSP_RED_T(true, false);
+            else if (vb16)
+                ARMSP_RED_T(false, true);
             else
-                ARMSP_RED(4);
+                ARMSP_RED_T(false, false);
+#undef ARMSP_RED_T
 #undef ARMSP_RED
             hipError_t e = hipGetLastError();
             if (e == hipSuccess) {

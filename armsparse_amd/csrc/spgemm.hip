@@ -16,12 +16,6 @@ namespace {
 
 constexpr int kTN = 4096;  // columns per wave tile (16 KiB of LDS per wave; 1024 and 2048 time the same)
 
-__device__ __forceinline__ void wave_sync_lds() {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
-
 __global__ __launch_bounds__(256) void k_spgemm_dense(int M, int N, int ntiles, const int* __restrict__ rpA,
                                                       const int* __restrict__ ciA, const float* __restrict__ vA,
                                                       const int* __restrict__ rpB, const int* __restrict__ ciB,

@@ -21,12 +21,6 @@
 namespace armsp {
 namespace {
 
-__device__ __forceinline__ void wave_sync_lds() {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
-
 template <int G>
 __global__ __launch_bounds__(256) void k_spmv_row(int M, const int* __restrict__ rowptr, const int* __restrict__ colind,
                                                   const float* __restrict__ val, const float* __restrict__ x,

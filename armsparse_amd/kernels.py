@@ -480,15 +480,21 @@ def workspace_reserve(nbytes: int = 0, device: Optional[torch.device] = None) ->
     return r.value
 
 
-def to_bf16(x: torch.Tensor, pitched: bool = False) -> torch.Tensor:
+def to_bf16(x: torch.Tensor, pitched: bool = False, rows16: bool = False) -> torch.Tensor:
     """float32 → bfloat16 (round to nearest even) with the library kernel;
     ``pitched`` (2-D x with unit column stride): the result's rows at row_pitch,
-    for an SpMM that gathers them."""
+    for an SpMM that gathers them; ``rows16``: rows padded to a multiple of 16
+    bytes, so the GEMM reads them with 16-byte loads."""
     _require(x.is_cuda and x.dtype == torch.float32, "to_bf16: needs a float32 device tensor")
-    if pitched:
+    if pitched or rows16:
         _require(x.dim() == 2 and x.stride(1) == 1, "to_bf16: pitched output needs a 2-D x with unit column stride")
         m, n = x.shape
-        y = empty_rows(m, n, torch.bfloat16, x.device)
+        if pitched:
+            y = empty_rows(m, n, torch.bfloat16, x.device)
+        else:
+            p = (n + 7) // 8 * 8
+            y = torch.empty((m, p), dtype=torch.bfloat16, device=x.device)[:, :n] if p != n else \
+                torch.empty((m, n), dtype=torch.bfloat16, device=x.device)
         call("armsp_convert_rows_f32_to_bf16", m, n, _ptr(x), x.stride(0), _ptr(y), y.stride(0), _stream(x.device))
         return y
     x = x.contiguous()

@@ -169,7 +169,8 @@ class _BF16Linear(torch.autograd.Function):
     @staticmethod
     def backward(ctx, gy):
         xb, wb = ctx.saved_tensors
-        gyb = gy.contiguous() if gy.dtype == torch.bfloat16 else kernels.to_bf16(gy.contiguous())
+        # rows on 16 bytes: both GEMMs read them with 16-byte loads
+        gyb = gy.contiguous() if gy.dtype == torch.bfloat16 else kernels.to_bf16(_rows(gy), rows16=True)
         gx = gw = None
         if ctx.needs_input_grad[0]:
             gx = kernels.gemm_bf16(gyb, wb.t(), out_dtype=ctx.x_dtype)  # [M×H]·[H×F]
@@ -186,7 +187,7 @@ class _LinearBiasReluDropout(torch.autograd.Function):
 
     @staticmethod
     def forward(ctx, z, weight, bias, p, seed):
-        zb = z if z.dtype == torch.bfloat16 else kernels.to_bf16(z.contiguous())
+        zb = z if z.dtype == torch.bfloat16 else kernels.to_bf16(_rows(z), rows16=True)  # 16-byte GEMM loads
         wb = kernels.to_bf16(weight.detach())
         y = kernels.gemm_bf16(zb, wb)
         h, mask = kernels.bias_relu_dropout(y, None if bias is None else bias.detach(), p, seed)
