@@ -104,6 +104,7 @@ _SIGS = {
     "armsp_csr_transpose": (c_int, [_I, _I, _I, _P, _P, _P, _P, _P, _P, _P]),
     "armsp_coo_to_csr": (c_int, [_I, _I, _I, _P, _P, _P, _I, _P, _P, _P, _P, _P]),
     "armsp_csr_to_vcsr_device": (c_int, [_I, _I, _I, _P, _P, _P, c_float, _P, _P, _P, c_int64, _P, _P]),
+    "armsp_csr_column_split": (c_int, [_I, _I, _I, _P, _P, _P, _I, _P, _P, _P, _P, _P]),
     "armsp_spmm_max_f32": (c_int, [_I, _I, _I, _I, _P, _P, _P, _P, _I, c_float, _P, _I, _P, _P]),
     "armsp_spmm_datareuse_f32": (c_int, [_I, _I, _I, _I, _P, _P, _P, _P, _P, _I, _P, _I, _I, _P]),
     "armsp_spmm_max_backward_f32": (c_int, [_I, _I, _I, _I, _P, _P, _P, _I, _P, _P, _I, _P]),

@@ -271,6 +271,85 @@ void radix_sort(int n, int range, const int* keys_in, const int* idx_in, int* ke
     *iout_final = iin;
 }
 
+// ------------------------------------------------ column blocks of a CSR matrix
+// A CSR matrix split into nb blocks of columns [x·W, (x+1)·W), each a CSR matrix
+// of its own over all M rows with the row's entries in their order.  Block x of
+// row r is a run of the row's entries (columns increase along a row), found by a
+// binary search per block boundary: one wave per row, lane x searching boundary
+// x.  The (block, row) counts in block-major order are scanned once, which places
+// every run; a second pass copies the runs with the row's wave.
+
+// entries of row [a, e) with column < c (a binary search)
+__device__ __forceinline__ int cols_below(const int* __restrict__ colind, int a, int e, int c) {
+    int lo = a, hi = e;
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (colind[mid] < c)
+            lo = mid + 1;
+        else
+            hi = mid;
+    }
+    return lo;
+}
+
+// cnt[x·M + r] = entries of row r in block x; cnt[nb·M] = 0 (the scan total)
+__global__ __launch_bounds__(256) void k_split_count(int M, int nb, int W, const int* __restrict__ rowptr,
+                                                     const int* __restrict__ colind, int* __restrict__ cnt) {
+    const int lane = threadIdx.x & 63;
+    const int r = uniform((int)(blockIdx.x * 4u + (threadIdx.x >> 6)));
+    if (r == M && lane == 0) cnt[(int64_t)nb * M] = 0;
+    if (r >= M) return;
+    const int a = rowptr[r], e = rowptr[r + 1];
+    const int sx = lane <= nb ? (lane == 0 ? a : (lane == nb ? e : cols_below(colind, a, e, lane * W))) : e;
+    const int nx = __shfl(sx, lane + 1 <= nb ? lane + 1 : nb);
+    if (lane < nb) cnt[(int64_t)lane * M + r] = nx - sx;
+}
+
+// out_rowptr[x·(M+1) + r] = pos[x·M + r] − pos[x·M], r = 0..M
+__global__ void k_split_rowptr(int M, int nb, const int* __restrict__ pos, int* __restrict__ out_rowptr) {
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= (int64_t)nb * (M + 1)) return;
+    const int x = (int)(i / (M + 1)), r = (int)(i - (int64_t)x * (M + 1));
+    const int64_t b = (int64_t)x * M;
+    out_rowptr[i] = pos[b + r] - pos[b];
+}
+
+// base[x] = pos[x·M], x = 0..nb (the blocks' first entries; base[nb] = nnz)
+__global__ void k_split_base(int M, int nb, const int* __restrict__ pos, int* __restrict__ base) {
+    const int x = threadIdx.x;
+    if (x <= nb) base[x] = pos[(int64_t)x * M];
+}
+
+__global__ __launch_bounds__(256) void k_split_scatter(int M, int nb, int W, const int* __restrict__ rowptr,
+                                                       const int* __restrict__ colind, const float* __restrict__ val,
+                                                       const int* __restrict__ pos, int* __restrict__ out_colind,
+                                                       float* __restrict__ out_val) {
+    const int lane = threadIdx.x & 63;
+    const int r = uniform((int)(blockIdx.x * 4u + (threadIdx.x >> 6)));
+    if (r >= M) return;
+    const int a = rowptr[r], e = rowptr[r + 1];
+    const int sx = lane <= nb ? (lane == 0 ? a : (lane == nb ? e : cols_below(colind, a, e, lane * W))) : e;
+    for (int x = 0; x < nb; ++x) {
+        const int s0 = __shfl(sx, x), s1 = __shfl(sx, x + 1);
+        const int64_t d = pos[(int64_t)x * M + r];
+        for (int p = s0 + lane; p < s1; p += 64) {
+            out_colind[d + (p - s0)] = colind[p];
+            if (out_val) out_val[d + (p - s0)] = val ? val[p] : 1.0f;
+        }
+    }
+}
+
+// err bit 2: a column out of [0, K) or not above the previous one in its row
+// (start[p] = 1 where a non-empty row begins, from k_vcsr_rows)
+__global__ void k_cols_increasing(int nnz, int K, const int* __restrict__ colind, const uint8_t* __restrict__ start,
+                                  int* __restrict__ err) {
+    const int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (p >= nnz) return;
+    const int c = colind[p];
+    const bool first = p == 0 || start[p] != 0;
+    if (c < 0 || c >= K || (!first && colind[p - 1] >= c)) atomicOr(err, 2);
+}
+
 // ------------------------------------------------ CSR → vector-aware CSR
 // toVectorAwareCSR (SpMV.cpp:15-38) on the device, from CSR instead of a dense
 // matrix.  A row's aligned block of columns [4b, 4b+4) is stored iff one of the
@@ -563,6 +642,68 @@ extern "C" int armsp_csr_to_vcsr_device(int M, int K, int nnz, const int* rowptr
     e = hipGetLastError();
     ws_free(ws, s);
     if (e != hipSuccess) return hip_fail(e);
+    t.stop();
+    return ARMSP_SUCCESS;
+}
+
+extern "C" int armsp_csr_column_split(int M, int K, int nnz, const int* rowptr, const int* colind, const float* val,
+                                      int nblocks, int* out_rowptr, int* out_colind, float* out_val,
+                                      int64_t* block_start, void* stream) {
+    using namespace armsp;
+    if (M < 0 || K < 0 || nnz < 0 || nblocks < 1 || nblocks > 63) return ARMSP_ERROR_INVALID_VALUE;
+    if (!rowptr || !out_rowptr || !block_start || (nnz > 0 && (!colind || !out_colind))) return ARMSP_ERROR_INVALID_VALUE;
+    if (val && !out_val) return ARMSP_ERROR_INVALID_VALUE;
+    if (M == 0 && nnz > 0) return ARMSP_ERROR_INVALID_VALUE;
+    if ((int64_t)nblocks * M + 1 > INT_MAX) return ARMSP_ERROR_NOT_SUPPORTED;
+    hipStream_t s = as_stream(stream);
+    TimedLaunch t("csr_column_split", s);
+    const int W = K > 0 ? (K + nblocks - 1) / nblocks : 1;
+    const int64_t ncnt = (int64_t)nblocks * M + 1;  // (block, row) counts and the total
+    const size_t c4 = ((size_t)ncnt * 4 + 255) & ~(size_t)255;
+    const size_t s4 = ((size_t)scan_ws_ints(ncnt) * 4 + 255) & ~(size_t)255;
+    const size_t b1 = ((size_t)(nnz > 0 ? nnz : 1) + 255) & ~(size_t)255;
+    void* ws = nullptr;
+    int st = ws_alloc(&ws, c4 + s4 + b1 + 512, s);
+    if (st) return st;
+    char* w = static_cast<char*>(ws);
+    int* pos = reinterpret_cast<int*>(w);
+    int* scanws = reinterpret_cast<int*>(w + c4);
+    uint8_t* start = reinterpret_cast<uint8_t*>(w + c4 + s4);
+    int* err = reinterpret_cast<int*>(w + c4 + s4 + b1);
+    int* base = err + 1;  // nblocks + 1 ints
+    auto fail = [&](int code) {
+        ws_free(ws, s);
+        return code;
+    };
+    // canonical CSR: rowptr from 0 to nnz, non-decreasing; columns in range and increasing along a row
+    const unsigned gs = (unsigned)(((int64_t)(nnz > 0 ? nnz : 1) + 255) / 256);
+    hipLaunchKernelGGL(k_vcsr_clear, dim3(gs), dim3(256), 0, s, (int64_t)nnz, start, err);
+    if (M > 0)
+        hipLaunchKernelGGL(k_vcsr_rows, dim3((unsigned)(((int64_t)M + 255) / 256)), dim3(256), 0, s, M, nnz, rowptr,
+                           start, err);
+    if (nnz > 0) hipLaunchKernelGGL(k_cols_increasing, dim3(gs), dim3(256), 0, s, nnz, K, colind, start, err);
+    int h = 0;
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipMemcpyAsync(&h, err, sizeof(int), hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e != hipSuccess) return fail(hip_fail(e));
+    if (h) return fail(ARMSP_ERROR_INVALID_VALUE);
+    const unsigned gr = (unsigned)(((int64_t)M + 1 + 3) / 4);  // a wave per row, and one for the total
+    hipLaunchKernelGGL(k_split_count, dim3(gr), dim3(256), 0, s, M, nblocks, W, rowptr, colind, pos);
+    exclusive_scan(ncnt, pos, scanws, s);
+    hipLaunchKernelGGL(k_split_rowptr, dim3((unsigned)(((int64_t)nblocks * (M + 1) + 255) / 256)), dim3(256), 0, s, M,
+                       nblocks, (const int*)pos, out_rowptr);
+    if (nnz > 0)
+        hipLaunchKernelGGL(k_split_scatter, dim3((unsigned)(((int64_t)M + 3) / 4)), dim3(256), 0, s, M, nblocks, W,
+                           rowptr, colind, val, (const int*)pos, out_colind, out_val);
+    hipLaunchKernelGGL(k_split_base, dim3(1), dim3(64), 0, s, M, nblocks, (const int*)pos, base);
+    int hb[64];
+    e = hipGetLastError();
+    if (e == hipSuccess) e = hipMemcpyAsync(hb, base, sizeof(int) * (size_t)(nblocks + 1), hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    ws_free(ws, s);
+    if (e != hipSuccess) return hip_fail(e);
+    for (int x = 0; x <= nblocks; ++x) block_start[x] = hb[x];
     t.stop();
     return ARMSP_SUCCESS;
 }

@@ -240,6 +240,48 @@ def csr_to_vcsr(rowptr: torch.Tensor, colind: torch.Tensor, values: Optional[tor
     return tuple(t.cpu() for t in res) if host else res
 
 
+def column_blocks(rowptr: torch.Tensor, colind: torch.Tensor, values: Optional[torch.Tensor], num_cols: int,
+                  nblocks: int):
+    """[(rowptr_x, colind_x, values_x)]: the matrix split into nblocks blocks of
+    columns (armsp_csr_column_split), each a CSR matrix over all rows; device tensors."""
+    fn = "column_blocks"
+    _check_csr(fn, rowptr, colind, values)
+    _require(rowptr.is_cuda, f"{fn}: needs device tensors")
+    m, nnz = rowptr.numel() - 1, colind.numel()
+    dev = rowptr.device
+    out_rp = torch.empty(nblocks * (m + 1), dtype=torch.int32, device=dev)
+    out_ci = torch.empty(max(nnz, 1), dtype=torch.int32, device=dev)
+    out_va = torch.empty(max(nnz, 1), dtype=torch.float32, device=dev) if values is not None else None
+    starts = (ctypes.c_int64 * (nblocks + 1))()
+    call("armsp_csr_column_split", m, num_cols, nnz, _ptr(rowptr), _ptr(colind), _ptr(values), nblocks,
+         _ptr(out_rp), _ptr(out_ci), _ptr(out_va), starts, _stream(dev))
+    return [(out_rp[x * (m + 1):(x + 1) * (m + 1)], out_ci[starts[x]:starts[x + 1]],
+             None if out_va is None else out_va[starts[x]:starts[x + 1]]) for x in range(nblocks)]
+
+
+def gather_blocks(num_cols: int, row_bytes: int, budget: int = 4 << 20) -> int:
+    """Column blocks for a product whose gathered rows of B (num_cols rows of
+    row_bytes) outgrow one XCD's 4 MiB L2: the blocks' rows of B fit the budget
+    (1: no split).  Measured on the MI355X, N=3 fp32 over a 2.4M-node graph with
+    126M edges: 1986 us in one product, 1347 us in 4 column blocks, 1338 in 8."""
+    n = -(-num_cols * row_bytes // budget)
+    return max(1, min(63, n))
+
+
+def spmm_blocks(blocks, dense: torch.Tensor, *, row_scale: Optional[torch.Tensor] = None,
+                bias: Optional[torch.Tensor] = None, out: Optional[torch.Tensor] = None) -> torch.Tensor:
+    """out = row_scale·(Σ_x A_x·dense) + bias over column blocks of A (column_blocks),
+    block after block: the first writes out (bias included), the others add to it."""
+    first = True
+    for rp, ci, va in blocks:
+        if not first and ci.numel() == 0:
+            continue
+        out = spmm(rp, ci, va, dense, out=out, accumulate=not first, row_scale=row_scale,
+                   bias=bias if first else None)
+        first = False
+    return out
+
+
 # ---------------------------------------------------------------- SDDMM
 def sddmm(rows: torch.Tensor, colind: torch.Tensor, U: torch.Tensor, V: torch.Tensor, *,
           scale: Optional[torch.Tensor] = None, coo: bool = False, config: ConfigLike = None) -> torch.Tensor:

@@ -105,8 +105,14 @@ class GCNAggregate(torch.autograd.Function):
     (fp32 accumulation; y and the gradients are fp32)."""
 
     @staticmethod
-    def forward(ctx, h, rowptr, colind, colptr, rowind, coef_fwd, coef_bwd, in_norm, out_norm, bias):
-        y = kernels.spmm(rowptr, colind, coef_fwd, h, row_scale=in_norm, bias=bias)
+    def forward(ctx, h, rowptr, colind, colptr, rowind, coef_fwd, coef_bwd, in_norm, out_norm, bias, blocks=None):
+        # blocks: the graph's column blocks (forward, backward) for narrow features,
+        # whose rows outgrow the L2 (kernels.spmm_blocks); None: one product
+        if blocks is not None:
+            y = kernels.spmm_blocks(blocks[0], h, row_scale=in_norm, bias=bias)
+        else:
+            y = kernels.spmm(rowptr, colind, coef_fwd, h, row_scale=in_norm, bias=bias)
+        ctx.blocks = None if blocks is None else blocks[1]
         ctx.graph = (colptr, rowind, coef_bwd, out_norm, h.dtype)
         # learnable edge weights reach coef_fwd (= w_csr·out_norm[colind]); its gradient
         # needs h and the forward pattern
@@ -120,11 +126,14 @@ class GCNAggregate(torch.autograd.Function):
         dh = dcoef = None
         if ctx.needs_input_grad[0]:
             gb = kernels.to_bf16(g.float(), pitched=True) if hdt == torch.bfloat16 else g
-            dh = kernels.spmm(colptr, rowind, coef_bwd, gb, row_scale=out_norm).to(hdt)
+            if ctx.blocks is not None:
+                dh = kernels.spmm_blocks(ctx.blocks, gb, row_scale=out_norm).to(hdt)
+            else:
+                dh = kernels.spmm(colptr, rowind, coef_bwd, gb, row_scale=out_norm).to(hdt)
         if ctx.edge_grad is not None:
             dcoef = _edge_coef_grad(g, *ctx.edge_grad)
         dbias = kernels.colsum(gy.float()) if ctx.needs_input_grad[9] else None
-        return dh, None, None, None, None, dcoef, None, None, None, dbias
+        return dh, None, None, None, None, dcoef, None, None, None, dbias, None
 
 
 def _edge_coef_grad(gy, h, rowptr, colind, in_norm):
@@ -284,6 +293,7 @@ class GCNConv(torch.nn.Module):
         self.cached_result = None
         self.cached_num_edges = None
         self.cached_coef = None
+        self.cached_blocks = None  # (graph key, (forward blocks, backward blocks))
 
     @staticmethod
     def in_deg_sqrt(indptr):
@@ -344,6 +354,26 @@ class GCNConv(torch.nn.Module):
                 self.cached_coef = coefs
         return in_n, out_n, coefs
 
+    def _column_blocks(self, h, rowptr, colind, colptr, rowind, coefs):
+        """Column blocks of the graph for an aggregation of narrow features (at most 4
+        per node) whose rows outgrow the L2, cached with the coefficients (cached=True
+        and coefficients that need no gradient); None: one product per aggregation."""
+        if not (self.cached and h.dim() == 2 and h.shape[1] <= 4 and h.is_cuda):
+            return None
+        if coefs[0].requires_grad or coefs[1].requires_grad:
+            return None
+        row_bytes = h.shape[1] * h.element_size()
+        nb = kernels.gather_blocks(h.shape[0], row_bytes)
+        if nb < 2:
+            return None
+        key = (rowptr.data_ptr(), colind.data_ptr(), colptr.data_ptr(), rowind.data_ptr(), nb)
+        if self.cached_blocks is None or self.cached_blocks[0] != key:
+            m = rowptr.numel() - 1
+            fwd = kernels.column_blocks(rowptr, colind, coefs[0], h.shape[0], nb)
+            bwd = kernels.column_blocks(colptr, rowind, coefs[1], m, kernels.gather_blocks(m, row_bytes))
+            self.cached_blocks = (key, (fwd, bwd))
+        return self.cached_blocks[1]
+
     def _forward_aggregate_first(self, x, rowptr, colind, colptr, rowind, edge_weight_csr, edge_weight_csc):
         """(diag(in_norm)·A·diag(out_norm)·x)·W + bias: the SpMM gathers in_channels per
         edge; x·W's rounding moves after the aggregation (bf16: x and the aggregate are
@@ -373,8 +403,9 @@ class GCNConv(torch.nn.Module):
             # next to nothing, so those stay fp32 (no conversions either way)
             in_n, out_n, coefs = self._norms_and_coefs(rowptr, colind, colptr, rowind, edge_weight_csr,
                                                        edge_weight_csc)
-            return GCNAggregate.apply(x.contiguous(), rowptr, colind, colptr, rowind, coefs[0], coefs[1], in_n,
-                                      out_n, self.bias)
+            x = x.contiguous()
+            return GCNAggregate.apply(x, rowptr, colind, colptr, rowind, coefs[0], coefs[1], in_n, out_n,
+                                      self.bias, self._column_blocks(x, rowptr, colind, colptr, rowind, coefs))
         if not self.cached or self.cached_result is None:
             if self.normalize:
                 in_deg_norm = self.in_deg_sqrt(rowptr)

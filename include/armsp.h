@@ -309,6 +309,23 @@ int armsp_csr_to_vcsr_device(int M, int K, int nnz, const int* rowptr, const int
                              float threshold, int* out_rowptr, int* out_colind, float* out_val, int64_t capacity,
                              int64_t* padded, void* stream);
 
+/* A CSR matrix split into nblocks (1..63) blocks of columns [x·W, (x+1)·W),
+ * W = ceil(K / nblocks), each a CSR matrix over all M rows whose rows keep
+ * their entries' order: block x's row pointers are
+ * out_rowptr[x·(M+1) .. x·(M+1)+M] (from 0), its entries
+ * out_colind/out_val[block_start[x] .. block_start[x+1]) (block_start: host,
+ * nblocks+1 entries).  A product whose gathered rows of B outgrow the L2 then
+ * runs block after block with a block's rows of B cached (the product over the
+ * blocks, accumulated, is A·B; the sums are ordered per block).  No counterpart
+ * in the reference: it is the column split of the NUMA drivers
+ * (numaAware/numaAwareSDMM.cpp:100-113 DistributetoCSR), taken in time instead
+ * of across nodes.  The input must be canonical CSR (rowptr from 0 to nnz,
+ * columns in [0, K) strictly increasing in a row; else
+ * ARMSP_ERROR_INVALID_VALUE).  Synchronises the stream. */
+int armsp_csr_column_split(int M, int K, int nnz, const int* rowptr, const int* colind, const float* val,
+                           int nblocks, int* out_rowptr, int* out_colind, float* out_val, int64_t* block_start,
+                           void* stream);
+
 /* -------------------------------------------------------- dense helpers */
 /* C[M×N] (=|+=) op(A)[M×Kd] · op(B)[Kd×N] in bfloat16 on MFMA with fp32
  * accumulation; element (i,k) of op(A) is A[i*sam + k*sak], element (k,j) of

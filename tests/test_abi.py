@@ -80,6 +80,12 @@ def test_status_and_invalid_arguments_without_gpu():
     assert lib.armsp_csr_to_vcsr_device(4, 4, 4, p8, p8, None, 0.0, p8, None, None, 0, None, None) == 1
     assert lib.armsp_csr_to_vcsr_device(4, 4, 4, p8, p8, None, 0.0, p8, p8, None, 16, ctypes.byref(n), None) == 1
     assert lib.armsp_csr_to_vcsr_device(0, 4, 3, p8, p8, None, 0.0, p8, None, None, 0, ctypes.byref(n), None) == 1
+    # column split: block count 1..63, the host start array, the value output when values are given
+    starts = (ctypes.c_int64 * 65)()
+    assert lib.armsp_csr_column_split(4, 8, 4, p8, p8, None, 0, p8, p8, None, starts, None) == 1
+    assert lib.armsp_csr_column_split(4, 8, 4, p8, p8, None, 64, p8, p8, None, starts, None) == 1
+    assert lib.armsp_csr_column_split(4, 8, 4, p8, p8, None, 2, p8, p8, None, None, None) == 1
+    assert lib.armsp_csr_column_split(4, 8, 4, p8, p8, p8, 2, p8, p8, None, starts, None) == 1
     cfg = _lib.Config.make("row", group=3)
 
     # group must be a power of two in [4, 64]: rejected on the host

@@ -202,6 +202,35 @@ def test_fused_gcn_layer_matches_unfused(graph512):
         assert np.max(np.abs(a - b)) <= 1e-5 * (np.max(np.abs(a)) + 1e-12)
 
 
+def test_narrow_layer_over_column_blocks(graph512, monkeypatch):
+    """A fused 3-class layer whose aggregations run over column blocks (forced to 4
+    here; the library splits only graphs whose gathered rows outgrow the L2) gives
+    the one-product layer's output and gradients."""
+    g, *_ = graph512
+    gt = dev(g.rowptr, g.colind, g.colptr, g.rowind)
+    torch.manual_seed(2)
+    one = GCNConv(40, 3, cached=True, fused=True).cuda()
+    blk = GCNConv(40, 3, cached=True, fused=True).cuda()
+    blk.load_state_dict(one.state_dict())
+    with torch.no_grad():
+        one.bias.uniform_(-0.2, 0.2)
+        blk.bias.copy_(one.bias)
+    x = torch.rand(512, 40, device="cuda")
+    G = torch.randn(512, 3, device="cuda")
+    res = []
+    for conv, nb in ((one, 1), (blk, 4)):
+        monkeypatch.setattr(kernels, "gather_blocks", lambda *a, nb=nb, **k: nb)
+        xi = x.clone().requires_grad_(True)
+        y = conv(xi, *gt)
+        y.backward(G)
+        res.append((y.detach().cpu().numpy(), xi.grad.cpu().numpy(), conv.weight.grad.cpu().numpy(),
+                    conv.bias.grad.cpu().numpy()))
+    assert one.cached_blocks is None and blk.cached_blocks is not None
+    assert np.max(oracle.row_rel_err(res[1][0], res[0][0])) <= 1e-5
+    for a, b in zip(res[0][1:], res[1][1:]):
+        assert np.max(np.abs(a - b)) <= 1e-5 * (np.max(np.abs(a)) + 1e-12)
+
+
 def _gcn_dense_fp64(g):
     """Â = diag(1/sqrt(in-degree counts))·A·diag(1/sqrt(out-degree counts)), dense fp64."""
     n = g.rowptr.size - 1

@@ -345,6 +345,59 @@ def test_csr_to_vcsr_device_edges_and_errors():
     assert st == _lib.ERROR_INVALID_VALUE and padded.value > 4
 
 
+# ------------------------------------------------------ column blocks
+@pytest.mark.parametrize("nb", [1, 2, 5, 8, 63])
+@pytest.mark.parametrize("values", [True, False])
+def test_column_split_matches_numpy(nb, values):
+    """Block x holds, row by row in the row's order, the entries with columns in
+    [x·W, (x+1)·W), W = ceil(K/nb); rowptrs from 0; bit-exact values."""
+    case(f"column split nb={nb} values={values}")
+    a = data.rmat_csr(12, 1 << 16, seed=7, values=values)
+    M, K = a.shape
+    W = -(-K // nb)
+    rows = np.repeat(np.arange(M), np.diff(a.rowptr))
+    assert (np.diff(a.rowptr) == 0).any()
+    blocks = kernels.column_blocks(*dev(a.rowptr, a.colind, a.values), K, nb)
+    assert len(blocks) == nb
+    for x, (rp, ci, va) in enumerate(blocks):
+        m = (a.colind >= x * W) & (a.colind < (x + 1) * W)
+        exp_rp = np.concatenate([[0], np.cumsum(np.bincount(rows[m], minlength=M))])
+        assert np.array_equal(rp.cpu().numpy(), exp_rp) and np.array_equal(ci.cpu().numpy(), a.colind[m])
+        if values:
+            assert np.array_equal(va.cpu().numpy().view(np.uint32), a.values[m].view(np.uint32))
+        else:
+            assert va is None
+
+
+def test_column_split_rejects_non_canonical_rows():
+    case("column split errors")
+    for rp, ci in ((np.array([0, 2, 3]), np.array([2, 0, 1])), (np.array([0, 2, 3]), np.array([0, 1, 9])),
+                   (np.array([0, 2, 2]), np.array([0, 1, 3]))):
+        with pytest.raises(RuntimeError):
+            kernels.column_blocks(*dev(rp.astype(np.int32), ci.astype(np.int32), np.ones(3, np.float32)), 8, 2)
+
+
+@pytest.mark.parametrize("n", [1, 3, 4])
+def test_spmm_over_column_blocks(n):
+    """Σ over column blocks (the first writes, the rest add) with the row scale and
+    bias of the fused epilogue: within the FP64 bound of the one-product result."""
+    case(f"spmm blocks n={n}")
+    a = data.rmat_csr(14, 1 << 18, seed=12)
+    M, K = a.shape
+    d_rp, d_ci, d_va = dev(a.rowptr, a.colind, a.values)
+    B = torch.randn(K, n, device="cuda")
+    rs = torch.rand(M, device="cuda") + 0.5
+    bias = torch.randn(n, device="cuda")
+    for nb in (2, 7):
+        blocks = kernels.column_blocks(d_rp, d_ci, d_va, K, nb)
+        got = kernels.spmm_blocks(blocks, B, row_scale=rs, bias=bias).cpu().numpy().astype(np.float64)
+        Bh = B.cpu().numpy()
+        ref = oracle.spmm_f64(a.rowptr, a.colind, a.values, Bh) * rs.cpu().numpy()[:, None].astype(np.float64)
+        ref += bias.cpu().numpy()
+        bound = oracle.spmm_f64(a.rowptr, a.colind, np.abs(a.values), np.abs(Bh)) * rs.cpu().numpy()[:, None]
+        assert np.all(np.abs(got - ref) <= TOL * (bound + np.abs(bias.cpu().numpy())) + 1e-30)
+
+
 def test_spmv_full_rmat_1m():
     case("spmv full 1M R-MAT")
     a = rmat_1m_cached()
