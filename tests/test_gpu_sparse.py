@@ -569,16 +569,21 @@ def test_gemm_bf16_unaligned_views(m, n, k):
 
 
 @pytest.mark.parametrize("m,n,k", [(20000, 256, 100), (20000, 3, 256), (37, 70, 50), (100, 256, 70000),
-                                   (19717, 64, 500)])
+                                   (19717, 64, 500), (20011, 250, 99), (4099, 37, 200)])
 def test_gemm_bf16_output(m, n, k):
     """bf16 output = the fp32 product rounded to nearest even (tall kernel writes it
-    directly, other shapes round a workspace)."""
+    directly, staged as 16-byte rows, other shapes round a workspace); also into rows
+    of a pitch that is not a multiple of 8 (the tall kernel's 2-byte stores)."""
+    case(f"gemm bf16 out {m}x{n}x{k}")
     r = np.random.default_rng(m + n + k + 1)
     a = torch.from_numpy(r.standard_normal((m, k)).astype(np.float32)).to(torch.bfloat16).cuda()
     b = torch.from_numpy(r.standard_normal((k, n)).astype(np.float32)).to(torch.bfloat16).cuda()
     ref = kernels.gemm_bf16(a, b).to(torch.bfloat16)
     out = kernels.gemm_bf16(a, b, out_dtype=torch.bfloat16)
     assert out.dtype == torch.bfloat16 and torch.equal(out, ref)
+    odd = torch.zeros((m, n + 3), dtype=torch.bfloat16, device="cuda")[:, :n]
+    kernels.gemm_bf16(a, b, out=odd, out_dtype=torch.bfloat16)
+    assert torch.equal(odd, ref)
 
 
 # ----------------------------------------------------------------- fills
