@@ -321,16 +321,27 @@ int vcsr_long_blocks() {
 // Lists the long rows (more than long_blocks blocks) in list[0, count[0]) and the
 // medium rows (more than VCSR_MED_MIN blocks, at most long_blocks) in
 // list_med[0, count[2]); the order within a list does not matter, each row is
-// summed by one walker.
+// summed by one walker.  One atomic per wave and list: a wave's listed rows take
+// consecutive slots by ballot rank (an atomic per row serialised about 10^5
+// medium rows of the 1M R-MAT on one counter: 1.5 ms).
 __global__ void k_vcsr_find_long(int M, const int* __restrict__ rowptr, int long_blocks, int* __restrict__ list,
                                  int* __restrict__ list_med, int* __restrict__ count) {
     const int r = blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= M) return;
-    const int n = rowptr[r + 1] - rowptr[r];
-    if (n > 4 * long_blocks)
-        list[atomicAdd(&count[0], 1)] = r;
-    else if (n > 4 * VCSR_MED_MIN)
-        list_med[atomicAdd(&count[2], 1)] = r;
+    const int n = r < M ? rowptr[r + 1] - rowptr[r] : 0;
+    const bool is_long = r < M && n > 4 * long_blocks;
+    const bool is_med = r < M && !is_long && n > 4 * VCSR_MED_MIN;
+    const uint64_t bl = __ballot(is_long), bm = __ballot(is_med);
+    const int lane = (int)__lane_id();
+    const uint64_t below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+    int base_l = 0, base_m = 0;
+    if (lane == 0) {
+        if (bl) base_l = atomicAdd(&count[0], __popcll(bl));
+        if (bm) base_m = atomicAdd(&count[2], __popcll(bm));
+    }
+    base_l = __shfl(base_l, 0);
+    base_m = __shfl(base_m, 0);
+    if (is_long) list[base_l + __popcll(bl & below)] = r;
+    if (is_med) list_med[base_m + __popcll(bm & below)] = r;
 }
 
 // count[0] = listed long rows, count[1] = the long rows' work counter, count[2] =

@@ -639,13 +639,17 @@ def main():
         kernels.vcsr_spmv(rv, cv, vv, xv, y)
         errs["vcsr_spmv"] = spmv_rows_err(Av.rowptr, Av.colind, Av.values, xv.cpu().numpy(), y.cpu().numpy()[rows],
                                           rows)
-        el_w, fam_w = timed(lambda: kernels.vcsr_spmv(rv, cv, vv, xv, y), steps_x * 2, 5, ["spmv_vcsr"])
+        # bit-exact: the whole call (listing the medium and long rows, then the walkers)
+        # with timing events off, then the walkers' kernel on its own
+        el_w, _ = timed(lambda: kernels.vcsr_spmv(rv, cv, vv, xv, y), steps_x * 2, 5, [])
+        _, fam_w = timed(lambda: kernels.vcsr_spmv(rv, cv, vv, xv, y), steps_x * 2, 2, ["spmv_vcsr"])
         pv = int(cv.numel())
         wb = 4 * (M + 1) + 8 * pv + 4 * K + 8 * M
         kw = fam_w["spmv_vcsr"][0] / max(fam_w["spmv_vcsr"][1], 1) * 1e-3
         extras["vcsr_spmv_padded_nnz"] = pv
-        extras["vcsr_spmv_kernel_us"] = round(kw * 1e6, 1)  # bit-exact: one 4-lane group per row
-        extras["vcsr_spmv_GBps"] = round(wb / kw / 1e9, 1)
+        extras["vcsr_spmv_us"] = round(el_w / (steps_x * 2) * 1e6, 1)  # wall per call
+        extras["vcsr_spmv_kernel_us"] = round(kw * 1e6, 1)
+        extras["vcsr_spmv_GBps"] = round(wb * steps_x * 2 / el_w / 1e9, 1)
         el_wb, _ = timed(lambda: kernels.vcsr_spmv(rv, cv, vv, xv, y, balanced=True), steps_x * 2, 5, [])
         extras["vcsr_spmv_balanced_us"] = round(el_wb / (steps_x * 2) * 1e6, 1)  # merge path, wall per call
         extras["vcsr_spmv_balanced_GBps"] = round(wb * steps_x * 2 / el_wb / 1e9, 1)
