@@ -321,27 +321,44 @@ int vcsr_long_blocks() {
 // Lists the long rows (more than long_blocks blocks) in list[0, count[0]) and the
 // medium rows (more than VCSR_MED_MIN blocks, at most long_blocks) in
 // list_med[0, count[2]); the order within a list does not matter, each row is
-// summed by one walker.  One atomic per wave and list: a wave's listed rows take
-// consecutive slots by ballot rank (an atomic per row serialised about 10^5
-// medium rows of the 1M R-MAT on one counter: 1.5 ms).
-__global__ void k_vcsr_find_long(int M, const int* __restrict__ rowptr, int long_blocks, int* __restrict__ list,
-                                 int* __restrict__ list_med, int* __restrict__ count) {
-    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+// summed by one walker.  A workgroup of 1024 rows ranks its listed rows by wave
+// ballots and a scan of the 16 wave counts, and takes its slots with one atomic
+// per list (an atomic per row, or per wave, serialises on the counter: 1.5 ms and
+// 117 us for the 1M R-MAT's 10^5 medium rows).
+constexpr int VCSR_FIND_ROWS = 1024;  // rows per workgroup of k_vcsr_find_long
+
+__global__ __launch_bounds__(VCSR_FIND_ROWS) void k_vcsr_find_long(int M, const int* __restrict__ rowptr,
+                                                                   int long_blocks, int* __restrict__ list,
+                                                                   int* __restrict__ list_med,
+                                                                   int* __restrict__ count) {
+    constexpr int NW = VCSR_FIND_ROWS / 64;
+    __shared__ int s_off[2][NW];  // per wave: its rows' first slot within the workgroup
+    __shared__ int s_base[2];     // the workgroup's first slot in each list
+    const int r = blockIdx.x * VCSR_FIND_ROWS + threadIdx.x;
     const int n = r < M ? rowptr[r + 1] - rowptr[r] : 0;
     const bool is_long = r < M && n > 4 * long_blocks;
     const bool is_med = r < M && !is_long && n > 4 * VCSR_MED_MIN;
     const uint64_t bl = __ballot(is_long), bm = __ballot(is_med);
-    const int lane = (int)__lane_id();
-    const uint64_t below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
-    int base_l = 0, base_m = 0;
+    const int lane = (int)__lane_id(), wid = (int)(threadIdx.x >> 6);
     if (lane == 0) {
-        if (bl) base_l = atomicAdd(&count[0], __popcll(bl));
-        if (bm) base_m = atomicAdd(&count[2], __popcll(bm));
+        s_off[0][wid] = __popcll(bl);
+        s_off[1][wid] = __popcll(bm);
     }
-    base_l = __shfl(base_l, 0);
-    base_m = __shfl(base_m, 0);
-    if (is_long) list[base_l + __popcll(bl & below)] = r;
-    if (is_med) list_med[base_m + __popcll(bm & below)] = r;
+    __syncthreads();
+    if (threadIdx.x < 2) {
+        const int t = (int)threadIdx.x;
+        int acc = 0;
+        for (int w = 0; w < NW; ++w) {
+            const int c = s_off[t][w];
+            s_off[t][w] = acc;
+            acc += c;
+        }
+        s_base[t] = acc > 0 ? atomicAdd(&count[t == 0 ? 0 : 2], acc) : 0;
+    }
+    __syncthreads();
+    const uint64_t below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+    if (is_long) list[s_base[0] + s_off[0][wid] + __popcll(bl & below)] = r;
+    if (is_med) list_med[s_base[1] + s_off[1][wid] + __popcll(bm & below)] = r;
 }
 
 // count[0] = listed long rows, count[1] = the long rows' work counter, count[2] =
@@ -704,7 +721,8 @@ int armsp_spmv_vcsr_f32(int M, int nnz_padded, const int* rowptr, const int* col
         list = count + 64;
         list_med = list + M;
         hipLaunchKernelGGL(k_zero_count, dim3(1), dim3(1), 0, s, count);
-        hipLaunchKernelGGL(k_vcsr_find_long, dim3((unsigned)((M + 255) / 256)), dim3(256), 0, s, M, rowptr,
+        hipLaunchKernelGGL(k_vcsr_find_long, dim3((unsigned)((M + VCSR_FIND_ROWS - 1) / VCSR_FIND_ROWS)),
+                           dim3(VCSR_FIND_ROWS), 0, s, M, rowptr,
                            long_blocks, list, list_med, count);
     }
     TimedLaunch t("spmv_vcsr", s);
