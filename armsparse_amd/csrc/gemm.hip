@@ -335,6 +335,65 @@ __global__ __launch_bounds__(256) void k_gemm_tall(int M, int N, int K, int KP, 
         }
         return;
     }
+    // Longer rows (a multiple of PF k steps) with one column group per wave: the k
+    // steps of the block's tiles form one sequence and the step PF ahead is loaded
+    // into the register slot of the step just staged, so every step waits on a load
+    // issued PF steps earlier (x·W at 19,717 × 500 → 64: one tile of 16 steps per
+    // block).  Only NTW = 1 carries this path: a second accumulator set would cost
+    // the wider kernels their occupancy.
+    if constexpr (NTW == 1) if (sak == 1 && va == 8 && nsteps % PF == 0) {
+        uint4 pre[PF][NVW];
+        const int kk = (tid & 3) * 8;
+        const int my_tiles = (int)blockIdx.x < ntiles ? (ntiles - 1 - (int)blockIdx.x) / (int)gridDim.x + 1 : 0;
+        const int nitems = my_tiles * nsteps;
+        // item i: k step i % nsteps of the block's tile i / nsteps (rows past M read row
+        // M-1, never stored; only a tile's last step reaches past K, zero there)
+        auto fetch = [&](int i, uint4 (&dst)[NVW]) {
+            const int tq = i / nsteps, st = i - tq * nsteps;
+            const int m0 = ((int)blockIdx.x + tq * (int)gridDim.x) * TR;
+            const int gk = st * TK + kk;
+#pragma unroll
+            for (int v = 0; v < NVW; ++v) {
+                const int gm = m0 + ((tid + v * 256) >> 2);
+                const uint16_t* rp = A + (int64_t)(gm < M ? gm : M - 1) * sam + gk;
+                if (st + 1 < nsteps || gk + 8 <= K) {
+                    dst[v] = *reinterpret_cast<const uint4*>(rp);
+                } else {
+                    uint16_t t[8];
+#pragma unroll
+                    for (int e = 0; e < 8; ++e) t[e] = gk + e < K ? rp[e] : (uint16_t)0;
+                    dst[v] = make_uint4(t[0] | ((uint32_t)t[1] << 16), t[2] | ((uint32_t)t[3] << 16),
+                                        t[4] | ((uint32_t)t[5] << 16), t[6] | ((uint32_t)t[7] << 16));
+                }
+            }
+        };
+#pragma unroll
+        for (int j = 0; j < PF; ++j)
+            if (j < nitems) fetch(j, pre[j]);
+        for (int tq = 0; tq < my_tiles; ++tq) {
+            f32x16 acc[NTW];
+#pragma unroll
+            for (int q = 0; q < NTW; ++q)
+#pragma unroll
+                for (int e = 0; e < 16; ++e) acc[q][e] = 0.0f;
+            // a tile's steps come in groups of PF: slot j holds step base + j
+            for (int base = 0; base < nsteps; base += PF) {
+#pragma unroll
+                for (int j = 0; j < PF; ++j) {
+                    const int i = tq * nsteps + base + j;
+#pragma unroll
+                    for (int v = 0; v < NVW; ++v)
+                        *reinterpret_cast<uint4*>(&sA[((tid + v * 256) >> 2) * TKP + kk]) = pre[j][v];
+                    if (i + PF < nitems) fetch(i + PF, pre[j]);
+                    __syncthreads();
+                    mma_step(acc, (base + j) * TK);
+                    __syncthreads();
+                }
+            }
+            store_tile(acc, ((int)blockIdx.x + tq * (int)gridDim.x) * TR);
+        }
+        return;
+    }
     for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
         const int m0 = tile * TR;
         f32x16 acc[NTW];
