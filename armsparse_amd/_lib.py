@@ -74,6 +74,15 @@ class Epilogue(ctypes.Structure):
     _fields_ = [("col_scale", c_void_p), ("row_scale", c_void_p), ("bias", c_void_p), ("relu", ctypes.c_int32)]
 
 
+class AdamTensor(ctypes.Structure):
+    """armsp_adam_tensor (armsp_optim.h): one float32 tensor of an Adam step (device pointers)."""
+
+    _fields_ = [("param", c_void_p), ("grad", c_void_p), ("exp_avg", c_void_p), ("exp_avg_sq", c_void_p),
+                ("step", c_void_p), ("n", ctypes.c_int64), ("weight_decay", ctypes.c_float)]
+
+
+ADAM_MAX_TENSORS = 16
+
 DTYPE_F32, DTYPE_BF16, DTYPE_FP8_E4M3 = 0, 1, 2
 
 _P = c_void_p  # device or host pointer
@@ -126,6 +135,7 @@ _SIGS = {
     "armsp_bias_relu_dropout_bf16": (c_int, [_I, _I, _P, c_int64, _P, c_float, _P, _P, _P, _P]),
     "armsp_bias_relu_dropout_backward_bf16": (c_int, [_I, _I, _P, _P, c_float, _P, c_int64, _P, _P]),
     "armsp_adam_step": (c_int, [c_int64, _P, _P, _P, _P, c_float, c_float, c_float, c_float, c_float, _I, _P]),
+    "armsp_adam_f32": (c_int, [POINTER(AdamTensor), _I, c_double, c_double, c_double, c_double, _P]),
     "armsp_probe_gather_f32": (c_int, [_I, _P, _P, _I, _I, _I, _I, _I, _P, _P]),
     "armsp_selector_set": (c_int, [_I, _I, _I, _I, _P, _P, _P, _P, _P, _P, _P, _P, _P]),
     "armsp_selector_clear": (c_int, [_I]),

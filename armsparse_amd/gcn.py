@@ -15,7 +15,7 @@ import numpy as np
 import torch
 import torch.nn.functional as F
 
-from . import data, kernels
+from . import data, kernels, optim
 from .op import GCNConv
 
 
@@ -60,13 +60,17 @@ class GCNTask:
             self.y_train = self.y.index_select(0, self.train_idx)
 
 
-def _adam(model: Net, capturable: bool):
-    # on the GPU torch's fused Adam: two kernels per parameter group per step
-    # instead of a dozen multi-tensor passes (which dominate a PubMed-size epoch)
-    fused = next(model.parameters()).is_cuda
-    return torch.optim.Adam([dict(params=model.reg_params, weight_decay=5e-4),
-                             dict(params=model.non_reg_params, weight_decay=0)], lr=0.01, capturable=capturable,
-                            fused=fused)
+def _adam(model: Net, capturable: bool, optimizer: str = "native"):
+    """Adam over the two parameter groups of gcn_custom.py:116-119.  "native": the
+    library's kernels (armsparse_amd.optim.Adam, capturable, one call for all four
+    tensors); "torch": torch.optim.Adam, fused on the GPU."""
+    groups = [dict(params=model.reg_params, weight_decay=5e-4), dict(params=model.non_reg_params, weight_decay=0)]
+    if optimizer == "native":
+        return optim.Adam(groups, lr=0.01)
+    if optimizer == "torch":
+        fused = next(model.parameters()).is_cuda
+        return torch.optim.Adam(groups, lr=0.01, capturable=capturable, fused=fused)
+    raise ValueError(f"optimizer must be 'native' or 'torch', got {optimizer!r}")
 
 
 def line_pitched(x: torch.Tensor) -> torch.Tensor:
@@ -86,9 +90,11 @@ def line_pitched(x: torch.Tensor) -> torch.Tensor:
 def make_task(precision: str = "fp32", seed: int = data.CFG_SEED + 3, n: int = 19_717, pairs: int = 44_324,
               features: int = 500, hidden: int = 64, classes: int = 3, feat_density: float = 0.01,
               device: str = "cuda", dropout: float = 0.5, capturable: bool = False,
-              order: str = "transform_first", graph: "data.GCNGraph" = None) -> GCNTask:
+              order: str = "transform_first", graph: "data.GCNGraph" = None, optimizer: str = "native") -> GCNTask:
     """Synthetic features, labels and training nodes on the seeded power-law graph of
-    n nodes and `pairs` undirected edges (or on `graph`, e.g. a Matrix Market file's)."""
+    n nodes and `pairs` undirected edges (or on `graph`, e.g. a Matrix Market file's).
+    `optimizer`: "native" (the library's Adam, always capturable) or "torch"
+    (torch.optim.Adam, capturable when `capturable`)."""
     g = graph if graph is not None else data.gcn_graph(n, pairs, seed)
     n = g.n
     r = np.random.default_rng(seed)
@@ -109,21 +115,26 @@ def make_task(precision: str = "fp32", seed: int = data.CFG_SEED + 3, n: int = 1
         xt = line_pitched(xt.to(torch.bfloat16))
     graph = (t(g.rowptr), t(g.colind), t(g.colptr), t(g.rowind), t(g.value_csr), t(g.value_csc))
     model = Net(features, hidden, classes, precision, dropout, order).to(dev)
-    return GCNTask(model, _adam(model, capturable), xt, graph, t(y), t(train), precision, int(g.colind.size))
+    return GCNTask(model, _adam(model, capturable, optimizer), xt, graph, t(y), t(train), precision,
+                   int(g.colind.size))
 
 
 SCALED = dict(n=2_400_000, pairs=62_000_000, features=100, hidden=256, feat_density=0.1)
 
 
-def retarget(task: GCNTask, precision: str, capturable: bool = False, order: str = "transform_first") -> GCNTask:
+def retarget(task: GCNTask, precision: str, capturable: bool = False, order: str = "transform_first",
+             optimizer: str = None) -> GCNTask:
     """A fresh model and optimizer of `precision` (and layer `order`, which needs
-    bf16's fused aggregation) on the same graph, features and labels."""
+    bf16's fused aggregation) on the same graph, features and labels; the
+    optimizer of the task's kind unless `optimizer` names one."""
+    if optimizer is None:
+        optimizer = "native" if isinstance(task.optimizer, optim.Adam) else "torch"
     x = task.x.float() if precision == "fp32" else line_pitched(task.x.to(torch.bfloat16))
     dev = x.device
     model = Net(x.shape[1], task.model.conv1.out_channels, task.model.conv2.out_channels, precision,
                 task.model.p, order).to(dev)
-    return GCNTask(model, _adam(model, capturable), x, task.graph, task.y, task.train_mask, precision, task.nnz,
-                   task.train_idx, task.y_train)
+    return GCNTask(model, _adam(model, capturable, optimizer), x, task.graph, task.y, task.train_mask, precision,
+                   task.nnz, task.train_idx, task.y_train)
 
 
 def _step(task: GCNTask) -> torch.Tensor:
@@ -147,14 +158,15 @@ class GraphedStep:
     """The training step captured once in a HIP graph and replayed: one launch
     per epoch instead of one per kernel, for graphs small enough that an epoch
     is launch-bound (PubMed size).  The task's optimizer must be capturable
-    (make_task(..., capturable=True)).  Warm-up steps run eagerly on a side
+    (the native Adam is; torch's needs make_task(..., capturable=True)).  Warm-up steps run eagerly on a side
     stream (they fill the GCN layers' caches and the library's workspace
     high-water mark), the library's capture arena is reserved, then one step
     is captured; replay() runs it again, loss() reads its last loss."""
 
     def __init__(self, task: GCNTask, warmup: int = 3):
         if not task.optimizer.defaults.get("capturable"):
-            raise ValueError("GraphedStep needs a capturable optimizer (make_task(..., capturable=True))")
+            raise ValueError("GraphedStep needs a capturable optimizer (the native Adam, or "
+                             "make_task(..., capturable=True))")
         self.task = task
         dev = task.x.device
         side = torch.cuda.Stream(device=dev)

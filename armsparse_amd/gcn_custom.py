@@ -3,7 +3,7 @@
     python -m armsparse_amd.gcn_custom --n-hidden=64 [--precision bf16] [--graphed]
 
 Same network (GCNConv → ReLU → dropout → GCNConv → log_softmax, gcn_custom.py:72-90),
-optimizer (Adam, lr 0.01, weight decay 5e-4 on the first layer only, :104-108),
+optimizer (Adam, lr 0.01, weight decay 5e-4 on the first layer only, :116-119),
 200-epoch loop of one training step and one evaluation per epoch with the
 reference's per-epoch line (:133-152).  Planetoid cannot be downloaded here, so the
 graph is the seeded PubMed-shaped power-law graph (19,717 nodes, 88,648 directed
@@ -39,10 +39,12 @@ def parse(argv=None):
     p.add_argument("--features", type=int, default=500)
     p.add_argument("--classes", type=int, default=3)
     p.add_argument("--seed", type=int, default=data.CFG_SEED + 3)
+    p.add_argument("--optimizer", choices=("native", "torch"), default="native",
+                   help="Adam on the library's kernels (native) or torch.optim.Adam (torch)")
     return p.parse_args(argv)
 
 
-def _mtx_task(path, precision, hidden, features, classes, seed, capturable, order):
+def _mtx_task(path, precision, hidden, features, classes, seed, capturable, order, optimizer):
     """GCNTask on a Matrix Market graph: symmetric adjacency (mirrored, stored entries
     only, weights dropped as gcn_custom.py's all-ones COO does) plus self loops."""
     a = data.read_matrix_market(path)
@@ -62,7 +64,7 @@ def _mtx_task(path, precision, hidden, features, classes, seed, capturable, orde
                       None, None, None)
     g.colptr, g.rowind, g.value_csc = g.rowptr.copy(), g.colind.copy(), g.value_csr.copy()
     return gcn.make_task(precision, seed=seed, n=n, features=features, hidden=hidden, classes=classes,
-                         capturable=capturable, order=order, graph=g)
+                         capturable=capturable, order=order, graph=g, optimizer=optimizer)
 
 
 def splits(n, classes, seed):
@@ -82,10 +84,10 @@ def main(argv=None) -> int:
     capturable = args.graphed
     if args.mtx:
         task = _mtx_task(args.mtx, args.precision, args.n_hidden, args.features, args.classes, args.seed, capturable,
-                         args.order)
+                         args.order, args.optimizer)
     else:
         task = gcn.make_task(args.precision, seed=args.seed, hidden=args.n_hidden, features=args.features,
-                             classes=args.classes, capturable=capturable, order=args.order)
+                             classes=args.classes, capturable=capturable, order=args.order, optimizer=args.optimizer)
     n = task.x.shape[0]
     tr, va, te = splits(n, args.classes, args.seed)
     dev = task.x.device

@@ -16,7 +16,7 @@ LIB = os.path.join(ROOT, "armsparse_amd", "libarmsparse.so")
 
 def _declared_c_functions():
     names = set()
-    for h in ("armsp.h", "armsp_gen.h"):
+    for h in ("armsp.h", "armsp_gen.h", "armsp_optim.h"):
         src = open(os.path.join(INCLUDE, h)).read()
         src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
         names |= set(re.findall(r"\b(armsp_\w+)\s*\(", src))
@@ -128,6 +128,40 @@ def test_selector_and_probe_arguments_checked_on_the_host():
     assert lib.armsp_probe_gather_f32(8, d, d, 8, 128, 128, 32, 3, d, None) == 1   # vec
     assert lib.armsp_probe_gather_f32(8, d, d, 8, 96, 96, 16, 2, d, None) == 2     # 3 slices: not a divisor of 8
     assert lib.armsp_probe_gather_f32(0, d, d, 8, 128, 128, 32, 2, d, None) == 0   # nothing to gather
+
+
+def test_adam_arguments_checked_on_the_host():
+    """armsp_adam_f32 rejects bad tensor lists and hyper-parameters before any
+    device call; an empty list is a no-op."""
+    import ctypes
+
+    from armsparse_amd import _lib
+
+    lib = _lib.lib
+    d = lambda k: ctypes.c_void_p(256 * (k + 1))  # noqa: E731  (never dereferenced)
+
+    def tensors(*specs):
+        arr = (_lib.AdamTensor * max(1, len(specs)))()
+        for i, (n, step, wd) in enumerate(specs):
+            arr[i] = _lib.AdamTensor(d(4 * i), d(4 * i + 1), d(4 * i + 2), d(4 * i + 3), step, n, wd)
+        return arr
+
+    ok = (0.01, 0.9, 0.999, 1e-8)
+    assert lib.armsp_adam_f32(tensors(), 0, *ok, None) == 0                                  # nothing to update
+    assert lib.armsp_adam_f32(None, 1, *ok, None) == 1
+    assert lib.armsp_adam_f32(tensors(), -1, *ok, None) == 1
+    many = tensors(*[(8, d(100 + i), 0.0) for i in range(_lib.ADAM_MAX_TENSORS + 1)])
+    assert lib.armsp_adam_f32(many, _lib.ADAM_MAX_TENSORS + 1, *ok, None) == 1             # over the limit
+    assert lib.armsp_adam_f32(tensors((-1, d(90), 0.0)), 1, *ok, None) == 1                 # negative size
+    assert lib.armsp_adam_f32(tensors((8, None, 0.0)), 1, *ok, None) == 1                   # no counter
+    assert lib.armsp_adam_f32(tensors((8, d(90), -1e-4)), 1, *ok, None) == 1                # negative decay
+    assert lib.armsp_adam_f32(tensors((8, d(90), 0.0), (8, d(90), 0.0)), 2, *ok, None) == 1  # shared counter
+    one = tensors((8, d(90), 0.0))
+    one[0].grad = None
+    assert lib.armsp_adam_f32(one, 1, *ok, None) == 1                                       # no gradient
+    for bad in ((-0.01, 0.9, 0.999, 1e-8), (0.01, 1.0, 0.999, 1e-8), (0.01, 0.9, -0.1, 1e-8),
+                (0.01, 0.9, 0.999, -1e-8), (float("nan"), 0.9, 0.999, 1e-8)):
+        assert lib.armsp_adam_f32(tensors((8, d(90), 0.0)), 1, *bad, None) == 1, bad
 
 
 def test_no_gpu_fails_loudly():
