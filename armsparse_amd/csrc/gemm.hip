@@ -682,16 +682,47 @@ __global__ __launch_bounds__(64 * RW) void k_gemm_reduce(int M, int N, int K, in
     }
 }
 
-__global__ void k_splitk_reduce(int M, int N, int splits, const float* __restrict__ part, float* __restrict__ C,
-                                int64_t ldc, int accumulate) {
-    const int64_t idx = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    if (idx >= (int64_t)M * N) return;
-    const int m = (int)(idx / N), n = (int)(idx % N);
-    float s = 0.0f;
-#pragma unroll 8
-    for (int z = 0; z < splits; ++z) s += part[(int64_t)z * M * N + idx];  // loads run ahead of the in-order adds
-    float* p = C + (int64_t)m * ldc + n;
-    *p = accumulate ? *p + s : s;
+// C[m, n] (=|+=) Σ_z part[z][m·N + n].  A workgroup takes OPB consecutive
+// outputs; 256/OPB lanes share each output, lane phase q summing partials
+// q, q + 256/OPB, ... in order, and the phases' sums are then added in phase
+// order (deterministic for given splits and OPB).  A lane alone per output
+// would chain all `splits` loads: 62–512 of them in the GCN's products.
+template <int OPB>
+__global__ __launch_bounds__(256) void k_splitk_reduce(int M, int N, int splits, const float* __restrict__ part,
+                                                       float* __restrict__ C, int64_t ldc, int accumulate) {
+    constexpr int PH = 256 / OPB;
+    __shared__ float sum[PH][OPB];
+    const int o = threadIdx.x % OPB, q = threadIdx.x / OPB;
+    const int64_t MN = (int64_t)M * N;
+    const int64_t idx = blockIdx.x * (int64_t)OPB + o;
+    float acc = 0.0f;
+    if (idx < MN) {
+#pragma unroll 4
+        for (int z = q; z < splits; z += PH) acc += part[(int64_t)z * MN + idx];
+    }
+    sum[q][o] = acc;
+    __syncthreads();
+    if (q == 0 && idx < MN) {
+        float t = sum[0][o];
+#pragma unroll
+        for (int r = 1; r < PH; ++r) t += sum[r][o];
+        float* p = C + (idx / N) * ldc + idx % N;
+        *p = accumulate ? *p + t : t;
+    }
+}
+
+// the split-K sum of `splits` partial [M×N] products into C: 16 outputs per
+// workgroup (16 lanes each) until the outputs alone fill 2 workgroups per CU,
+// then 64 (4 lanes each)
+void splitk_reduce(int M, int N, int splits, const float* part, float* C, int64_t ldc, int accumulate,
+                   hipStream_t s) {
+    const int64_t n = (int64_t)M * N;
+    if (n / 64 >= 2 * device_cus())
+        hipLaunchKernelGGL(k_splitk_reduce<64>, dim3((unsigned)((n + 63) / 64)), dim3(256), 0, s, M, N, splits, part,
+                           C, ldc, accumulate);
+    else
+        hipLaunchKernelGGL(k_splitk_reduce<16>, dim3((unsigned)((n + 15) / 16)), dim3(256), 0, s, M, N, splits, part,
+                           C, ldc, accumulate);
 }
 
 }  // namespace
@@ -861,9 +892,7 @@ int gemm_f32_out(int M, int N, int Kd, const uint16_t* A, int64_t sam, int64_t s
 #undef ARMSP_RED
             hipError_t e = hipGetLastError();
             if (e == hipSuccess) {
-                const int64_t n = (int64_t)M * N;
-                hipLaunchKernelGGL(k_splitk_reduce, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, M, N, nblk,
-                                   part, C, ldc, accumulate);
+                splitk_reduce(M, N, nblk, part, C, ldc, accumulate, s);
                 e = hipGetLastError();
             }
             ws_free(ws, s);
@@ -874,10 +903,12 @@ int gemm_f32_out(int M, int N, int Kd, const uint16_t* A, int64_t sam, int64_t s
     }
     const int gx = (M + BM - 1) / BM, gy = (N + BN - 1) / BN;
     if (gy > 65535) return ARMSP_ERROR_NOT_SUPPORTED;
-    // split K when the tile grid is small and K is long
+    // split K when the tile grid is small and K is long: up to 4 workgroups per CU,
+    // each at least 4 k steps (a workgroup's steps run one global-load latency
+    // apart; 19,717-deep PubMed products: 5 steps each instead of 10)
     int splits = 1;
-    const int tiles = gx * gy, target = 2 * device_cus();
-    while (tiles * splits < target && Kd / (splits * 2) >= 4 * BK && splits < 64) splits *= 2;
+    const int tiles = gx * gy, target = 4 * device_cus();
+    while (tiles * splits < target && Kd / (splits * 2) >= 4 * BK && splits < 128) splits *= 2;
     int kchunk = (Kd + splits - 1) / splits;
     kchunk = (kchunk + BK - 1) / BK * BK;
     if (kchunk == 0) kchunk = BK;
@@ -899,9 +930,7 @@ int gemm_f32_out(int M, int N, int Kd, const uint16_t* A, int64_t sam, int64_t s
                        sbn, bmode, part, (int64_t)N, (int64_t)M * N, 0);
     hipError_t e = hipGetLastError();
     if (e == hipSuccess) {
-        const int64_t n = (int64_t)M * N;
-        hipLaunchKernelGGL(k_splitk_reduce, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, M, N, splits, part, C,
-                           ldc, accumulate);
+        splitk_reduce(M, N, splits, part, C, ldc, accumulate, s);
         e = hipGetLastError();
     }
     ws_free(ws, s);
