@@ -158,10 +158,11 @@ class GraphedStep:
     """The training step captured once in a HIP graph and replayed: one launch
     per epoch instead of one per kernel, for graphs small enough that an epoch
     is launch-bound (PubMed size).  The task's optimizer must be capturable
-    (the native Adam is; torch's needs make_task(..., capturable=True)).  Warm-up steps run eagerly on a side
-    stream (they fill the GCN layers' caches and the library's workspace
-    high-water mark), the library's capture arena is reserved, then one step
-    is captured; replay() runs it again, loss() reads its last loss."""
+    (the native Adam is; torch's needs make_task(..., capturable=True)).
+    Warm-up steps run eagerly on a side stream (they fill the GCN layers'
+    caches and the library's workspace high-water mark), the library's capture
+    arena is reserved, then one step is captured; replay() runs it again,
+    loss() reads its last loss."""
 
     def __init__(self, task: GCNTask, warmup: int = 3):
         if not task.optimizer.defaults.get("capturable"):
