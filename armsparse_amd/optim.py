@@ -11,11 +11,10 @@ off; the counters stay on the device, so a step can be captured in a HIP graph.
 """
 from __future__ import annotations
 
-import ctypes
-
 import torch
 
 from . import _lib
+from .kernels import _stream
 
 
 class Adam(torch.optim.Optimizer):
@@ -34,6 +33,8 @@ class Adam(torch.optim.Optimizer):
             raise ValueError(f"Invalid weight_decay value: {weight_decay}")
         super().__init__(params, dict(lr=lr, betas=tuple(betas), eps=eps, weight_decay=weight_decay,
                                       capturable=True))
+        # (device, hyper-parameters, offset) -> (tensor pointers and sizes, ctypes argument array)
+        self._args: dict = {}
 
     def _state(self, p: torch.Tensor) -> dict:
         st = self.state[p]
@@ -67,13 +68,23 @@ class Adam(torch.optim.Optimizer):
                 st = self._state(p)
                 batches.setdefault((p.device, key), []).append(
                     (p, g, st["exp_avg"], st["exp_avg_sq"], st["step"], wd))
-        for (dev, (lr, b1, b2, eps)), items in batches.items():
-            with torch.cuda.device(dev):
-                stream = ctypes.c_void_p(torch.cuda.current_stream(dev).cuda_stream)
-                for lo in range(0, len(items), _lib.ADAM_MAX_TENSORS):
-                    part = items[lo:lo + _lib.ADAM_MAX_TENSORS]
-                    arr = (_lib.AdamTensor * len(part))(*(
-                        _lib.AdamTensor(p.data_ptr(), g.data_ptr(), m.data_ptr(), v.data_ptr(), s.data_ptr(),
-                                        p.numel(), wd) for p, g, m, v, s, wd in part))
-                    _lib.call("armsp_adam_f32", arr, len(part), lr, b1, b2, eps, stream)
+        for (dev, hyper), items in batches.items():
+            if dev.index != torch.cuda.current_device():
+                with torch.cuda.device(dev):
+                    self._launch(dev, hyper, items)
+            else:
+                self._launch(dev, hyper, items)
         return loss
+
+    def _launch(self, dev, hyper, items):
+        stream = _stream(dev)
+        for lo in range(0, len(items), _lib.ADAM_MAX_TENSORS):
+            part = items[lo:lo + _lib.ADAM_MAX_TENSORS]
+            key = tuple((p.data_ptr(), g.data_ptr(), m.data_ptr(), v.data_ptr(), s.data_ptr(), p.numel(), wd)
+                        for p, g, m, v, s, wd in part)
+            slot = (dev, hyper, lo)
+            cached = self._args.get(slot)
+            if cached is None or cached[0] != key:  # the argument array only changes with the tensors
+                arr = (_lib.AdamTensor * len(part))(*(_lib.AdamTensor(*k) for k in key))
+                cached = self._args[slot] = (key, arr)
+            _lib.call("armsp_adam_f32", cached[1], len(part), *hyper, stream)

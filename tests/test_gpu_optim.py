@@ -129,6 +129,25 @@ def test_adam_over_more_tensors_than_one_call_takes():
         _close(a, oa, b, ob, t)
 
 
+def test_adam_after_load_state_dict():
+    """load_state_dict replaces the moment and counter tensors: the next steps must
+    update the new ones (the cached argument array is rebuilt), as torch's do."""
+    print("[case] adam after load_state_dict", flush=True)
+    a, oa, b, ob = _twins(51)
+    for t in (1, 2):
+        _set_grads(a, b, 600 + t)
+        oa.step()
+        ob.step()
+    oa.load_state_dict(oa.state_dict())
+    ob.load_state_dict(ob.state_dict())
+    for t in (3, 4):
+        _set_grads(a, b, 600 + t)
+        oa.step()
+        ob.step()
+        torch.cuda.synchronize()
+        _close(a, oa, b, ob, t)
+
+
 def test_adam_step_in_a_graph():
     """optim.Adam.step() captured in a HIP graph: every replay (new gradients
     copied in, eager work and a synchronisation in between) equals an eager step
