@@ -252,6 +252,69 @@ class _AggregateBiasReluDropout(torch.autograd.Function):
         return dh, None, None, None, None, dcoef, None, None, None, dbias, None, None
 
 
+class _Sub:
+    """Context of one of two Functions run as a single autograd node: it holds what
+    the inner forward saves and the input-gradient flags its backward reads."""
+
+    def __init__(self, needs_input_grad):
+        self.needs_input_grad = needs_input_grad
+        self.saved_tensors = ()
+
+    def save_for_backward(self, *tensors):
+        self.saved_tensors = tensors
+
+
+class _LinearAggregate(torch.autograd.Function):
+    """GCNAggregate(x·W) as one autograd node (the fused bf16 layer in the
+    reference's order): _BF16Linear's forward, then GCNAggregate's; the backward
+    runs theirs in reverse.  One node instead of two saves the host time of a
+    Python backward call per layer in an eager step."""
+
+    @staticmethod
+    def forward(ctx, x, weight, out_bf16, rowptr, colind, colptr, rowind, coef_fwd, coef_bwd, in_norm, out_norm,
+                bias, blocks):
+        n = ctx.needs_input_grad
+        ctx.lin = _Sub((n[0], n[1], False))
+        h = _BF16Linear.forward(ctx.lin, x, weight, out_bf16)
+        h = h if out_bf16 else h.contiguous()
+        ctx.agg = _Sub((n[0] or n[1], False, False, False, False, n[7], False, False, False, n[11], False))
+        return GCNAggregate.forward(ctx.agg, h, rowptr, colind, colptr, rowind, coef_fwd, coef_bwd, in_norm,
+                                    out_norm, bias, blocks)
+
+    @staticmethod
+    def backward(ctx, gy):
+        dh, _, _, _, _, dcoef, _, _, _, dbias, _ = GCNAggregate.backward(ctx.agg, gy)
+        gx = gw = None
+        if dh is not None:
+            gx, gw, _ = _BF16Linear.backward(ctx.lin, dh)
+        return gx, gw, None, None, None, None, None, dcoef, None, None, None, dbias, None
+
+
+class _LinearAggregateAct(torch.autograd.Function):
+    """_AggregateBiasReluDropout(x·W) as one autograd node (the fused hidden layer
+    in the reference's order): _BF16Linear's forward with bf16 output, then the
+    aggregation with its bias, ReLU, dropout and cast; the backward runs theirs in
+    reverse."""
+
+    @staticmethod
+    def forward(ctx, x, weight, rowptr, colind, colptr, rowind, coef_fwd, coef_bwd, in_norm, out_norm, bias, p,
+                seed):
+        n = ctx.needs_input_grad
+        ctx.lin = _Sub((n[0], n[1], False))
+        h = _BF16Linear.forward(ctx.lin, x, weight, True)
+        ctx.agg = _Sub((n[0] or n[1], False, False, False, False, n[6], False, False, False, n[10], False, False))
+        return _AggregateBiasReluDropout.forward(ctx.agg, h, rowptr, colind, colptr, rowind, coef_fwd, coef_bwd,
+                                                 in_norm, out_norm, bias, p, seed)
+
+    @staticmethod
+    def backward(ctx, gout):
+        dh, _, _, _, _, dcoef, _, _, _, dbias, _, _ = _AggregateBiasReluDropout.backward(ctx.agg, gout)
+        gx = gw = None
+        if dh is not None:
+            gx, gw, _ = _BF16Linear.backward(ctx.lin, dh)
+        return gx, gw, None, None, None, None, dcoef, None, None, None, dbias, None, None
+
+
 class GCNConv(torch.nn.Module):
     """GCN layer of gcn/pytorch-custom/op.py:78-153: x·W, scale by
     1/sqrt(out-degree) (counts from colptr), aggregate with SPMMFunction,
@@ -322,10 +385,9 @@ class GCNConv(torch.nn.Module):
             if getattr(self, "_seed", None) is None or self._seed.device != x.device:
                 self._seed = torch.randint(0, 2**62, (1,), dtype=torch.int64).to(x.device)
             seed = self._seed
-        if not self.aggregates_first():
-            h = _BF16Linear.apply(_rows(x), self.weight, True)  # bf16 features for the gathers
-            return _AggregateBiasReluDropout.apply(h, rowptr, colind, colptr, rowind, coefs[0], coefs[1], in_n,
-                                                   out_n, self.bias, float(p), seed)
+        if not self.aggregates_first():  # x·W to bf16 features for the gathers, then the aggregation
+            return _LinearAggregateAct.apply(_rows(x), self.weight, rowptr, colind, colptr, rowind, coefs[0],
+                                             coefs[1], in_n, out_n, self.bias, float(p), seed)
         h = x
         if h.dtype != torch.bfloat16:
             h = h.to(torch.bfloat16) if h.requires_grad else kernels.to_bf16(h.contiguous())
@@ -358,18 +420,25 @@ class GCNConv(torch.nn.Module):
         """Column blocks of the graph for an aggregation of narrow features (at most 4
         per node) whose rows outgrow the L2, cached with the coefficients (cached=True
         and coefficients that need no gradient); None: one product per aggregation."""
-        if not (self.cached and h.dim() == 2 and h.shape[1] <= 4 and h.is_cuda):
+        if h.dim() != 2:
+            return None
+        return self._column_blocks_for(h.shape[0], h.shape[1], h.element_size(), h.is_cuda, rowptr, colind,
+                                       colptr, rowind, coefs)
+
+    def _column_blocks_for(self, rows, cols, esize, is_cuda, rowptr, colind, colptr, rowind, coefs):
+        """_column_blocks for features of rows × cols elements of esize bytes."""
+        if not (self.cached and cols <= 4 and is_cuda):
             return None
         if coefs[0].requires_grad or coefs[1].requires_grad:
             return None
-        row_bytes = h.shape[1] * h.element_size()
-        nb = kernels.gather_blocks(h.shape[0], row_bytes)
+        row_bytes = cols * esize
+        nb = kernels.gather_blocks(rows, row_bytes)
         if nb < 2:
             return None
         key = (rowptr.data_ptr(), colind.data_ptr(), colptr.data_ptr(), rowind.data_ptr(), nb)
         if self.cached_blocks is None or self.cached_blocks[0] != key:
             m = rowptr.numel() - 1
-            fwd = kernels.column_blocks(rowptr, colind, coefs[0], h.shape[0], nb)
+            fwd = kernels.column_blocks(rowptr, colind, coefs[0], rows, nb)
             bwd = kernels.column_blocks(colptr, rowind, coefs[1], m, kernels.gather_blocks(m, row_bytes))
             self.cached_blocks = (key, (fwd, bwd))
         return self.cached_blocks[1]
@@ -395,6 +464,13 @@ class GCNConv(torch.nn.Module):
             # bf16 features for the aggregation straight from the GEMM (fused path with
             # more than 8 output channels); fp32 otherwise
             out_bf16 = self.fused and self.out_channels > 8
+            if self.fused:  # x·W and the aggregation as one autograd node
+                in_n, out_n, coefs = self._norms_and_coefs(rowptr, colind, colptr, rowind, edge_weight_csr,
+                                                           edge_weight_csc)
+                blocks = self._column_blocks_for(x.shape[0], self.out_channels, 2 if out_bf16 else 4, x.is_cuda,
+                                                 rowptr, colind, colptr, rowind, coefs)
+                return _LinearAggregate.apply(_rows(x), self.weight, out_bf16, rowptr, colind, colptr, rowind,
+                                              coefs[0], coefs[1], in_n, out_n, self.bias, blocks)
             x = _BF16Linear.apply(_rows(x), self.weight, out_bf16)
         else:
             x = torch.matmul(x, self.weight)
