@@ -44,6 +44,8 @@ def _close(a, oa, b, ob, steps):
     for pa, pb in zip(a, b):
         sa, sb = oa.state[pa], ob.state[pb]
         assert float(sa["step"]) == float(sb["step"]) == steps
+        if pa.numel() == 0:
+            continue
         # the first moment sums gradients of either sign (and here of magnitudes 0.1 to
         # 10): where they cancel, the two roundings differ by ulps of the terms, not of
         # the sum, so the bound scales with the tensor's largest moment
@@ -186,6 +188,20 @@ def test_adam_step_in_a_graph():
             assert float(oa.state[pa]["step"]) == float(ob.state[pb]["step"]) == r + 2
             assert torch.equal(pa.detach(), pb.detach()), r
             assert torch.equal(oa.state[pa]["exp_avg_sq"], ob.state[pb]["exp_avg_sq"]), r
+
+
+def test_adam_with_empty_tensors():
+    """Zero-element parameters advance their counters (as torch's do) and launch
+    no update workgroups; the tensors around them are updated as usual."""
+    print("[case] adam with empty tensors", flush=True)
+    shapes = [(0,), (300,), (0, 5), (7,)]
+    a, oa, b, ob = _twins(61, shapes=shapes)
+    for t in (1, 2):
+        _set_grads(a, b, 700 + t)
+        oa.step()
+        ob.step()
+        torch.cuda.synchronize()
+        _close(a, oa, b, ob, t)
 
 
 def test_adam_rejects_what_it_cannot_update():

@@ -57,3 +57,23 @@ def test_row_pitch_rule():
         assert p >= n
         if n * esz >= 128:
             assert (p * esz) % 128 == 0 and ((p * esz) // 128 == 1 or (p * esz) // 128 % 2 == 1)
+
+
+def test_adam_refuses_host_tensors_and_bad_hyper_parameters():
+    """The library's Adam has no CPU path: a step over host tensors raises before any
+    library call; hyper-parameters outside torch.optim.Adam's ranges are rejected."""
+    import torch
+
+    from armsparse_amd import optim
+
+    p = torch.nn.Parameter(torch.zeros(4))
+    p.grad = torch.ones(4)
+    with pytest.raises(RuntimeError, match="no CPU fallback"):
+        optim.Adam([p]).step()
+    for kw in (dict(lr=-1.0), dict(eps=-1e-8), dict(betas=(0.9, 1.0)), dict(betas=(-0.1, 0.9)),
+               dict(weight_decay=-1e-4)):
+        with pytest.raises(ValueError):
+            optim.Adam([p], **kw)
+    q = torch.nn.Parameter(torch.zeros(3))  # no gradient: skipped, nothing to do
+    optim.Adam([q]).step()
+    assert optim.Adam([p]).defaults["capturable"]
